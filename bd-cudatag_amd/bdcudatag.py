@@ -1,0 +1,288 @@
+"""ctypes binding of libbdcudatag.so (the HIP detector), shaped like the
+reference's Python caller (detector/bench/compare_detectors.py:87-206): the
+same structures and the same create / detect_host / get_stats / destroy calls,
+plus the version 3 batch, device-pointer and introspection entry points.
+
+Loading fails loudly when the library is missing: there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libbdcudatag.so")
+
+BD_OK = 0
+BD_ERROR_INVALID_ARGUMENT = -1
+BD_ERROR_CUDA = -2
+BD_ERROR_OUT_OF_MEMORY = -3
+
+STAGES = {
+    "tile_min": 0, "tile_max": 1, "tile_mean": 2, "hist": 3, "scalars": 4, "row_roi": 5,
+    "bits": 6, "row_count": 7, "runs": 8, "labels": 9, "area": 10, "bbox": 11,
+    "candidates": 12, "quads": 13, "hull_quads": 14, "decode": 15, "tile_thresh": 16,
+}
+KERNELS = ["tile_stats", "frame_ref", "binarize_runs", "merge", "stats", "collect", "select",
+           "fit", "decode", "finalize"]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32), ("max_hamming", ctypes.c_int32),
+        ("min_cluster_pixels", ctypes.c_int32), ("min_tag_pixels", ctypes.c_int32),
+        ("max_detections", ctypes.c_int32), ("max_candidates", ctypes.c_int32),
+        ("refine_edges", ctypes.c_int32), ("use_cuda_graph", ctypes.c_int32),
+        ("min_decision_margin", ctypes.c_float), ("decode_sharpening", ctypes.c_float),
+    ]
+
+
+class Detection(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_int32), ("hamming", ctypes.c_int32),
+        ("decision_margin", ctypes.c_float), ("corners", ctypes.c_float * 8),
+        ("reserved", ctypes.c_int32), ("homography", ctypes.c_float * 9),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("frames", ctypes.c_uint64), ("last_gpu_ms", ctypes.c_float),
+        ("average_gpu_ms", ctypes.c_float), ("last_total_ms", ctypes.c_float),
+        ("runs", ctypes.c_int32), ("candidates", ctypes.c_int32),
+        ("detections", ctypes.c_int32), ("quads", ctypes.c_int32), ("roi_rows", ctypes.c_int32),
+    ]
+
+
+class Frame(ctypes.Structure):
+    _fields_ = [
+        ("gray", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+        ("stride", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    ]
+
+
+# Every symbol the public header declares, with (restype, argtypes).
+_P = ctypes.c_void_p
+_I = ctypes.c_int32
+SIGNATURES = {
+    "bd_api_version": (ctypes.c_uint32, []),
+    "bd_version_string": (ctypes.c_char_p, []),
+    "bd_family_count": (_I, []),
+    "bd_family_name": (ctypes.c_char_p, [_I]),
+    "bd_family_bits_per_side": (_I, [_I]),
+    "bd_family_min_hamming": (_I, [_I]),
+    "bd_family_default_max_hamming": (_I, [_I]),
+    "bd_config_init": (None, [ctypes.POINTER(Config)]),
+    "bd_detector_create": (_P, [_I, _I, _I, ctypes.POINTER(Config), ctypes.POINTER(_I)]),
+    "bd_detector_destroy": (None, [_P]),
+    "bd_detector_detect_host": (_I, [_P, _P, _I, ctypes.POINTER(Detection), _I]),
+    "bd_detector_detect_device": (_I, [_P, _P, _I, ctypes.POINTER(Detection), _I]),
+    "bd_detector_get_stats": (None, [_P, ctypes.POINTER(Stats)]),
+    "bd_last_error": (ctypes.c_char_p, []),
+    "bd_batch_create": (_P, [_I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(Config),
+                             ctypes.POINTER(_I)]),
+    "bd_batch_destroy": (None, [_P]),
+    "bd_batch_detect_host": (_I, [_P, ctypes.POINTER(Frame), _I, ctypes.POINTER(Detection), _I,
+                                  ctypes.POINTER(_I)]),
+    "bd_batch_detect_device": (_I, [_P, ctypes.POINTER(Frame), _I, ctypes.POINTER(Detection), _I,
+                                    ctypes.POINTER(_I)]),
+    "bd_batch_get_stats": (None, [_P, _I, ctypes.POINTER(Stats)]),
+    "bd_batch_debug_copy": (ctypes.c_int64, [_P, _I, _I, _P, ctypes.c_int64]),
+    "bd_batch_frame_info": (_I, [_P, _I, ctypes.POINTER(_I)]),
+    "bd_detector_batch": (_P, [_P]),
+    "bd_batch_time_stages": (_I, [_P, _I, ctypes.POINTER(ctypes.c_float)]),
+    "bd_batch_replay": (_I, [_P, _I, ctypes.POINTER(ctypes.c_float)]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built: the HIP detector is required (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    msg = lib().bd_last_error()
+    return msg.decode() if msg else ""
+
+
+def default_config(**overrides) -> Config:
+    c = Config()
+    lib().bd_config_init(ctypes.byref(c))
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c
+
+
+def detection_to_dict(d: Detection) -> dict:
+    return {
+        "id": int(d.id), "hamming": int(d.hamming), "decision_margin": float(d.decision_margin),
+        "corners": np.array(d.corners[:], dtype=np.float32).reshape(4, 2),
+        "homography": np.array(d.homography[:], dtype=np.float32),
+    }
+
+
+def _stats_dict(s: Stats) -> dict:
+    return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+
+class Batch:
+    """Several frames of fixed (possibly different) sizes per call."""
+
+    def __init__(self, family: int, sizes, handle=None, owner=None, **config):
+        self._owner = owner
+        if handle is not None:
+            self.handle = handle
+            self._owned = False
+        else:
+            sizes = list(sizes)
+            w = (ctypes.c_int32 * len(sizes))(*[s[0] for s in sizes])
+            h = (ctypes.c_int32 * len(sizes))(*[s[1] for s in sizes])
+            status = ctypes.c_int32(0)
+            cfg = default_config(**config)
+            self.handle = lib().bd_batch_create(family, len(sizes), w, h, ctypes.byref(cfg),
+                                                ctypes.byref(status))
+            if not self.handle:
+                raise RuntimeError(f"bd_batch_create failed ({status.value}): {last_error()}")
+            self._owned = True
+        info = (ctypes.c_int32 * 8)()
+        self.frames = []
+        i = 0
+        while lib().bd_batch_frame_info(self.handle, i, info) == BD_OK:
+            self.frames.append({
+                "width": info[0], "height": info[1], "tile_w": info[2], "tile_h": info[3],
+                "words64": info[4], "row_cap": info[5], "max_candidates": info[6],
+                "max_detections": info[7]})
+            i += 1
+        self.max_detections = self.frames[0]["max_detections"]
+
+    def close(self):
+        if getattr(self, "_owned", False) and self.handle:
+            lib().bd_batch_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def _run(self, frames, device: bool):
+        n = len(self.frames)
+        arr = (Frame * n)()
+        keep = []
+        for i, f in enumerate(frames):
+            if device:
+                ptr, stride = f
+            else:
+                img = np.ascontiguousarray(f, dtype=np.uint8)
+                keep.append(img)
+                ptr, stride = img.ctypes.data, img.strides[0]
+            arr[i].gray = ptr
+            arr[i].width = self.frames[i]["width"]
+            arr[i].height = self.frames[i]["height"]
+            arr[i].stride = stride
+        md = self.max_detections
+        out = (Detection * (n * md))()
+        counts = (ctypes.c_int32 * n)()
+        fn = lib().bd_batch_detect_device if device else lib().bd_batch_detect_host
+        rc = fn(self.handle, arr, n, out, md, counts)
+        if rc < 0:
+            raise RuntimeError(f"batch detect failed ({rc}): {last_error()}")
+        return [[detection_to_dict(out[i * md + k]) for k in range(min(counts[i], md))]
+                for i in range(n)]
+
+    def detect(self, images):
+        return self._run(images, False)
+
+    def detect_device(self, ptr_strides):
+        return self._run(ptr_strides, True)
+
+    def stats(self, index=0) -> dict:
+        s = Stats()
+        lib().bd_batch_get_stats(self.handle, index, ctypes.byref(s))
+        return _stats_dict(s)
+
+    def debug(self, stage: str, index: int = 0) -> np.ndarray:
+        """Raw copy of one stage buffer of frame `index` of the last call."""
+        code = STAGES[stage]
+        n = lib().bd_batch_debug_copy(self.handle, index, code, None, 0)
+        if n < 0:
+            raise RuntimeError(f"bd_batch_debug_copy failed: {last_error()}")
+        buf = np.empty(int(n), dtype=np.uint8)
+        rc = lib().bd_batch_debug_copy(self.handle, index, code, buf.ctypes.data, int(n))
+        if rc < 0:
+            raise RuntimeError(f"bd_batch_debug_copy failed: {last_error()}")
+        return buf
+
+    def time_stages(self, reps: int = 10) -> dict:
+        ms = (ctypes.c_float * 11)()
+        rc = lib().bd_batch_time_stages(self.handle, reps, ms)
+        if rc != BD_OK:
+            raise RuntimeError(f"bd_batch_time_stages failed: {last_error()}")
+        out = {k: float(ms[i]) for i, k in enumerate(KERNELS)}
+        out["pipeline"] = float(ms[10])
+        return out
+
+    def replay(self, reps: int) -> float:
+        ms = ctypes.c_float()
+        rc = lib().bd_batch_replay(self.handle, reps, ctypes.byref(ms))
+        if rc != BD_OK:
+            raise RuntimeError(f"bd_batch_replay failed: {last_error()}")
+        return float(ms.value)
+
+
+class Detector:
+    """One family, one frame size (the reference's bd_detector)."""
+
+    def __init__(self, family: int, width: int, height: int, **config):
+        status = ctypes.c_int32(0)
+        cfg = default_config(**config)
+        self.width, self.height = width, height
+        self.handle = lib().bd_detector_create(family, width, height, ctypes.byref(cfg),
+                                               ctypes.byref(status))
+        if not self.handle:
+            raise RuntimeError(f"bd_detector_create failed ({status.value}): {last_error()}")
+        self.status = status.value
+        self.max_detections = max(1, min(cfg.max_detections, 256))
+        self.out = (Detection * self.max_detections)()
+        self.batch = Batch(family, None, handle=lib().bd_detector_batch(self.handle), owner=self)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().bd_detector_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def detect(self, image: np.ndarray, stride: int | None = None):
+        img = np.ascontiguousarray(image, dtype=np.uint8) if stride is None else image
+        if stride is None:
+            stride = img.strides[0]
+        rc = lib().bd_detector_detect_host(self.handle, img.ctypes.data, stride, self.out,
+                                           self.max_detections)
+        if rc < 0:
+            raise RuntimeError(f"bd_detector_detect_host failed ({rc}): {last_error()}")
+        return [detection_to_dict(self.out[i]) for i in range(min(rc, self.max_detections))]
+
+    def detect_device(self, ptr: int, stride: int):
+        rc = lib().bd_detector_detect_device(self.handle, ptr, stride, self.out, self.max_detections)
+        if rc < 0:
+            raise RuntimeError(f"bd_detector_detect_device failed ({rc}): {last_error()}")
+        return [detection_to_dict(self.out[i]) for i in range(min(rc, self.max_detections))]
+
+    def stats(self) -> dict:
+        s = Stats()
+        lib().bd_detector_get_stats(self.handle, ctypes.byref(s))
+        return _stats_dict(s)

@@ -1,0 +1,495 @@
+// C ABI and C++ wrapper (reference detector/src/detector.cpp:1-382): the
+// same validation order, status codes, thread-local last error and config
+// defaults, on top of the HIP engine.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "bd_engine.h"
+#include "bdcudatag/bdcudatag.h"
+#include "bdcudatag/detector.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string& message) { g_last_error = message; }
+
+bool valid_family(int family) { return family == 0 || family == 1; }
+
+bool valid_dimensions(int width, int height) {
+    return width >= 16 && width <= BD_MAX_WIDTH && height >= 16 && height <= BD_MAX_HEIGHT;
+}
+
+// detector.cpp:39-57 (config_to_gpu)
+bd::Params to_params(int family, const bd_config* config) {
+    bd_config defaults;
+    bd_config_init(&defaults);
+    const bd_config* c = config ? config : &defaults;
+    bd::Params p;
+    p.family = family;
+    p.max_hamming = c->max_hamming;
+    p.min_cluster_pixels = c->min_cluster_pixels;
+    p.min_tag_pixels = c->min_tag_pixels;
+    p.max_detections = c->max_detections;
+    p.max_candidates = c->max_candidates;
+    p.refine_edges = c->refine_edges != 0;
+    p.use_graph = c->use_cuda_graph != 0;
+    p.min_decision_margin = c->min_decision_margin;
+    p.decode_sharpening = c->decode_sharpening;
+    return p;
+}
+
+bool check_output(const bd_detection* out, int32_t capacity) {
+    if (capacity < 0 || (capacity > 0 && !out)) {
+        set_error("invalid output buffer");
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+struct bd_batch {
+    bd::Engine* engine = nullptr;
+};
+
+struct bd_detector {
+    bd_batch batch;
+    int family = 0;
+    int width = 0;
+    int height = 0;
+};
+
+extern "C" {
+
+uint32_t bd_api_version(void) { return BD_API_VERSION; }
+
+const char* bd_version_string(void) { return "bdcudatag 0.1.0 (hip gfx950)"; }
+
+int32_t bd_family_count(void) { return 2; }
+
+const char* bd_family_name(int32_t family) {
+    if (!valid_family(family)) return nullptr;
+    return family == 0 ? "tag36h11" : "tag16h5";
+}
+
+int32_t bd_family_bits_per_side(int32_t family) {
+    if (!valid_family(family)) return -1;
+    return family == 0 ? 6 : 4;
+}
+
+int32_t bd_family_min_hamming(int32_t family) {
+    if (!valid_family(family)) return -1;
+    return family == 0 ? 11 : 5;
+}
+
+int32_t bd_family_default_max_hamming(int32_t family) {
+    if (!valid_family(family)) return -1;
+    return family == 0 ? 2 : 1;
+}
+
+void bd_config_init(bd_config* config) {
+    if (!config) return;
+    std::memset(config, 0, sizeof(*config));
+    config->struct_size = sizeof(bd_config);
+    config->max_hamming = -1;
+    config->min_cluster_pixels = 24;
+    config->min_tag_pixels = 8;
+    config->max_detections = 64;
+    config->max_candidates = 0;
+    config->refine_edges = 1;
+    config->use_cuda_graph = 1;
+    config->min_decision_margin = 3.0f;
+    config->decode_sharpening = 0.25f;
+}
+
+// detector.cpp:112-156
+bd_detector* bd_detector_create(int32_t family, int32_t width, int32_t height,
+                                const bd_config* config, int32_t* status_out) {
+    if (status_out) *status_out = BD_OK;
+    if (!valid_family(family)) {
+        set_error("unknown tag family");
+        if (status_out) *status_out = BD_ERROR_INVALID_ARGUMENT;
+        return nullptr;
+    }
+    if (!valid_dimensions(width, height)) {
+        set_error("image dimensions out of range (16..8192 x 16..16384)");
+        if (status_out) *status_out = BD_ERROR_INVALID_ARGUMENT;
+        return nullptr;
+    }
+    if (config && config->struct_size != 0 && config->struct_size < sizeof(bd_config)) {
+        set_error("bd_config.struct_size is too small for this library version");
+        if (status_out) *status_out = BD_ERROR_INVALID_ARGUMENT;
+        return nullptr;
+    }
+    std::string error;
+    bd::Engine* engine = bd::Engine::create(to_params(family, config), {width}, {height}, &error);
+    if (!engine) {
+        set_error(error.empty() ? "failed to create HIP detector" : error);
+        if (status_out) *status_out = BD_ERROR_CUDA;
+        return nullptr;
+    }
+    bd_detector* detector = new (std::nothrow) bd_detector();
+    if (!detector) {
+        delete engine;
+        set_error("out of memory allocating detector handle");
+        if (status_out) *status_out = BD_ERROR_OUT_OF_MEMORY;
+        return nullptr;
+    }
+    detector->batch.engine = engine;
+    detector->family = family;
+    detector->width = width;
+    detector->height = height;
+    return detector;
+}
+
+void bd_detector_destroy(bd_detector* detector) {
+    if (!detector) return;
+    delete detector->batch.engine;
+    delete detector;
+}
+
+static int32_t detect_one(bd_detector* detector, const uint8_t* gray, int32_t stride,
+                          bd_detection* out, int32_t out_capacity, bool device) {
+    if (!detector) {
+        set_error("null detector");
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    if (!gray) {
+        set_error(device ? "null device image pointer" : "null image pointer");
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    if (stride < detector->width) {
+        set_error("stride is smaller than the image width");
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    if (!check_output(out, out_capacity)) return BD_ERROR_INVALID_ARGUMENT;
+    bd_frame frame{gray, detector->width, detector->height, stride, 0};
+    int32_t found = 0;
+    std::string error;
+    const int written = detector->batch.engine->detect(&frame, device, out, out_capacity, &found, &error);
+    if (written < 0) {
+        set_error(error.empty() ? "HIP detection failed" : error);
+        return BD_ERROR_CUDA;
+    }
+    return found;
+}
+
+// detector.cpp:164-190
+int32_t bd_detector_detect_host(bd_detector* detector, const uint8_t* gray, int32_t stride,
+                                bd_detection* out, int32_t out_capacity) {
+    return detect_one(detector, gray, stride, out, out_capacity, false);
+}
+
+// detector.cpp:192-218
+int32_t bd_detector_detect_device(bd_detector* detector, const uint8_t* gray_device,
+                                  int32_t stride, bd_detection* out, int32_t out_capacity) {
+    return detect_one(detector, gray_device, stride, out, out_capacity, true);
+}
+
+void bd_detector_get_stats(bd_detector* detector, bd_stats* out) {
+    if (!detector || !out) return;
+    detector->batch.engine->stats(0, out);
+}
+
+const char* bd_last_error(void) { return g_last_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Version 3: batches
+// ---------------------------------------------------------------------------
+
+bd_batch* bd_batch_create(int32_t family, int32_t count, const int32_t* widths,
+                          const int32_t* heights, const bd_config* config, int32_t* status_out) {
+    if (status_out) *status_out = BD_OK;
+    if (!valid_family(family)) {
+        set_error("unknown tag family");
+        if (status_out) *status_out = BD_ERROR_INVALID_ARGUMENT;
+        return nullptr;
+    }
+    if (count < 1 || !widths || !heights) {
+        set_error("a batch needs at least one frame size");
+        if (status_out) *status_out = BD_ERROR_INVALID_ARGUMENT;
+        return nullptr;
+    }
+    for (int i = 0; i < count; ++i) {
+        if (!valid_dimensions(widths[i], heights[i])) {
+            set_error("image dimensions out of range (16..8192 x 16..16384)");
+            if (status_out) *status_out = BD_ERROR_INVALID_ARGUMENT;
+            return nullptr;
+        }
+    }
+    if (config && config->struct_size != 0 && config->struct_size < sizeof(bd_config)) {
+        set_error("bd_config.struct_size is too small for this library version");
+        if (status_out) *status_out = BD_ERROR_INVALID_ARGUMENT;
+        return nullptr;
+    }
+    std::string error;
+    bd::Engine* engine = bd::Engine::create(to_params(family, config),
+                                            std::vector<int>(widths, widths + count),
+                                            std::vector<int>(heights, heights + count), &error);
+    if (!engine) {
+        set_error(error.empty() ? "failed to create HIP batch detector" : error);
+        if (status_out) *status_out = BD_ERROR_CUDA;
+        return nullptr;
+    }
+    bd_batch* batch = new (std::nothrow) bd_batch();
+    if (!batch) {
+        delete engine;
+        set_error("out of memory allocating batch handle");
+        if (status_out) *status_out = BD_ERROR_OUT_OF_MEMORY;
+        return nullptr;
+    }
+    batch->engine = engine;
+    return batch;
+}
+
+void bd_batch_destroy(bd_batch* batch) {
+    if (!batch) return;
+    delete batch->engine;
+    delete batch;
+}
+
+static int32_t batch_detect(bd_batch* batch, const bd_frame* frames, int32_t count,
+                            bd_detection* out, int32_t out_stride, int32_t* counts, bool device) {
+    if (!batch || !batch->engine) {
+        set_error("null batch");
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    bd::Engine& e = *batch->engine;
+    if (!frames || count != e.frame_count()) {
+        set_error("frame count does not match the batch");
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    for (int i = 0; i < count; ++i) {
+        if (!frames[i].gray) {
+            set_error(device ? "null device image pointer" : "null image pointer");
+            return BD_ERROR_INVALID_ARGUMENT;
+        }
+        if (frames[i].width != e.width(i) || frames[i].height != e.height(i)) {
+            set_error("frame size does not match the batch");
+            return BD_ERROR_INVALID_ARGUMENT;
+        }
+        if (frames[i].stride < frames[i].width) {
+            set_error("stride is smaller than the image width");
+            return BD_ERROR_INVALID_ARGUMENT;
+        }
+    }
+    if (out_stride < 0 || (out_stride > 0 && !out)) {
+        set_error("invalid output buffer");
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    std::string error;
+    const int written = e.detect(frames, device, out, out_stride, counts, &error);
+    if (written < 0) {
+        set_error(error.empty() ? "HIP detection failed" : error);
+        return BD_ERROR_CUDA;
+    }
+    return written;
+}
+
+int32_t bd_batch_detect_host(bd_batch* batch, const bd_frame* frames, int32_t count,
+                             bd_detection* out, int32_t out_stride, int32_t* counts) {
+    return batch_detect(batch, frames, count, out, out_stride, counts, false);
+}
+
+int32_t bd_batch_detect_device(bd_batch* batch, const bd_frame* frames, int32_t count,
+                               bd_detection* out, int32_t out_stride, int32_t* counts) {
+    return batch_detect(batch, frames, count, out, out_stride, counts, true);
+}
+
+void bd_batch_get_stats(bd_batch* batch, int32_t index, bd_stats* out) {
+    if (!batch || !batch->engine || !out || index < 0 || index >= batch->engine->frame_count()) return;
+    batch->engine->stats(index, out);
+}
+
+int64_t bd_batch_debug_copy(bd_batch* batch, int32_t index, int32_t stage, void* dst,
+                            int64_t capacity) {
+    if (!batch || !batch->engine) {
+        set_error("null batch");
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    std::string error;
+    const int64_t n = batch->engine->debug_copy(index, stage, dst, capacity, &error);
+    if (n == -1) {
+        set_error(error);
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    if (n < 0) {
+        set_error(error);
+        return BD_ERROR_CUDA;
+    }
+    return n;
+}
+
+int32_t bd_batch_frame_info(bd_batch* batch, int32_t index, int32_t out[8]) {
+    if (!batch || !batch->engine || !out || !batch->engine->frame_info(index, out)) {
+        set_error("invalid batch or frame index");
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    return BD_OK;
+}
+
+bd_batch* bd_detector_batch(bd_detector* detector) { return detector ? &detector->batch : nullptr; }
+
+int32_t bd_batch_time_stages(bd_batch* batch, int32_t reps, float* ms) {
+    if (!batch || !batch->engine) {
+        set_error("null batch");
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    std::string error;
+    if (!batch->engine->time_stages(reps, ms, &error)) {
+        set_error(error);
+        return BD_ERROR_CUDA;
+    }
+    return BD_OK;
+}
+
+int32_t bd_batch_replay(bd_batch* batch, int32_t reps, float* ms) {
+    if (!batch || !batch->engine) {
+        set_error("null batch");
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    std::string error;
+    if (!batch->engine->replay(reps, ms, &error)) {
+        set_error(error);
+        return BD_ERROR_CUDA;
+    }
+    return BD_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// C++ API (reference detector.cpp:227-380)
+// ---------------------------------------------------------------------------
+
+namespace bdcudatag {
+
+struct Detector::Impl {
+    bd_detector* handle = nullptr;
+    int max_detections = 0;
+    std::vector<bd_detection> scratch;
+    std::mutex mutex;
+};
+
+Detector::Detector(Family family, int width, int height, const DetectorConfig& config)
+    : family_(family), width_(width), height_(height) {
+    bd_config native;
+    bd_config_init(&native);
+    native.max_hamming = config.max_hamming;
+    native.min_cluster_pixels = config.min_cluster_pixels;
+    native.min_tag_pixels = config.min_tag_pixels;
+    native.max_detections = config.max_detections;
+    native.max_candidates = config.max_candidates;
+    native.refine_edges = config.refine_edges ? 1 : 0;
+    native.use_cuda_graph = config.use_cuda_graph ? 1 : 0;
+    native.min_decision_margin = config.min_decision_margin;
+    native.decode_sharpening = config.decode_sharpening;
+    int32_t status = BD_OK;
+    impl_ = new Impl();
+    impl_->handle = bd_detector_create(static_cast<int32_t>(family), width, height, &native, &status);
+    if (!impl_->handle) {
+        const std::string message = bd_last_error();
+        delete impl_;
+        impl_ = nullptr;
+        throw std::runtime_error(message.empty() ? "failed to create bdcudatag detector" : message);
+    }
+    impl_->max_detections = std::min(std::max(native.max_detections, 1), 256);
+    impl_->scratch.resize(static_cast<size_t>(impl_->max_detections));
+}
+
+Detector::~Detector() {
+    if (!impl_) return;
+    if (impl_->handle) bd_detector_destroy(impl_->handle);
+    delete impl_;
+}
+
+Detector::Detector(Detector&& other) noexcept
+    : impl_(other.impl_), family_(other.family_), width_(other.width_), height_(other.height_) {
+    other.impl_ = nullptr;
+}
+
+Detector& Detector::operator=(Detector&& other) noexcept {
+    if (this == &other) return *this;
+    if (impl_) {
+        if (impl_->handle) bd_detector_destroy(impl_->handle);
+        delete impl_;
+    }
+    impl_ = other.impl_;
+    family_ = other.family_;
+    width_ = other.width_;
+    height_ = other.height_;
+    other.impl_ = nullptr;
+    return *this;
+}
+
+static void copy_out(const bd_detection& src, Detection& dst) {
+    dst.id = src.id;
+    dst.hamming = src.hamming;
+    dst.decision_margin = src.decision_margin;
+    for (int k = 0; k < 4; ++k) {
+        dst.corners[static_cast<size_t>(k)][0] = src.corners[k * 2];
+        dst.corners[static_cast<size_t>(k)][1] = src.corners[k * 2 + 1];
+    }
+    for (int k = 0; k < 9; ++k) dst.homography[static_cast<size_t>(k)] = src.homography[k];
+}
+
+int Detector::detect(const uint8_t* gray, int stride, Detection* out, int out_capacity) {
+    if (!impl_) throw std::runtime_error("detector has been moved from");
+    if (out_capacity < 0 || (out_capacity > 0 && !out)) {
+        throw std::invalid_argument("invalid detection output buffer");
+    }
+    if (out_capacity > impl_->max_detections) out_capacity = impl_->max_detections;
+    const std::lock_guard<std::mutex> lock(impl_->mutex);
+    const int count = bd_detector_detect_host(impl_->handle, gray, stride, impl_->scratch.data(), out_capacity);
+    if (count < 0) throw std::runtime_error(bd_last_error());
+    const int written = std::min(count, out_capacity);
+    for (int i = 0; i < written; ++i) copy_out(impl_->scratch[static_cast<size_t>(i)], out[i]);
+    return written;
+}
+
+std::vector<Detection> Detector::detect(const uint8_t* gray, int stride) {
+    if (!impl_) throw std::runtime_error("detector has been moved from");
+    std::vector<Detection> result(static_cast<size_t>(impl_->max_detections));
+    const int count = detect(gray, stride, result.data(), impl_->max_detections);
+    result.resize(static_cast<size_t>(count));
+    return result;
+}
+
+int Detector::detectDevice(const uint8_t* gray_device, int stride, Detection* out, int out_capacity) {
+    if (!impl_) throw std::runtime_error("detector has been moved from");
+    if (out_capacity < 0 || (out_capacity > 0 && !out)) {
+        throw std::invalid_argument("invalid detection output buffer");
+    }
+    if (out_capacity > impl_->max_detections) out_capacity = impl_->max_detections;
+    const std::lock_guard<std::mutex> lock(impl_->mutex);
+    const int count = bd_detector_detect_device(impl_->handle, gray_device, stride, impl_->scratch.data(), out_capacity);
+    if (count < 0) throw std::runtime_error(bd_last_error());
+    const int written = std::min(count, out_capacity);
+    for (int i = 0; i < written; ++i) copy_out(impl_->scratch[static_cast<size_t>(i)], out[i]);
+    return written;
+}
+
+Stats Detector::stats() const {
+    if (!impl_) return {};
+    const std::lock_guard<std::mutex> lock(impl_->mutex);
+    bd_stats native{};
+    bd_detector_get_stats(impl_->handle, &native);
+    Stats out;
+    out.frames = native.frames;
+    out.last_gpu_ms = native.last_gpu_ms;
+    out.average_gpu_ms = native.average_gpu_ms;
+    out.last_total_ms = native.last_total_ms;
+    out.runs = native.runs;
+    out.candidates = native.candidates;
+    out.detections = native.detections;
+    out.quads = native.quads;
+    out.roi_rows = native.roi_rows;
+    return out;
+}
+
+}  // namespace bdcudatag

@@ -1,0 +1,524 @@
+// Host engine: arena layout, graph capture and the detect call.
+//
+// Mirrors the reference's GpuDetector::Impl (detector/src/gpu_detector.cu:
+// 1784-2254): construction-time allocation of every buffer, a non-blocking
+// stream, events around each frame, and a captured graph replayed per call
+// (H2D copy -> kernels -> D2H of counts and detections).  Differences: one
+// engine serves a list of frames (the batch API), the device-pointer path is
+// a graph too (its first node copies the frame descriptors), and the
+// homography is computed on the device inside the graph.
+#include "bd_engine.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <sstream>
+
+#include "tag_codes.inc"
+
+namespace bd {
+namespace {
+
+std::string hip_error(const char* what, hipError_t e) {
+    std::ostringstream ss;
+    ss << what << ": " << hipGetErrorString(e);
+    return ss.str();
+}
+
+int next_power_of_two(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace
+
+Engine* Engine::create(const Params& params, const std::vector<int>& widths,
+                       const std::vector<int>& heights, std::string* error) {
+    Engine* e = new (std::nothrow) Engine();
+    if (!e) {
+        if (error) *error = "out of memory allocating detector";
+        return nullptr;
+    }
+    if (!e->init(params, widths, heights, error)) {
+        delete e;
+        return nullptr;
+    }
+    return e;
+}
+
+Engine::~Engine() { release(); }
+
+void Engine::release() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_host_) (void)hipGraphExecDestroy(graph_host_);
+    if (graph_dev_) (void)hipGraphExecDestroy(graph_dev_);
+    if (ev_start_) (void)hipEventDestroy(ev_start_);
+    if (ev_stop_) (void)hipEventDestroy(ev_stop_);
+    for (hipEvent_t& e : stage_events_) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+    if (stream_) (void)hipStreamDestroy(stream_);
+    if (arena_) (void)hipFree(arena_);
+    if (h_input_) (void)hipHostFree(h_input_);
+    if (h_scal_) (void)hipHostFree(h_scal_);
+    if (h_out_) (void)hipHostFree(h_out_);
+    if (h_desc_dev_) (void)hipHostFree(h_desc_dev_);
+    graph_host_ = graph_dev_ = nullptr;
+    ev_start_ = ev_stop_ = nullptr;
+    stream_ = nullptr;
+    arena_ = nullptr;
+    h_input_ = nullptr;
+    h_scal_ = nullptr;
+    h_out_ = nullptr;
+    h_desc_dev_ = nullptr;
+}
+
+bool Engine::init(const Params& params, const std::vector<int>& widths,
+                  const std::vector<int>& heights, std::string* error) {
+    p_ = params;
+    w_ = widths;
+    h_ = heights;
+    const int nf = static_cast<int>(w_.size());
+    // Clamps of GpuDetector::Impl::initialize (gpu_detector.cu:1906-1943).
+    max_det_ = std::min(std::max(p_.max_detections, 1), 256);
+    const int hamming = p_.max_hamming >= 0 ? std::min(p_.max_hamming, 3) : (p_.family == 0 ? 2 : 1);
+    const uint64_t* codes = p_.family == 0 ? bd_codes_tag36h11 : bd_codes_tag16h5;
+    const int code_count = p_.family == 0 ? 587 : 30;
+    const int bits = p_.family == 0 ? 6 : 4;
+
+    // Arena layout: every frame's buffers, then the shared scalar and output
+    // blocks (contiguous so one copy returns them for all frames).
+    struct Off {
+        size_t tmin, tmax, tmean, tthr, trlo, trhi, roi, hist, bits, rowc, runs, label, area, bbox,
+            keys, cands, quads, hquads, dec;
+    };
+    std::vector<Off> off(nf);
+    std::vector<int> maxc(nf);
+    size_t cur = 0;
+    auto take = [&cur](size_t bytes) {
+        const size_t at = cur;
+        cur = align_up(cur + std::max<size_t>(bytes, 1), 256);
+        return at;
+    };
+    input_off_.assign(nf, 0);
+    input_bytes_ = 0;
+    for (int f = 0; f < nf; ++f) {
+        const int W = w_[f], H = h_[f];
+        const int tw = (W + kTile - 1) / kTile, th = (H + kTile - 1) / kTile;
+        const int w64 = (W + 63) / 64;
+        const size_t slots = static_cast<size_t>(H) * (W / 2 + 2);
+        maxc[f] = p_.max_candidates > 0
+                      ? std::min(std::max(p_.max_candidates, 16), 1024)
+                      : std::min(std::max(next_power_of_two((W * H) / 4096) * 3 / 2, 96), 1024);
+        const size_t tiles = static_cast<size_t>(tw) * th;
+        Off& o = off[f];
+        o.tmin = take(tiles);
+        o.tmax = take(tiles);
+        o.tmean = take(tiles);
+        o.tthr = take(tiles);
+        o.trlo = take(th * 4);
+        o.trhi = take(th * 4);
+        o.roi = take(th * 12);
+        o.hist = take(320 * 4);
+        o.bits = take(static_cast<size_t>(H) * w64 * 8);
+        o.rowc = take(static_cast<size_t>(H) * 4);
+        o.runs = take(slots * 4);
+        o.label = take(slots * 4);
+        o.area = take(slots * 4);
+        o.bbox = take(slots * 16);
+        o.keys = take(slots * 8);
+        o.cands = take(static_cast<size_t>(maxc[f]) * 4);
+        o.quads = take(static_cast<size_t>(maxc[f]) * 32);
+        o.hquads = take(static_cast<size_t>(maxc[f]) * 32);
+        o.dec = take(static_cast<size_t>(maxc[f]) * sizeof(bd_detection));
+        input_off_[f] = input_bytes_;
+        input_bytes_ = align_up(input_bytes_ + static_cast<size_t>(W) * H, 256);
+    }
+    const size_t o_scal = take(static_cast<size_t>(nf) * S_COUNT * 4);
+    const size_t o_out = take(static_cast<size_t>(nf) * max_det_ * sizeof(bd_detection));
+    const size_t o_desc_h = take(static_cast<size_t>(nf) * sizeof(FrameDesc));
+    const size_t o_desc_d = take(static_cast<size_t>(nf) * sizeof(FrameDesc));
+    const size_t o_prefix = take(static_cast<size_t>(SH_COUNT) * (nf + 1) * 4);
+    const size_t o_codes = take(static_cast<size_t>(code_count) * 8);
+    const size_t o_input = take(input_bytes_);
+    const size_t total = cur;
+
+    hipError_t st = hipMalloc(&arena_, total);
+    if (st != hipSuccess) {
+        if (error) *error = hip_error("hipMalloc", st);
+        arena_ = nullptr;
+        return false;
+    }
+    uint8_t* base = static_cast<uint8_t*>(arena_);
+    d_scal_ = reinterpret_cast<uint32_t*>(base + o_scal);
+    d_out_ = reinterpret_cast<bd_detection*>(base + o_out);
+    d_desc_host_ = reinterpret_cast<FrameDesc*>(base + o_desc_h);
+    d_desc_dev_ = reinterpret_cast<FrameDesc*>(base + o_desc_d);
+    d_prefix_ = reinterpret_cast<int32_t*>(base + o_prefix);
+    d_codes_ = reinterpret_cast<uint64_t*>(base + o_codes);
+    d_input_ = base + o_input;
+
+    st = hipHostMalloc(reinterpret_cast<void**>(&h_input_), std::max<size_t>(input_bytes_, 1), hipHostMallocPortable);
+    if (st == hipSuccess) st = hipHostMalloc(reinterpret_cast<void**>(&h_scal_), static_cast<size_t>(nf) * S_COUNT * 4, hipHostMallocPortable);
+    if (st == hipSuccess) st = hipHostMalloc(reinterpret_cast<void**>(&h_out_), static_cast<size_t>(nf) * max_det_ * sizeof(bd_detection), hipHostMallocPortable);
+    if (st == hipSuccess) st = hipHostMalloc(reinterpret_cast<void**>(&h_desc_dev_), static_cast<size_t>(nf) * sizeof(FrameDesc), hipHostMallocPortable);
+    if (st != hipSuccess) {
+        if (error) *error = hip_error("hipHostMalloc", st);
+        return false;
+    }
+
+    desc_.assign(nf, FrameDesc{});
+    std::vector<int32_t> prefix(static_cast<size_t>(SH_COUNT) * (nf + 1), 0);
+    for (int f = 0; f < nf; ++f) {
+        const int W = w_[f], H = h_[f];
+        FrameDesc& d = desc_[f];
+        const Off& o = off[f];
+        d.gray = d_input_ + input_off_[f];
+        d.width = W;
+        d.height = H;
+        d.stride = W;
+        d.tile_w = (W + kTile - 1) / kTile;
+        d.tile_h = (H + kTile - 1) / kTile;
+        d.words64 = (W + 63) / 64;
+        d.row_cap = W / 2 + 2;
+        d.max_candidates = maxc[f];
+        d.max_detections = max_det_;
+        d.dim = bits + 2;
+        d.code_count = code_count;
+        d.max_hamming = hamming;
+        d.min_cluster = std::max(p_.min_cluster_pixels, 1);
+        d.min_tag = std::max(p_.min_tag_pixels, 4);
+        d.refine = p_.refine_edges ? 1 : 0;
+        d.min_margin = p_.min_decision_margin;
+        d.sharpening = p_.decode_sharpening;
+        d.codes = d_codes_;
+        d.tile_min = base + o.tmin;
+        d.tile_max = base + o.tmax;
+        d.tile_mean = base + o.tmean;
+        d.tile_thresh = base + o.tthr;
+        d.trow_lo = reinterpret_cast<int32_t*>(base + o.trlo);
+        d.trow_hi = reinterpret_cast<int32_t*>(base + o.trhi);
+        d.row_roi = reinterpret_cast<int32_t*>(base + o.roi);
+        d.hist = reinterpret_cast<uint32_t*>(base + o.hist);
+        d.scal = d_scal_ + static_cast<size_t>(f) * S_COUNT;
+        d.bits = reinterpret_cast<uint64_t*>(base + o.bits);
+        d.row_count = reinterpret_cast<uint32_t*>(base + o.rowc);
+        d.runs = reinterpret_cast<uint32_t*>(base + o.runs);
+        d.label = reinterpret_cast<uint32_t*>(base + o.label);
+        d.area = reinterpret_cast<uint32_t*>(base + o.area);
+        d.bbox = reinterpret_cast<uint32_t*>(base + o.bbox);
+        d.cand_keys = reinterpret_cast<uint64_t*>(base + o.keys);
+        d.candidates = reinterpret_cast<uint32_t*>(base + o.cands);
+        d.quads = reinterpret_cast<float*>(base + o.quads);
+        d.hull_quads = reinterpret_cast<float*>(base + o.hquads);
+        d.dec = reinterpret_cast<bd_detection*>(base + o.dec);
+        d.out = d_out_ + static_cast<size_t>(f) * max_det_;
+        prefix[SH_TILE_ROW * (nf + 1) + f + 1] = prefix[SH_TILE_ROW * (nf + 1) + f] + d.tile_h;
+        prefix[SH_ROW4 * (nf + 1) + f + 1] = prefix[SH_ROW4 * (nf + 1) + f] + (H + 3) / 4;
+        prefix[SH_CAND * (nf + 1) + f + 1] = prefix[SH_CAND * (nf + 1) + f] + maxc[f];
+        prefix[SH_FRAME * (nf + 1) + f + 1] = f + 1;
+    }
+    std::memcpy(h_desc_dev_, desc_.data(), desc_.size() * sizeof(FrameDesc));
+
+    st = hipMemcpy(d_desc_host_, desc_.data(), desc_.size() * sizeof(FrameDesc), hipMemcpyHostToDevice);
+    if (st == hipSuccess) st = hipMemcpy(d_desc_dev_, desc_.data(), desc_.size() * sizeof(FrameDesc), hipMemcpyHostToDevice);
+    if (st == hipSuccess) st = hipMemcpy(d_prefix_, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice);
+    if (st == hipSuccess) st = hipMemcpy(d_codes_, codes, static_cast<size_t>(code_count) * 8, hipMemcpyHostToDevice);
+    // Accumulators start cleared; afterwards each consumer kernel clears them.
+    for (int f = 0; f < nf && st == hipSuccess; ++f) {
+        st = hipMemset(desc_[f].hist, 0, 320 * 4);
+        if (st == hipSuccess) st = hipMemset(desc_[f].dec, 0xff, static_cast<size_t>(maxc[f]) * sizeof(bd_detection));
+    }
+    if (st == hipSuccess) {
+        std::vector<uint32_t> scal(static_cast<size_t>(nf) * S_COUNT, 0);
+        for (int f = 0; f < nf; ++f) scal[static_cast<size_t>(f) * S_COUNT + S_TOP] = 0xffffffffu;
+        st = hipMemcpy(d_scal_, scal.data(), scal.size() * 4, hipMemcpyHostToDevice);
+    }
+    if (st == hipSuccess) st = hipMemset(d_input_, 0, std::max<size_t>(input_bytes_, 1));
+    if (st != hipSuccess) {
+        if (error) *error = hip_error("detector setup copy", st);
+        return false;
+    }
+
+    plan_host_.frames = d_desc_host_;
+    plan_host_.prefix = d_prefix_;
+    plan_host_.nframes = nf;
+    for (int k = 0; k < SH_COUNT; ++k) plan_host_.blocks[k] = prefix[k * (nf + 1) + nf];
+    plan_dev_ = plan_host_;
+    plan_dev_.frames = d_desc_dev_;
+
+    st = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
+    if (st == hipSuccess) st = hipEventCreate(&ev_start_);
+    if (st == hipSuccess) st = hipEventCreate(&ev_stop_);
+    if (st == hipSuccess) st = hipDeviceSynchronize();
+    if (st != hipSuccess) {
+        if (error) *error = hip_error("stream/event setup", st);
+        return false;
+    }
+    stats_.assign(nf, FrameStats{});
+
+    if (p_.use_graph) {
+        // Graph capture is an optimisation: on failure the engine falls back
+        // to direct launches, as the reference does (gpu_detector.cu:2051-2056).
+        std::string ignored;
+        if (!capture(false, &graph_host_, &ignored)) graph_host_ = nullptr;
+        if (!capture(true, &graph_dev_, &ignored)) graph_dev_ = nullptr;
+    }
+    return true;
+}
+
+hipError_t Engine::enqueue(bool device_path) {
+    const int nf = frame_count();
+    hipError_t st;
+    if (device_path) {
+        st = hipMemcpyAsync(d_desc_dev_, h_desc_dev_, static_cast<size_t>(nf) * sizeof(FrameDesc),
+                            hipMemcpyHostToDevice, stream_);
+        if (st == hipSuccess) st = launch_pipeline(plan_dev_, stream_);
+    } else {
+        st = hipMemcpyAsync(d_input_, h_input_, input_bytes_, hipMemcpyHostToDevice, stream_);
+        if (st == hipSuccess) st = launch_pipeline(plan_host_, stream_);
+    }
+    if (st == hipSuccess)
+        st = hipMemcpyAsync(h_scal_, d_scal_, static_cast<size_t>(nf) * S_COUNT * 4,
+                            hipMemcpyDeviceToHost, stream_);
+    if (st == hipSuccess)
+        st = hipMemcpyAsync(h_out_, d_out_, static_cast<size_t>(nf) * max_det_ * sizeof(bd_detection),
+                            hipMemcpyDeviceToHost, stream_);
+    return st;
+}
+
+bool Engine::capture(bool device_path, hipGraphExec_t* exec, std::string* error) {
+    hipError_t st = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
+    if (st != hipSuccess) {
+        if (error) *error = hip_error("hipStreamBeginCapture", st);
+        return false;
+    }
+    const hipError_t queued = enqueue(device_path);
+    hipGraph_t graph = nullptr;
+    st = hipStreamEndCapture(stream_, &graph);
+    if (queued != hipSuccess || st != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (error) *error = hip_error("hipStreamEndCapture", st);
+        (void)hipGetLastError();
+        return false;
+    }
+    st = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (st != hipSuccess) {
+        if (error) *error = hip_error("hipGraphInstantiate", st);
+        *exec = nullptr;
+        return false;
+    }
+    return true;
+}
+
+int Engine::detect(const bd_frame* frames, bool device, bd_detection* out, int out_stride,
+                   int32_t* counts, std::string* error) {
+    const std::lock_guard<std::mutex> lock(mutex_);
+    const int nf = frame_count();
+    const auto start = std::chrono::steady_clock::now();
+    if (device) {
+        for (int f = 0; f < nf; ++f) {
+            h_desc_dev_[f].gray = frames[f].gray;
+            h_desc_dev_[f].stride = frames[f].stride;
+        }
+    } else {
+        // Repack into pinned staging (gpu_detector.cu:2169-2176).
+        for (int f = 0; f < nf; ++f) {
+            const int W = w_[f], H = h_[f], stride = frames[f].stride;
+            uint8_t* dst = h_input_ + input_off_[f];
+            if (stride == W) {
+                std::memcpy(dst, frames[f].gray, static_cast<size_t>(W) * H);
+            } else {
+                for (int y = 0; y < H; ++y)
+                    std::memcpy(dst + static_cast<size_t>(y) * W, frames[f].gray + static_cast<size_t>(y) * stride, W);
+            }
+        }
+    }
+    hipError_t st = hipEventRecord(ev_start_, stream_);
+    hipGraphExec_t g = device ? graph_dev_ : graph_host_;
+    if (st == hipSuccess) st = g ? hipGraphLaunch(g, stream_) : enqueue(device);
+    if (st == hipSuccess) st = hipEventRecord(ev_stop_, stream_);
+    if (st != hipSuccess) {
+        if (error) *error = hip_error(g ? "hipGraphLaunch" : "pipeline launch", st);
+        return -1;
+    }
+    const hipError_t sync = hipEventSynchronize(ev_stop_);
+    if (sync != hipSuccess) {
+        if (error) *error = hip_error("hipEventSynchronize", sync);
+        return -1;
+    }
+    float gpu_ms = 0.0f;
+    (void)hipEventElapsedTime(&gpu_ms, ev_start_, ev_stop_);
+    const auto end = std::chrono::steady_clock::now();
+    const float total_ms = std::chrono::duration<float, std::milli>(end - start).count();
+
+    // finish() of gpu_detector.cu:2100-2127, per frame.
+    int written_total = 0;
+    for (int f = 0; f < nf; ++f) {
+        const uint32_t* c = h_scal_ + static_cast<size_t>(f) * S_COUNT + S_COUNTS;
+        const int found = std::min(static_cast<int>(c[1]), max_det_);
+        const int written = std::min(found, std::max(out_stride, 0));
+        if (written > 0 && out) {
+            std::memcpy(out + static_cast<size_t>(f) * out_stride, h_out_ + static_cast<size_t>(f) * max_det_,
+                        static_cast<size_t>(written) * sizeof(bd_detection));
+        }
+        if (counts) counts[f] = found;
+        written_total += written;
+        FrameStats& s = stats_[f];
+        s.candidates = std::min(static_cast<int>(c[0]), desc_[f].max_candidates);
+        s.runs = static_cast<int>(c[2]);
+        s.detections = found;
+        s.quads = static_cast<int>(c[3]);
+        s.roi_rows = std::min(static_cast<int>(c[4]) * kTile, h_[f]);
+        s.last_gpu_ms = gpu_ms;
+        s.last_total_ms = total_ms;
+        s.average_gpu_ms = s.frames == 0 ? gpu_ms : 0.9f * s.average_gpu_ms + 0.1f * gpu_ms;
+        ++s.frames;
+    }
+    return written_total;
+}
+
+void Engine::stats(int index, bd_stats* out) const {
+    const std::lock_guard<std::mutex> lock(mutex_);
+    const FrameStats& s = stats_[index];
+    out->frames = s.frames;
+    out->last_gpu_ms = s.last_gpu_ms;
+    out->average_gpu_ms = s.average_gpu_ms;
+    out->last_total_ms = s.last_total_ms;
+    out->runs = s.runs;
+    out->candidates = s.candidates;
+    out->detections = s.detections;
+    out->quads = s.quads;
+    out->roi_rows = s.roi_rows;
+}
+
+bool Engine::frame_info(int index, int32_t out[8]) const {
+    if (index < 0 || index >= frame_count()) return false;
+    const FrameDesc& d = desc_[index];
+    out[0] = d.width;
+    out[1] = d.height;
+    out[2] = d.tile_w;
+    out[3] = d.tile_h;
+    out[4] = d.words64;
+    out[5] = d.row_cap;
+    out[6] = d.max_candidates;
+    out[7] = d.max_detections;
+    return true;
+}
+
+int64_t Engine::debug_copy(int index, int stage, void* dst, int64_t capacity, std::string* error) {
+    const std::lock_guard<std::mutex> lock(mutex_);
+    if (index < 0 || index >= frame_count()) {
+        if (error) *error = "frame index out of range";
+        return -1;
+    }
+    const FrameDesc& d = desc_[index];
+    const size_t tiles = static_cast<size_t>(d.tile_w) * d.tile_h;
+    const size_t slots = static_cast<size_t>(d.height) * d.row_cap;
+    const void* src = nullptr;
+    size_t bytes = 0;
+    switch (stage) {
+        case BD_STAGE_TILE_MIN: src = d.tile_min; bytes = tiles; break;
+        case BD_STAGE_TILE_MAX: src = d.tile_max; bytes = tiles; break;
+        case BD_STAGE_TILE_MEAN: src = d.tile_mean; bytes = tiles; break;
+        case BD_STAGE_TILE_THRESH: src = d.tile_thresh; bytes = tiles; break;
+        case BD_STAGE_HIST: src = d.hist; bytes = 320 * 4; break;
+        case BD_STAGE_SCALARS: src = d.scal; bytes = S_COUNT * 4; break;
+        case BD_STAGE_ROW_ROI: src = d.row_roi; bytes = static_cast<size_t>(d.tile_h) * 12; break;
+        case BD_STAGE_BITS: src = d.bits; bytes = static_cast<size_t>(d.height) * d.words64 * 8; break;
+        case BD_STAGE_ROW_COUNT: src = d.row_count; bytes = static_cast<size_t>(d.height) * 4; break;
+        case BD_STAGE_RUNS: src = d.runs; bytes = slots * 4; break;
+        case BD_STAGE_LABELS: src = d.label; bytes = slots * 4; break;
+        case BD_STAGE_AREA: src = d.area; bytes = slots * 4; break;
+        case BD_STAGE_BBOX: src = d.bbox; bytes = slots * 16; break;
+        case BD_STAGE_CANDIDATES: src = d.candidates; bytes = static_cast<size_t>(d.max_candidates) * 4; break;
+        case BD_STAGE_QUADS: src = d.quads; bytes = static_cast<size_t>(d.max_candidates) * 32; break;
+        case BD_STAGE_HULL_QUADS: src = d.hull_quads; bytes = static_cast<size_t>(d.max_candidates) * 32; break;
+        case BD_STAGE_DECODE: src = d.dec; bytes = static_cast<size_t>(d.max_candidates) * sizeof(bd_detection); break;
+        default:
+            if (error) *error = "unknown stage";
+            return -1;
+    }
+    if (dst && capacity > 0) {
+        const size_t n = std::min(bytes, static_cast<size_t>(capacity));
+        const hipError_t st = hipMemcpy(dst, src, n, hipMemcpyDeviceToHost);
+        if (st != hipSuccess) {
+            if (error) *error = hip_error("hipMemcpy(debug)", st);
+            return -2;
+        }
+    }
+    return static_cast<int64_t>(bytes);
+}
+
+bool Engine::time_stages(int reps, float* ms, std::string* error) {
+    const std::lock_guard<std::mutex> lock(mutex_);
+    if (reps < 1 || !ms) {
+        if (error) *error = "invalid repetition count";
+        return false;
+    }
+    if (!stage_events_[0]) {
+        for (int i = 0; i <= kStageCount; ++i) {
+            const hipError_t st = hipEventCreate(&stage_events_[i]);
+            if (st != hipSuccess) {
+                if (error) *error = hip_error("hipEventCreate", st);
+                return false;
+            }
+        }
+    }
+    for (int i = 0; i <= kStageCount; ++i) ms[i] = 0.0f;
+    for (int r = 0; r < reps; ++r) {
+        for (int stage = 0; stage < kStageCount; ++stage) {
+            hipError_t st = hipEventRecord(stage_events_[stage], stream_);
+            if (st == hipSuccess) st = launch_stage(plan_host_, stage, stream_);
+            if (st != hipSuccess) {
+                if (error) *error = hip_error("kernel launch", st);
+                return false;
+            }
+        }
+        hipError_t st = hipEventRecord(stage_events_[kStageCount], stream_);
+        if (st == hipSuccess) st = hipEventSynchronize(stage_events_[kStageCount]);
+        if (st != hipSuccess) {
+            if (error) *error = hip_error("hipEventSynchronize", st);
+            return false;
+        }
+        for (int stage = 0; stage < kStageCount; ++stage) {
+            float t = 0.0f;
+            (void)hipEventElapsedTime(&t, stage_events_[stage], stage_events_[stage + 1]);
+            ms[stage] += t;
+        }
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, stage_events_[0], stage_events_[kStageCount]);
+        ms[kStageCount] += t;
+    }
+    for (int i = 0; i <= kStageCount; ++i) ms[i] /= static_cast<float>(reps);
+    return true;
+}
+
+bool Engine::replay(int reps, float* ms, std::string* error) {
+    const std::lock_guard<std::mutex> lock(mutex_);
+    if (reps < 1) {
+        if (error) *error = "invalid repetition count";
+        return false;
+    }
+    hipError_t st = hipEventRecord(ev_start_, stream_);
+    for (int r = 0; r < reps && st == hipSuccess; ++r) {
+        st = graph_host_ ? hipGraphLaunch(graph_host_, stream_) : enqueue(false);
+    }
+    if (st == hipSuccess) st = hipEventRecord(ev_stop_, stream_);
+    if (st == hipSuccess) st = hipEventSynchronize(ev_stop_);
+    if (st != hipSuccess) {
+        if (error) *error = hip_error("hipEventSynchronize", st);
+        return false;
+    }
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, ev_start_, ev_stop_);
+    if (ms) *ms = t;
+    return true;
+}
+
+}  // namespace bd

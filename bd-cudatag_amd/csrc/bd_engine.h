@@ -1,0 +1,111 @@
+// Host engine: owns the device arena, the stream, the events and the
+// captured hipGraphs for one list of frame sizes.  A single-frame detector
+// is an engine with one frame.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "bd_internal.h"
+#include "bdcudatag/bdcudatag.h"
+
+namespace bd {
+
+struct Params {
+    int family = 0;
+    int max_hamming = -1;
+    int min_cluster_pixels = 24;
+    int min_tag_pixels = 8;
+    int max_detections = 64;
+    int max_candidates = 0;
+    bool refine_edges = true;
+    bool use_graph = true;
+    float min_decision_margin = 3.0f;
+    float decode_sharpening = 0.25f;
+};
+
+struct FrameStats {
+    uint64_t frames = 0;
+    float last_gpu_ms = 0.0f;
+    float average_gpu_ms = 0.0f;
+    float last_total_ms = 0.0f;
+    int runs = 0, candidates = 0, detections = 0, quads = 0, roi_rows = 0;
+};
+
+class Engine {
+public:
+    static Engine* create(const Params& params, const std::vector<int>& widths,
+                          const std::vector<int>& heights, std::string* error);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    int frame_count() const { return static_cast<int>(w_.size()); }
+    int width(int i) const { return w_[i]; }
+    int height(int i) const { return h_[i]; }
+    int max_detections() const { return max_det_; }
+
+    // Runs every frame; per-frame results go to out + i * out_stride.
+    // `device` selects whether frames[i].gray are device pointers.  Returns the
+    // total written or -1 (error filled).
+    int detect(const bd_frame* frames, bool device, bd_detection* out, int out_stride,
+               int32_t* counts, std::string* error);
+
+    void stats(int index, bd_stats* out) const;
+    int64_t debug_copy(int index, int stage, void* dst, int64_t capacity, std::string* error);
+    bool frame_info(int index, int32_t out[8]) const;
+
+    // Per-kernel device time, averaged over `reps` runs of the pipeline on
+    // the frames already resident in the engine: HIP events on the engine
+    // stream between consecutive kernels.  ms[0..9] per kernel, ms[10] whole.
+    bool time_stages(int reps, float* ms, std::string* error);
+    // Replays the host-path graph `reps` times back to back (H2D of the
+    // staged frames, kernels, D2H) and returns the elapsed device time.
+    bool replay(int reps, float* ms, std::string* error);
+    hipStream_t stream() const { return stream_; }
+
+private:
+    Engine() = default;
+    bool init(const Params& params, const std::vector<int>& widths,
+              const std::vector<int>& heights, std::string* error);
+    bool capture(bool device_path, hipGraphExec_t* exec, std::string* error);
+    hipError_t enqueue(bool device_path);
+    void release();
+
+    Params p_;
+    std::vector<int> w_, h_;
+    int max_det_ = 0;
+    std::vector<FrameDesc> desc_;      // host copy, gray -> engine input buffers
+    std::vector<size_t> input_off_;    // per frame offset in the input region
+    size_t input_bytes_ = 0;
+
+    void* arena_ = nullptr;             // all per-frame device buffers
+    uint8_t* d_input_ = nullptr;        // frames for the host path
+    FrameDesc* d_desc_host_ = nullptr;  // descriptors of the host path
+    FrameDesc* d_desc_dev_ = nullptr;   // descriptors of the device path
+    int32_t* d_prefix_ = nullptr;
+    uint64_t* d_codes_ = nullptr;
+    uint32_t* d_scal_ = nullptr;        // nframes * S_COUNT
+    bd_detection* d_out_ = nullptr;     // nframes * max_det
+
+    uint8_t* h_input_ = nullptr;        // pinned staging
+    uint32_t* h_scal_ = nullptr;
+    bd_detection* h_out_ = nullptr;
+    FrameDesc* h_desc_dev_ = nullptr;   // pinned staging of the device-path descriptors
+
+    Plan plan_host_{};
+    Plan plan_dev_{};
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr;
+    hipEvent_t stage_events_[kStageCount + 1] = {};
+    hipGraphExec_t graph_host_ = nullptr, graph_dev_ = nullptr;
+
+    std::vector<FrameStats> stats_;
+    mutable std::mutex mutex_;
+};
+
+}  // namespace bd

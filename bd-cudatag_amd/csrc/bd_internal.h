@@ -1,0 +1,99 @@
+// Internal declarations shared by the HIP kernels and the host engine.
+//
+// One launch processes a list of frames ("a batch"; the single-frame
+// detector is a batch of one).  Every frame has a FrameDesc in device memory
+// pointing at its own slice of the detector's arena; every kernel's grid
+// spans all frames and a block finds its frame through a per-kernel prefix
+// table (Plan).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "bdcudatag/bdcudatag.h"
+
+namespace bd {
+
+constexpr int kTile = 8;             // reference gpu_detector.cu:38
+constexpr int kMinTileContrast = 12; // reference gpu_detector.cu:39
+constexpr int kFitDirections = 32;   // reference gpu_detector.cu:46
+constexpr int kMaxBoundaryRows = 1024;
+constexpr int kMaxHullVertices = 64;
+constexpr int kFitIterations = 2;
+constexpr int kFitProbeHalf = 24;
+constexpr int kMaxProbes = 2 * kFitProbeHalf + 1;
+constexpr int kMaxSamples = 16;
+
+// Candidate keys sorted inside one workgroup's LDS before falling back to a
+// radix select over global memory.
+constexpr int kSortCap = 8192;
+
+// Per-frame scalar block (u32 each).  Accumulators are cleared by the kernel
+// that consumes them, so a replayed graph needs no memset nodes.
+enum Scalar : int {
+    S_TOP = 0,        // min active tile row (atomicMin), 0xffffffff = none
+    S_BOTTOM1 = 1,    // max active tile row + 1 (atomicMax), 0 = none
+    S_REF0 = 2,       // lower-quartile tile span
+    S_REF1 = 3,       // Otsu split of the tile means
+    S_COUNTS = 4,     // counts[5]: candidates, detections, runs, quads, roi tile rows
+    S_CAND_ACC = 9,   // candidates appended by k_collect
+    S_NCAND = 10,     // min(candidates, max_candidates)
+    S_MAXRUNS = 11,   // maximal black runs in the frame
+    S_DECODED = 12,   // candidates that decoded (before de-duplication)
+    S_COUNT = 16
+};
+
+struct FrameDesc {
+    const uint8_t* gray;
+    int32_t width, height, stride;
+    int32_t tile_w, tile_h, words64, row_cap;
+    int32_t max_candidates, max_detections;
+    int32_t dim, code_count, max_hamming;
+    int32_t min_cluster, min_tag, refine;
+    float min_margin, sharpening;
+    const uint64_t* codes;
+    uint8_t* tile_min;
+    uint8_t* tile_max;
+    uint8_t* tile_mean;
+    uint8_t* tile_thresh;
+    int32_t* trow_lo;     // per tile row: leftmost active tile or tile_w
+    int32_t* trow_hi;     // per tile row: rightmost active tile or -1
+    int32_t* row_roi;     // 3 * tile_h: active, start, end
+    uint32_t* hist;       // 256 span bins + 64 mean bins
+    uint32_t* scal;       // S_COUNT scalars
+    uint64_t* bits;       // height * words64
+    uint32_t* row_count;  // height
+    uint32_t* runs;       // height * row_cap, x0 | x1 << 16
+    uint32_t* label;      // height * row_cap
+    uint32_t* area;       // height * row_cap
+    uint32_t* bbox;       // height * row_cap * 4
+    uint64_t* cand_keys;  // height * row_cap
+    uint32_t* candidates; // max_candidates
+    float* quads;         // max_candidates * 8
+    float* hull_quads;    // max_candidates * 8
+    bd_detection* dec;    // max_candidates, per-candidate decode result
+    bd_detection* out;    // max_detections, final detections
+};
+
+// Prefix tables over frames: prefix[k][f] = first block of frame f for
+// kernel shape k; prefix[k][nframes] = total blocks.
+enum Shape : int { SH_TILE_ROW = 0, SH_ROW4 = 1, SH_CAND = 2, SH_FRAME = 3, SH_COUNT = 4 };
+
+struct Plan {
+    const FrameDesc* frames;
+    const int32_t* prefix;  // SH_COUNT * (nframes + 1)
+    int32_t nframes;
+    int32_t blocks[SH_COUNT];
+};
+
+// Kernels in pipeline order: tile_stats, frame_ref, binarize_runs, merge,
+// stats, collect, select, fit, decode, finalize.
+constexpr int kStageCount = 10;
+
+// Enqueues one kernel (0..kStageCount-1) for every frame of the plan.
+hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream);
+// Enqueues the whole pipeline for every frame of the plan on `stream`.
+hipError_t launch_pipeline(const Plan& plan, hipStream_t stream);
+
+}  // namespace bd

@@ -1,0 +1,1635 @@
+// HIP kernels of the detection pipeline for gfx950 (CDNA4, wave64).
+//
+// Ten kernels, each with a grid that spans every frame of the plan:
+//   k_tile_stats      8x8 tile min/max/mean, histograms, active extents   (a)
+//   k_frame_ref       quartile span + Otsu split, row ROI                 (a)
+//   k_binarize_runs   pooled threshold, 64-px ballot-free bit packing,
+//                     maximal black runs per row                          (b)
+//   k_merge           8-connected union of runs in adjacent rows          (b)
+//   k_stats           root labels, component area and bounding box        (b)
+//   k_collect         tag-shaped component filter                         (c)
+//   k_select          largest-first candidate order (LDS bitonic sort)    (c)
+//   k_fit             row extremes, hull probe, quad, edge refinement     (d)
+//   k_decode          4 rotations x {refined, hull} decode                (e)
+//   k_finalize        de-duplication, stable id sort, FP64 homography     (e)
+//
+// Integer stages are bit-exact with the reference algorithm.  Floating-point
+// stages are written as the same sequence of IEEE +, -, *, /, sqrt as the
+// oracle and are compiled with -ffp-contract=off, so they are bit-identical
+// to it as well.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "bd_internal.h"
+
+namespace bd {
+namespace {
+
+constexpr int kWave = 64;
+
+// cos/sin of fl32(fl32(6.28318530718f * d) / 32), correctly rounded to FP32:
+// the probe directions of the reference hull search (gpu_detector.cu:1139-1142).
+__constant__ float c_dir_cos[32] = {
+    0x1p+0f,          0x1.f6297cp-1f,   0x1.d906bcp-1f,   0x1.a9b662p-1f,
+    0x1.6a09e6p-1f,   0x1.1c73b2p-1f,   0x1.87de2ap-2f,   0x1.8f8b78p-3f,
+    -0x1.777a5cp-25f, -0x1.8f8b84p-3f,  -0x1.87de30p-2f,  -0x1.1c73b8p-1f,
+    -0x1.6a09e6p-1f,  -0x1.a9b664p-1f,  -0x1.d906c0p-1f,  -0x1.f6297ep-1f,
+    -0x1p+0f,         -0x1.f6297cp-1f,  -0x1.d906bcp-1f,  -0x1.a9b660p-1f,
+    -0x1.6a09e2p-1f,  -0x1.1c73acp-1f,  -0x1.87de16p-2f,  -0x1.8f8b8cp-3f,
+    0x1.99bc5cp-27f,  0x1.8f8b90p-3f,   0x1.87de36p-2f,   0x1.1c73bap-1f,
+    0x1.6a09eep-1f,   0x1.a9b662p-1f,   0x1.d906bep-1f,   0x1.f6297ep-1f,
+};
+__constant__ float c_dir_sin[32] = {
+    0x0p+0f,          0x1.8f8b84p-3f,   0x1.87de2cp-2f,   0x1.1c73b4p-1f,
+    0x1.6a09e6p-1f,   0x1.a9b664p-1f,   0x1.d906bcp-1f,   0x1.f6297ep-1f,
+    0x1p+0f,          0x1.f6297cp-1f,   0x1.d906bcp-1f,   0x1.a9b660p-1f,
+    0x1.6a09e6p-1f,   0x1.1c73b2p-1f,   0x1.87de20p-2f,   0x1.8f8b82p-3f,
+    -0x1.777a5cp-24f, -0x1.8f8b9ap-3f,  -0x1.87de2ap-2f,  -0x1.1c73b6p-1f,
+    -0x1.6a09eap-1f,  -0x1.a9b668p-1f,  -0x1.d906c2p-1f,  -0x1.f6297cp-1f,
+    -0x1p+0f,         -0x1.f6297cp-1f,  -0x1.d906bap-1f,  -0x1.a9b65ep-1f,
+    -0x1.6a09dep-1f,  -0x1.1c73b6p-1f,  -0x1.87de2ap-2f,  -0x1.8f8b78p-3f,
+};
+
+// Frame of a block for one grid shape (prefix tables are tiny: one entry
+// per frame).
+__device__ inline int frame_of(const Plan& plan, int shape, int block, int* local) {
+    const int32_t* p = plan.prefix + shape * (plan.nframes + 1);
+    int lo = 0, hi = plan.nframes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (p[mid] <= block) lo = mid; else hi = mid - 1;
+    }
+    *local = block - p[lo];
+    return lo;
+}
+
+__device__ inline uint32_t wave_excl_scan(uint32_t v, int lane, uint32_t* total) {
+    uint32_t x = v;
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, kWave);
+        if (lane >= off) x += y;
+    }
+    *total = __shfl(x, kWave - 1, kWave);
+    return x - v;
+}
+
+__device__ inline uint32_t wave_sum(uint32_t v) {
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// (a) tile statistics
+// ---------------------------------------------------------------------------
+
+__device__ inline void acc_word(uint32_t w, uint32_t& mn, uint32_t& mx, uint32_t& sum) {
+    const uint32_t b0 = w & 0xffu, b1 = (w >> 8) & 0xffu, b2 = (w >> 16) & 0xffu, b3 = w >> 24;
+    mn = min(mn, min(min(b0, b1), min(b2, b3)));
+    mx = max(mx, max(max(b0, b1), max(b2, b3)));
+    sum = __builtin_amdgcn_sad_u8(w, 0u, sum);
+}
+
+// One workgroup per tile row (8 pixel rows).  Each thread owns a 16x8 pixel
+// block (two tiles) read as 16-byte rows, so a wave reads 1 KiB contiguous
+// per row.  reference gpu_detector.cu:104-166 (k_tile_stats, k_tile_histograms)
+// plus the per-row active extents that k_build_row_roi (:282-357) needs.
+__global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
+    int ty;
+    const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
+    const int W = d->width, H = d->height, stride = d->stride, tw = d->tile_w;
+    const uint8_t* gray = d->gray;
+    __shared__ uint32_t hs[256 + 64];
+    __shared__ int lo_s, hi_s;
+    for (int i = threadIdx.x; i < 320; i += blockDim.x) hs[i] = 0;
+    if (threadIdx.x == 0) {
+        lo_s = tw;
+        hi_s = -1;
+    }
+    __syncthreads();
+
+    const int y0 = ty * kTile;
+    const int y1 = min(y0 + kTile, H);
+    const int rows = y1 - y0;
+    const bool vec = ((reinterpret_cast<uintptr_t>(gray) | static_cast<uintptr_t>(stride)) & 15u) == 0;
+    const int pairs = (tw + 1) >> 1;
+    for (int p = threadIdx.x; p < pairs; p += blockDim.x) {
+        const int x0 = p * 16;
+        uint32_t mnA = 255, mxA = 0, sumA = 0, mnB = 255, mxB = 0, sumB = 0;
+        const int nA = min(8, W - x0);
+        const int nB = min(8, W - x0 - 8);
+        if (vec && x0 + 16 <= W) {
+            for (int y = y0; y < y1; ++y) {
+                const uint4 v = *reinterpret_cast<const uint4*>(gray + static_cast<size_t>(y) * stride + x0);
+                acc_word(v.x, mnA, mxA, sumA);
+                acc_word(v.y, mnA, mxA, sumA);
+                acc_word(v.z, mnB, mxB, sumB);
+                acc_word(v.w, mnB, mxB, sumB);
+            }
+        } else {
+            for (int y = y0; y < y1; ++y) {
+                const uint8_t* row = gray + static_cast<size_t>(y) * stride + x0;
+                for (int x = 0; x < nA; ++x) {
+                    const uint32_t v = row[x];
+                    mnA = min(mnA, v);
+                    mxA = max(mxA, v);
+                    sumA += v;
+                }
+                for (int x = 0; x < nB; ++x) {
+                    const uint32_t v = row[8 + x];
+                    mnB = min(mnB, v);
+                    mxB = max(mxB, v);
+                    sumB += v;
+                }
+            }
+        }
+        const int ta = 2 * p;
+        const uint32_t cA = static_cast<uint32_t>(nA * rows);
+        const uint32_t meanA = (sumA + cA / 2) / cA;
+        const size_t ia = static_cast<size_t>(ty) * tw + ta;
+        d->tile_min[ia] = static_cast<uint8_t>(mnA);
+        d->tile_max[ia] = static_cast<uint8_t>(mxA);
+        d->tile_mean[ia] = static_cast<uint8_t>(meanA);
+        atomicAdd(&hs[(mxA - mnA) & 0xffu], 1u);
+        atomicAdd(&hs[256 + (meanA >> 2)], 1u);
+        int lo = tw, hi = -1;
+        if (static_cast<int>(mxA - mnA) >= kMinTileContrast) lo = hi = ta;
+        if (nB > 0) {
+            const uint32_t cB = static_cast<uint32_t>(nB * rows);
+            const uint32_t meanB = (sumB + cB / 2) / cB;
+            d->tile_min[ia + 1] = static_cast<uint8_t>(mnB);
+            d->tile_max[ia + 1] = static_cast<uint8_t>(mxB);
+            d->tile_mean[ia + 1] = static_cast<uint8_t>(meanB);
+            atomicAdd(&hs[(mxB - mnB) & 0xffu], 1u);
+            atomicAdd(&hs[256 + (meanB >> 2)], 1u);
+            if (static_cast<int>(mxB - mnB) >= kMinTileContrast) {
+                lo = min(lo, ta + 1);
+                hi = ta + 1;
+            }
+        }
+        if (hi >= 0) {
+            atomicMin(&lo_s, lo);
+            atomicMax(&hi_s, hi);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 320; i += blockDim.x) {
+        if (hs[i]) atomicAdd(&d->hist[i], hs[i]);
+    }
+    if (threadIdx.x == 0) {
+        d->trow_lo[ty] = lo_s;
+        d->trow_hi[ty] = hi_s;
+        if (hi_s >= 0) {
+            atomicMin(&d->scal[S_TOP], static_cast<uint32_t>(ty));
+            atomicMax(&d->scal[S_BOTTOM1], static_cast<uint32_t>(ty + 1));
+        }
+    }
+}
+
+// One workgroup per frame: the noise-floor quartile and the Otsu split
+// (reference gpu_detector.cu:174-230), the ROI rows (:282-357), and the
+// reset of the accumulators the next frame will use.
+__global__ __launch_bounds__(256) void k_frame_ref(Plan plan) {
+    const FrameDesc* d = plan.frames + blockIdx.x;
+    const int t = threadIdx.x;
+    const int tw = d->tile_w, th = d->tile_h;
+    __shared__ uint32_t span[256];
+    __shared__ uint32_t scan[256];
+    __shared__ uint32_t mean[64];
+    __shared__ int s_top, s_bottom;
+    __shared__ uint32_t s_ref0;
+    __shared__ int s_otsu;
+
+    span[t] = d->hist[t];
+    if (t < 64) mean[t] = d->hist[256 + t];
+    if (t == 0) {
+        s_top = static_cast<int>(d->scal[S_TOP]);
+        s_bottom = static_cast<int>(d->scal[S_BOTTOM1]) - 1;
+        s_ref0 = 0;
+        s_otsu = 0;
+    }
+    scan[t] = span[t];
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t v = t >= off ? scan[t - off] : 0u;
+        __syncthreads();
+        scan[t] += v;
+        __syncthreads();
+    }
+    // First bin whose cumulative count exceeds total/4 (k_tile_reference :184-198).
+    const uint32_t total_span = scan[255];
+    if (total_span > 0) {
+        const uint32_t target = total_span / 4;
+        if (scan[t] > target && (t == 0 || scan[t - 1] <= target)) s_ref0 = static_cast<uint32_t>(t);
+    }
+
+    // Otsu over the 64 mean bins (:200-229).  The reference accumulates the
+    // class weights and sums in FP32 one bin at a time; while every partial
+    // sum is an integer below 2^24 those FP32 sums are exact, so each bin's
+    // terms equal the integer prefix sums converted once and the 64 bins can
+    // be evaluated in parallel.  Larger frames take the serial path.
+    const uint64_t tiles = static_cast<uint64_t>(tw) * th;
+    if (tiles * 63u < (1u << 24)) {
+        if (t < kWave) {
+            const uint32_t m = mean[t];
+            uint32_t tw_total, ts_total;
+            const uint32_t wex = wave_excl_scan(m, t, &tw_total);
+            const uint32_t sex = wave_excl_scan(static_cast<uint32_t>(t) * m, t, &ts_total);
+            const float total = static_cast<float>(tw_total);
+            const float sum = static_cast<float>(ts_total);
+            float between = -1.0f;
+            bool valid = false;
+            if (total > 0.0f) {
+                const float weight_below = static_cast<float>(wex + m);
+                const float weight_above = total - weight_below;
+                if (weight_below > 0.0f && weight_above > 0.0f) {
+                    const float sum_below = static_cast<float>(sex + static_cast<uint32_t>(t) * m);
+                    const float mean_below = sum_below / weight_below;
+                    const float mean_above = (sum - sum_below) / weight_above;
+                    const float delta = mean_below - mean_above;
+                    between = weight_below * weight_above * delta * delta;
+                    valid = true;
+                }
+            }
+            // First bin with the largest between-class variance.
+            float best = valid ? between : -1.0f;
+            int idx = valid ? t : kWave;
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+                const float ob = __shfl_xor(best, off, kWave);
+                const int oi = __shfl_xor(idx, off, kWave);
+                if (ob > best || (ob == best && oi < idx)) {
+                    best = ob;
+                    idx = oi;
+                }
+            }
+            if (t == 0) s_otsu = idx < kWave ? idx : 0;
+        }
+    } else if (t == 0) {
+        float total = 0.0f, sum = 0.0f;
+        for (int i = 0; i < 64; ++i) {
+            total += static_cast<float>(mean[i]);
+            sum += static_cast<float>(i) * static_cast<float>(mean[i]);
+        }
+        int otsu = 0;
+        if (total > 0.0f) {
+            float weight_below = 0.0f, sum_below = 0.0f, best = -1.0f;
+            for (int i = 0; i < 64; ++i) {
+                weight_below += static_cast<float>(mean[i]);
+                if (weight_below <= 0.0f) continue;
+                const float weight_above = total - weight_below;
+                if (weight_above <= 0.0f) break;
+                sum_below += static_cast<float>(i) * static_cast<float>(mean[i]);
+                const float mean_below = sum_below / weight_below;
+                const float mean_above = (sum - sum_below) / weight_above;
+                const float delta = mean_below - mean_above;
+                const float between = weight_below * weight_above * delta * delta;
+                if (between > best) {
+                    best = between;
+                    otsu = i;
+                }
+            }
+        }
+        s_otsu = otsu;
+    }
+    __syncthreads();
+
+    // ROI rows: every tile row between the first and last active row (+-1) is
+    // searched, between its own active extent +-4 tiles, or whole when it has
+    // none (k_build_row_roi :314-356).
+    const int top = s_top, bottom = s_bottom;
+    for (int ty = t; ty < th; ty += blockDim.x) {
+        int32_t* roi = d->row_roi + 3 * ty;
+        if (bottom < 0 || ty < top - 1 || ty > bottom + 1) {
+            roi[0] = 0;
+            roi[1] = 0;
+            roi[2] = -1;
+            continue;
+        }
+        const int lo = d->trow_lo[ty], hi = d->trow_hi[ty];
+        roi[0] = 1;
+        if (hi < 0) {
+            roi[1] = 0;
+            roi[2] = tw - 1;
+        } else {
+            roi[1] = max(0, lo - 4);
+            roi[2] = min(tw - 1, hi + 4);
+        }
+    }
+    if (t == 0) {
+        uint32_t* s = d->scal;
+        s[S_REF0] = s_ref0;
+        s[S_REF1] = static_cast<uint32_t>(s_otsu * 4 + 2);
+        s[S_COUNTS + 0] = 0;
+        s[S_COUNTS + 1] = 0;
+        s[S_COUNTS + 2] = 0;
+        s[S_COUNTS + 3] = 0;
+        s[S_COUNTS + 4] = bottom < 0 ? 0u : static_cast<uint32_t>(min(bottom + 1, th - 1) - max(top - 1, 0) + 1);
+        s[S_CAND_ACC] = 0;
+        s[S_NCAND] = 0;
+        s[S_MAXRUNS] = 0;
+        s[S_DECODED] = 0;
+        s[S_TOP] = 0xffffffffu;
+        s[S_BOTTOM1] = 0;
+    }
+    d->hist[t] = 0;
+    if (t < 64) d->hist[256 + t] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// (b) binarization and runs
+// ---------------------------------------------------------------------------
+
+// One workgroup per tile row.  The pooled thresholds of the row's tiles are
+// built once in LDS; each thread packs 64 pixels of one pixel row into a u64
+// (1 = white) from two 16-byte loads per 32-pixel group; then one wave per
+// pixel row turns the masks into maximal black runs (start/end bit masks,
+// popcount, wave prefix sums), so no fragment joining is needed later.
+// reference gpu_detector.cu:237-276 (k_tile_threshold), :362-391
+// (k_binarize), :427-504 (k_find_runs).
+__global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
+    int ty;
+    const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
+    const int W = d->width, H = d->height, tw = d->tile_w, th = d->tile_h;
+    const int w64 = d->words64, row_cap = d->row_cap, stride = d->stride;
+    const int y0 = ty * kTile;
+    const int nrows = min(kTile, H - y0);
+    const int t = threadIdx.x;
+    const int32_t* roi = d->row_roi + 3 * ty;
+    const int row_on = roi[0], rs = roi[1], re = roi[2];
+
+    __shared__ uint8_t thr[1024];
+    __shared__ uint64_t masks[kTile][128];
+
+    // Pooled 3x3 thresholds for this tile row (k_tile_threshold).
+    const int ref0 = static_cast<int>(d->scal[S_REF0]);
+    const int ref1 = static_cast<int>(d->scal[S_REF1]);
+    const int floor_contrast = max(kMinTileContrast, (ref0 * 3) / 2);
+    for (int tx = t; tx < tw; tx += blockDim.x) {
+        int mn = 255, mx = 0;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int yy = ty + dy;
+            if (yy < 0 || yy >= th) continue;
+            const uint8_t* rmin = d->tile_min + static_cast<size_t>(yy) * tw;
+            const uint8_t* rmax = d->tile_max + static_cast<size_t>(yy) * tw;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int xx = tx + dx;
+                if (xx < 0 || xx >= tw) continue;
+                mn = min(mn, static_cast<int>(rmin[xx]));
+                mx = max(mx, static_cast<int>(rmax[xx]));
+            }
+        }
+        const int tv = (mx - mn >= floor_contrast) ? ((mn + mx + 1) >> 1) : ref1;
+        thr[tx] = static_cast<uint8_t>(tv);
+        d->tile_thresh[static_cast<size_t>(ty) * tw + tx] = static_cast<uint8_t>(tv);
+    }
+    if (!row_on) {
+        for (int i = t; i < nrows * w64; i += blockDim.x) {
+            const int r = i / w64;
+            d->bits[static_cast<size_t>(y0 + r) * w64 + (i - r * w64)] = ~0ull;
+        }
+        if (t < nrows) d->row_count[y0 + t] = 0;
+        return;
+    }
+    __syncthreads();
+
+    const uint8_t* gray = d->gray;
+    const bool vec = ((reinterpret_cast<uintptr_t>(gray) | static_cast<uintptr_t>(stride)) & 15u) == 0;
+    const int items = nrows * w64;
+    for (int it = t; it < items; it += blockDim.x) {
+        const int r = it / w64;
+        const int w = it - r * w64;
+        const int y = y0 + r;
+        uint64_t m = ~0ull;
+        for (int g = 0; g < 2; ++g) {
+            const int grp = 2 * w + g;
+            const int xs = grp * 32;
+            if (xs >= W) continue;
+            const int tile0 = grp * 4;
+            if (!(tile0 <= re && tile0 + 3 >= rs)) continue;  // 32-px ROI test (:376-377)
+            const uint8_t* src = gray + static_cast<size_t>(y) * stride + xs;
+            uint32_t m32 = 0;
+            if (vec && xs + 32 <= W) {
+                for (int q = 0; q < 2; ++q) {
+                    const uint4 v = *reinterpret_cast<const uint4*>(src + 16 * q);
+                    const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+                    for (int j = 0; j < 4; ++j) {
+                        const uint32_t tv = thr[(xs >> 3) + 2 * q + (j >> 1)];
+                        for (int b = 0; b < 4; ++b) {
+                            const uint32_t px = (wd[j] >> (8 * b)) & 0xffu;
+                            m32 |= static_cast<uint32_t>(px > tv) << (16 * q + 4 * j + b);
+                        }
+                    }
+                }
+            } else {
+                for (int k = 0; k < 32; ++k) {
+                    const int x = xs + k;
+                    uint32_t white = 1;
+                    if (x < W) white = src[k] > thr[x >> 3] ? 1u : 0u;
+                    m32 |= white << k;
+                }
+            }
+            m = (m & ~(0xffffffffull << (32 * g))) | (static_cast<uint64_t>(m32) << (32 * g));
+        }
+        masks[r][w] = m;
+        d->bits[static_cast<size_t>(y) * w64 + w] = m;
+    }
+    __syncthreads();
+
+    const int wave = t / kWave, lane = t % kWave;
+    uint16_t* runs16 = reinterpret_cast<uint16_t*>(d->runs);
+    uint32_t block_frags = 0, block_runs = 0;
+    for (int r = wave; r < nrows; r += static_cast<int>(blockDim.x) / kWave) {
+        const int y = y0 + r;
+        const uint32_t base = static_cast<uint32_t>(y) * static_cast<uint32_t>(row_cap);
+        uint32_t carry_s = 0, carry_e = 0, frags = 0;
+        for (int w0 = 0; w0 < w64; w0 += kWave) {
+            const int w = w0 + lane;
+            uint64_t starts = 0, ends = 0;
+            if (w < w64) {
+                const uint64_t m = masks[r][w];
+                const uint64_t prev = w > 0 ? (masks[r][w - 1] >> 63) : 1ull;
+                const uint64_t next = w + 1 < w64 ? (masks[r][w + 1] & 1ull) : 1ull;
+                starts = ~m & ((m << 1) | prev);
+                ends = ~m & ((m >> 1) | (next << 63));
+                // Fragment count of the reference's 32-px words (:454-468),
+                // kept for the runs statistic.
+                for (int h = 0; h < 2; ++h) {
+                    if ((2 * w + h) * 32 >= W) break;
+                    const uint32_t m32 = static_cast<uint32_t>(m >> (32 * h));
+                    frags += __popc(~m32 & ((m32 << 1) | 1u));
+                }
+            }
+            uint32_t tot_s, tot_e;
+            uint32_t ks = carry_s + wave_excl_scan(__popcll(starts), lane, &tot_s);
+            uint32_t ke = carry_e + wave_excl_scan(__popcll(ends), lane, &tot_e);
+            while (starts) {
+                const int b = __ffsll(static_cast<unsigned long long>(starts)) - 1;
+                starts &= starts - 1;
+                const uint32_t slot = base + ks++;
+                runs16[2 * slot] = static_cast<uint16_t>(w * 64 + b);
+                d->label[slot] = slot;
+                d->area[slot] = 0;
+                uint4* bb = reinterpret_cast<uint4*>(d->bbox + 4 * static_cast<size_t>(slot));
+                *bb = make_uint4(0xffffffffu, 0xffffffffu, 0u, 0u);
+            }
+            while (ends) {
+                const int b = __ffsll(static_cast<unsigned long long>(ends)) - 1;
+                ends &= ends - 1;
+                runs16[2 * (base + ke++) + 1] = static_cast<uint16_t>(w * 64 + b);
+            }
+            carry_s += tot_s;
+            carry_e += tot_e;
+        }
+        frags = wave_sum(frags);
+        if (lane == 0) d->row_count[y] = carry_s;
+        block_frags += frags;
+        block_runs += carry_s;
+    }
+    if (lane == 0 && block_runs) {
+        atomicAdd(&d->scal[S_COUNTS + 2], block_frags);
+        atomicAdd(&d->scal[S_MAXRUNS], block_runs);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// (b) union-find over runs
+// ---------------------------------------------------------------------------
+
+// Root of x by plain reads only.  A stale read returns an older, larger
+// label that is still an ancestor, so a walk may be longer but never wrong.
+__device__ inline uint32_t uf_root(const uint32_t* label, uint32_t x) {
+    uint32_t p = label[x];
+    while (p != x) {
+        x = p;
+        p = label[x];
+    }
+    return x;
+}
+
+// Hooks the larger root under the smaller one with atomicMin; labels only
+// decrease, so the root of a component is its smallest slot, i.e. its first
+// run in raster order (the reference's uf_union, gpu_detector.cu:408-421).
+__device__ inline void uf_union(uint32_t* label, uint32_t a, uint32_t b) {
+    for (;;) {
+        a = uf_root(label, a);
+        b = uf_root(label, b);
+        if (a == b) return;
+        if (a > b) {
+            const uint32_t tmp = a;
+            a = b;
+            b = tmp;
+        }
+        const uint32_t old = atomicMin(&label[b], a);
+        if (old == b) return;
+        b = old;
+    }
+}
+
+// One wave per row y: every run unions with the runs of row y-1 that touch
+// [x0-1, x1+1] (8-connectivity; reference k_merge_runs :509-557).  Maximal
+// runs of one row never touch each other, so no same-row joins are needed.
+__global__ __launch_bounds__(256) void k_merge(Plan plan) {
+    int blk;
+    const FrameDesc* d = plan.frames + frame_of(plan, SH_ROW4, blockIdx.x, &blk);
+    const int y = blk * 4 + static_cast<int>(threadIdx.x) / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (y < 1 || y >= d->height) return;
+    const int n = static_cast<int>(d->row_count[y]);
+    const int pn = static_cast<int>(d->row_count[y - 1]);
+    if (n == 0 || pn == 0) return;
+    const uint32_t row_cap = static_cast<uint32_t>(d->row_cap);
+    const uint32_t* cur = d->runs + static_cast<size_t>(y) * row_cap;
+    const uint32_t* prev = d->runs + static_cast<size_t>(y - 1) * row_cap;
+    const uint32_t cur_base = static_cast<uint32_t>(y) * row_cap;
+    const uint32_t prev_base = static_cast<uint32_t>(y - 1) * row_cap;
+    for (int i = lane; i < n; i += kWave) {
+        const uint32_t run = cur[i];
+        const int x0 = static_cast<int>(run & 0xffffu);
+        const int x1 = static_cast<int>(run >> 16);
+        int lo = 0, hi = pn - 1, first = pn;
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1;
+            if (static_cast<int>(prev[mid] >> 16) >= x0 - 1) {
+                first = mid;
+                hi = mid - 1;
+            } else {
+                lo = mid + 1;
+            }
+        }
+        for (int k = first; k < pn; ++k) {
+            if (static_cast<int>(prev[k] & 0xffffu) > x1 + 1) break;
+            uf_union(d->label, cur_base + i, prev_base + k);
+        }
+    }
+}
+
+// Every run learns its root (written back, so later stages read one hop) and
+// adds its length and extent to the root (reference k_region_stats :562-586).
+__global__ __launch_bounds__(256) void k_stats(Plan plan) {
+    int blk;
+    const FrameDesc* d = plan.frames + frame_of(plan, SH_ROW4, blockIdx.x, &blk);
+    const int y = blk * 4 + static_cast<int>(threadIdx.x) / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (y >= d->height) return;
+    const int n = static_cast<int>(d->row_count[y]);
+    const uint32_t base = static_cast<uint32_t>(y) * static_cast<uint32_t>(d->row_cap);
+    for (int i = lane; i < n; i += kWave) {
+        const uint32_t slot = base + static_cast<uint32_t>(i);
+        const uint32_t root = uf_root(d->label, slot);
+        d->label[slot] = root;
+        const uint32_t run = d->runs[slot];
+        const uint32_t x0 = run & 0xffffu, x1 = run >> 16;
+        atomicAdd(&d->area[root], x1 - x0 + 1);
+        atomicMin(&d->bbox[4 * static_cast<size_t>(root) + 0], x0);
+        atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 2], x1);
+        atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 3], static_cast<uint32_t>(y));
+        if (root == slot) d->bbox[4 * static_cast<size_t>(root) + 1] = static_cast<uint32_t>(y);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// (c) candidates
+// ---------------------------------------------------------------------------
+
+// Roots that pass the tag-border filter append the sort key
+// (~area << 32 | root slot): ascending keys are largest first, ties by
+// raster order (reference k_collect_candidates :591-638).
+__global__ __launch_bounds__(256) void k_collect(Plan plan) {
+    int blk;
+    const FrameDesc* d = plan.frames + frame_of(plan, SH_ROW4, blockIdx.x, &blk);
+    const int y = blk * 4 + static_cast<int>(threadIdx.x) / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (y >= d->height) return;
+    const int W = d->width, H = d->height;
+    const int n = static_cast<int>(d->row_count[y]);
+    const uint32_t base = static_cast<uint32_t>(y) * static_cast<uint32_t>(d->row_cap);
+    for (int i = lane; i < n; i += kWave) {
+        const uint32_t idx = base + static_cast<uint32_t>(i);
+        if (d->label[idx] != idx) continue;
+        const int a = static_cast<int>(d->area[idx]);
+        if (a < d->min_cluster) continue;
+        const uint4 bb = *reinterpret_cast<const uint4*>(d->bbox + 4 * static_cast<size_t>(idx));
+        const int x0 = static_cast<int>(bb.x), y0 = y;
+        const int x1 = static_cast<int>(bb.z), y1 = static_cast<int>(bb.w);
+        const int bw = x1 - x0 + 1;
+        const int bh = y1 - y0 + 1;
+        if (bw < d->min_tag || bh < d->min_tag) continue;
+        if (x0 <= 0 || y0 <= 0 || x1 >= W - 1 || y1 >= H - 1) continue;
+        if (bw > 6 * bh || bh > 6 * bw) continue;
+        const float fill = static_cast<float>(a) / static_cast<float>(bw * bh);
+        if (fill < 0.18f || fill > 0.95f) continue;
+        const uint32_t pos = atomicAdd(&d->scal[S_CAND_ACC], 1u);
+        d->cand_keys[pos] = (static_cast<uint64_t>(~static_cast<uint32_t>(a)) << 32) | idx;
+    }
+}
+
+__device__ inline void bitonic_sort_lds(uint64_t* keys, int n) {
+    for (int k = 2; k <= n; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const uint64_t a = keys[i], b = keys[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        keys[i] = b;
+                        keys[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// One workgroup per frame: sorts the candidate keys and keeps the first
+// max_candidates (reference k_order_candidates :642-654, with the in-bucket
+// order fixed to (area desc, root slot asc)).  Lists longer than the LDS
+// buffer first find the key of rank max_candidates by an 8-bit radix select.
+__global__ __launch_bounds__(1024) void k_select(Plan plan) {
+    const FrameDesc* d = plan.frames + blockIdx.x;
+    __shared__ uint64_t keys[kSortCap];
+    __shared__ uint32_t hist[256];
+    __shared__ uint64_t s_prefix, s_mask;
+    __shared__ uint32_t s_rank, s_fill;
+    const uint32_t n = d->scal[S_CAND_ACC];
+    const uint32_t nsel = min(n, static_cast<uint32_t>(d->max_candidates));
+    const int t = threadIdx.x;
+
+    uint32_t m = n;
+    if (n > static_cast<uint32_t>(kSortCap)) {
+        if (t == 0) {
+            s_prefix = 0;
+            s_mask = 0;
+            s_rank = nsel;  // 1-based rank of the last kept key
+            s_fill = 0;
+        }
+        __syncthreads();
+        for (int shift = 56; shift >= 0; shift -= 8) {
+            for (int i = t; i < 256; i += blockDim.x) hist[i] = 0;
+            __syncthreads();
+            const uint64_t prefix = s_prefix, mask = s_mask;
+            for (uint32_t i = t; i < n; i += blockDim.x) {
+                const uint64_t key = d->cand_keys[i];
+                if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+            }
+            __syncthreads();
+            if (t == 0) {
+                uint32_t cum = 0, rank = s_rank;
+                int digit = 255;
+                for (int b = 0; b < 256; ++b) {
+                    if (cum + hist[b] >= rank) {
+                        digit = b;
+                        break;
+                    }
+                    cum += hist[b];
+                }
+                s_rank = rank - cum;
+                s_prefix = prefix | (static_cast<uint64_t>(digit) << shift);
+                s_mask = mask | (255ull << shift);
+            }
+            __syncthreads();
+        }
+        const uint64_t limit = s_prefix;
+        for (uint32_t i = t; i < n; i += blockDim.x) {
+            const uint64_t key = d->cand_keys[i];
+            if (key <= limit) {
+                const uint32_t pos = atomicAdd(&s_fill, 1u);
+                if (pos < static_cast<uint32_t>(kSortCap)) keys[pos] = key;
+            }
+        }
+        __syncthreads();
+        m = min(s_fill, static_cast<uint32_t>(kSortCap));
+    } else {
+        for (uint32_t i = t; i < n; i += blockDim.x) keys[i] = d->cand_keys[i];
+    }
+    int p2 = 1;
+    while (p2 < static_cast<int>(m)) p2 <<= 1;
+    for (int i = static_cast<int>(m) + t; i < p2; i += blockDim.x) keys[i] = ~0ull;
+    __syncthreads();
+    if (m > 1) bitonic_sort_lds(keys, p2);
+    for (uint32_t i = t; i < nsel; i += blockDim.x) d->candidates[i] = static_cast<uint32_t>(keys[i]);
+    if (t == 0) {
+        d->scal[S_COUNTS + 0] = n;
+        d->scal[S_NCAND] = nsel;
+        d->scal[S_CAND_ACC] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// (d) quad fit
+// ---------------------------------------------------------------------------
+
+__device__ inline float sample_bilinear(const uint8_t* image, int width, int height, int stride,
+                                        float x, float y) {
+    if (!(x >= 0.0f && y >= 0.0f && x < static_cast<float>(width - 1) &&
+          y < static_cast<float>(height - 1))) {
+        return -1.0f;
+    }
+    const int ix = static_cast<int>(x);
+    const int iy = static_cast<int>(y);
+    const float fx = x - static_cast<float>(ix);
+    const float fy = y - static_cast<float>(iy);
+    const uint8_t* p = image + static_cast<size_t>(iy) * stride + ix;
+    const float a = p[0], b = p[1], c = p[stride], e = p[stride + 1];
+    return (a * (1.0f - fx) + b * fx) * (1.0f - fy) + (c * (1.0f - fx) + e * fx) * fy;
+}
+
+__device__ inline float directional_rise(const uint8_t* image, int width, int height, int stride,
+                                         float x, float y, float dx, float dy) {
+    const float ahead = sample_bilinear(image, width, height, stride, x + dx * 0.7f, y + dy * 0.7f);
+    const float behind = sample_bilinear(image, width, height, stride, x - dx * 0.7f, y - dy * 0.7f);
+    if (ahead < 0.0f || behind < 0.0f) return 0.0f;
+    return ahead - behind;
+}
+
+__device__ inline float hyp(float x, float y) { return sqrtf(x * x + y * y); }
+
+// Major-axis normal of the moments by the half-angle identities (the same
+// closed form as the oracle; see the note on axis_normal there).
+__device__ inline void axis_normal(float cxx, float cxy, float cyy, float* nx, float* ny) {
+    const float a = cxx - cyy;
+    const float b = 2.0f * cxy;
+    const float r = sqrtf(a * a + b * b);
+    float c, s;
+    if (!(r > 0.0f)) {
+        c = 1.0f;
+        s = 0.0f;
+    } else if (a >= 0.0f) {
+        c = sqrtf((r + a) / (2.0f * r));
+        s = b / (2.0f * r * c);
+    } else {
+        const float sa = sqrtf((r - a) / (2.0f * r));
+        s = signbit(b) ? -sa : sa;
+        c = b / (2.0f * r * s);
+    }
+    *nx = -s;
+    *ny = c;
+}
+
+__device__ inline bool intersect_lines(const float* li, const float* lj, float* x, float* y) {
+    const float det = li[2] * lj[3] - li[3] * lj[2];
+    if (fabsf(det) < 0.2f) return false;
+    const float di = li[2] * li[0] + li[3] * li[1];
+    const float dj = lj[2] * lj[0] + lj[3] * lj[1];
+    *x = (di * lj[3] - li[3] * dj) / det;
+    *y = (li[2] * dj - di * lj[2]) / det;
+    return true;
+}
+
+// reference apply_refined_lines (gpu_detector.cu:1021-1076).
+__device__ float apply_refined_lines(float quad[4][2], const float lines[4][4], const int have[4]) {
+    int line_count = 0;
+    for (int i = 0; i < 4; ++i) line_count += have[i] ? 1 : 0;
+    if (line_count < 3) return 0.0f;
+    float refined[4][2];
+    for (int i = 0; i < 4; ++i) {
+        refined[i][0] = quad[i][0];
+        refined[i][1] = quad[i][1];
+    }
+    float extent = 0.0f;
+    for (int i = 0; i < 4; ++i) {
+        const int j = (i + 1) & 3;
+        extent = fmaxf(extent, hyp(quad[j][0] - quad[i][0], quad[j][1] - quad[i][1]));
+    }
+    const float max_shift = fminf(fmaxf(0.15f * extent, 4.0f), 16.0f);
+    float moved = 0.0f;
+    for (int i = 0; i < 4; ++i) {
+        const int j = (i + 1) & 3;
+        if (!have[i] || !have[j]) continue;
+        float x, y;
+        if (!intersect_lines(lines[i], lines[j], &x, &y)) continue;
+        const float dx = x - quad[j][0];
+        const float dy = y - quad[j][1];
+        if (dx * dx + dy * dy > max_shift * max_shift) continue;
+        refined[j][0] = x;
+        refined[j][1] = y;
+        moved = fmaxf(moved, sqrtf(dx * dx + dy * dy));
+    }
+    for (int i = 0; i < 4; ++i) {
+        const int j = (i + 1) & 3;
+        if (hyp(refined[j][0] - refined[i][0], refined[j][1] - refined[i][1]) < 3.0f) return 0.0f;
+    }
+    float area2 = 0.0f;
+    for (int i = 0; i < 4; ++i) {
+        const int j = (i + 1) & 3;
+        area2 += refined[i][0] * refined[j][1] - refined[j][0] * refined[i][1];
+    }
+    if (area2 <= 0.0f) return 0.0f;
+    for (int i = 0; i < 4; ++i) {
+        quad[i][0] = refined[i][0];
+        quad[i][1] = refined[i][1];
+    }
+    return moved;
+}
+
+// Geometry of one edge fit (reference fit_edge_line_warp :884-920).
+struct EdgeGeom {
+    float ax, ay, ex, ey, nx, ny, step;
+    int half, probes, samples;
+    bool ok;
+};
+
+__device__ inline EdgeGeom edge_geometry(const float (*quad)[2], int edge, int dim, bool first_pass) {
+    EdgeGeom g;
+    const int b = (edge + 1) & 3;
+    g.ax = quad[edge][0];
+    g.ay = quad[edge][1];
+    const float bx = quad[b][0], by = quad[b][1];
+    g.ex = bx - g.ax;
+    g.ey = by - g.ay;
+    const float len = sqrtf(g.ex * g.ex + g.ey * g.ey);
+    g.ok = !(len < 8.0f);
+    g.nx = g.ey / len;
+    g.ny = -g.ex / len;
+    const float cell = len / static_cast<float>(dim);
+    const float range = first_pass ? fminf(fmaxf(1.15f * cell, 1.5f), 2.5f * cell)
+                                   : fminf(fmaxf(0.30f * cell, 0.8f), 2.0f);
+    g.half = min(max(2, static_cast<int>(ceilf(range / 0.75f))), kFitProbeHalf);
+    g.probes = 2 * g.half + 1;
+    g.step = range / static_cast<float>(g.half);
+    g.samples = min(max(static_cast<int>(len / 6.0f), 8), kMaxSamples);
+    return g;
+}
+
+// One workgroup (4 waves) per candidate.
+//  1. Row extremes of the component over (at most 1024 sampled) bbox rows,
+//     by binary search in each row's runs (collect_row_extremes :676-715).
+//  2. The extreme boundary point in each of 32 directions: 8 slices of the
+//     boundary x 32 directions, first strictly larger projection wins, so the
+//     result is the first maximum in boundary order as in :1132-1170.
+//  3. Hull reduction to four corners by the smallest-triangle rule, one lane
+//     per vertex (hull_to_quad :736-831).
+//  4. Two passes of edge refinement: every (edge, sample, probe) rise is
+//     evaluated in parallel into LDS, then one lane per (edge, sample) runs
+//     the peak rules and the parabola, and the moments are summed in the
+//     reference's 32-lane shuffle-down order (fit_edge_line_warp :881-1016).
+__global__ __launch_bounds__(256) void k_fit(Plan plan) {
+    int ci;
+    const FrameDesc* d = plan.frames + frame_of(plan, SH_CAND, blockIdx.x, &ci);
+    const int t = threadIdx.x;
+    const int wave = t / kWave, lane = t % kWave;
+    float* out = d->quads + 8 * static_cast<size_t>(ci);
+    float* hull_out = d->hull_quads + 8 * static_cast<size_t>(ci);
+    if (t < 8) {
+        out[t] = __int_as_float(0x7fc00000);
+        hull_out[t] = __int_as_float(0x7fc00000);
+    }
+    if (ci >= static_cast<int>(d->scal[S_NCAND])) return;
+
+    __shared__ uint32_t boundary[2 * kMaxBoundaryRows];
+    __shared__ float pdot[8][kFitDirections];
+    __shared__ int pidx[8][kFitDirections];
+    __shared__ uint32_t extreme[kFitDirections];
+    __shared__ float mags[4][kMaxSamples][kMaxProbes];
+    __shared__ float red[4][32][5];
+    __shared__ int red_used[4][32];
+    __shared__ float fit_quad[4][2];
+    __shared__ float edge_lines[4][4];
+    __shared__ int edge_have[4];
+    __shared__ int quad_ok;
+    __shared__ float refine_moved;
+
+    const uint32_t root = d->candidates[ci];
+    const uint32_t row_cap = static_cast<uint32_t>(d->row_cap);
+    const uint4 bb = *reinterpret_cast<const uint4*>(d->bbox + 4 * static_cast<size_t>(root));
+    const int bx0 = static_cast<int>(bb.x), by0 = static_cast<int>(root / row_cap);
+    const int bx1 = static_cast<int>(bb.z), by1 = static_cast<int>(bb.w);
+    const int rows = by1 - by0 + 1;
+    const int kept = min(rows, kMaxBoundaryRows);
+    const int points = 2 * kept;
+
+    for (int s = t; s < kept; s += blockDim.x) {
+        const int y = by0 + (kept > 1 ? (s * (rows - 1)) / (kept - 1) : 0);
+        const int n = static_cast<int>(d->row_count[y]);
+        const uint32_t* rr = d->runs + static_cast<size_t>(y) * row_cap;
+        const uint32_t* lab = d->label + static_cast<size_t>(y) * row_cap;
+        int lo = 0, hi = n - 1, first = n;
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1;
+            if (static_cast<int>(rr[mid] >> 16) >= bx0) {
+                first = mid;
+                hi = mid - 1;
+            } else {
+                lo = mid + 1;
+            }
+        }
+        int left = -1, right = -1;
+        for (int k = first; k < n; ++k) {
+            const uint32_t run = rr[k];
+            const int x0 = static_cast<int>(run & 0xffffu);
+            const int x1 = static_cast<int>(run >> 16);
+            if (x0 > bx1) break;
+            if (lab[k] != root) continue;
+            left = left < 0 ? x0 : min(left, x0);
+            right = right < 0 ? x1 : max(right, x1);
+        }
+        boundary[2 * s] = left < 0 ? 0xffffffffu : ((static_cast<uint32_t>(left) & 0xffffu) | (static_cast<uint32_t>(y) << 16));
+        boundary[2 * s + 1] = right < 0 ? 0xffffffffu : ((static_cast<uint32_t>(right) & 0xffffu) | (static_cast<uint32_t>(y) << 16));
+    }
+    __syncthreads();
+
+    {
+        const int dir = t & 31, part = t >> 5;
+        const int chunk = (points + 7) / 8;
+        const int begin = part * chunk;
+        const int end = min(points, begin + chunk);
+        const float ux = c_dir_cos[dir], uy = c_dir_sin[dir];
+        float best = -FLT_MAX;
+        int bi = -1;
+        for (int i = begin; i < end; ++i) {
+            const uint32_t p = boundary[i];
+            if (p == 0xffffffffu) continue;
+            const float dd = static_cast<float>(p & 0xffffu) * ux + static_cast<float>(p >> 16) * uy;
+            if (dd > best) {
+                best = dd;
+                bi = i;
+            }
+        }
+        pdot[part][dir] = best;
+        pidx[part][dir] = bi;
+    }
+    __syncthreads();
+    if (t < kFitDirections) {
+        float best = -FLT_MAX;
+        uint32_t point = 0xffffffffu;
+        for (int p = 0; p < 8; ++p) {
+            if (pdot[p][t] > best) {
+                best = pdot[p][t];
+                point = boundary[pidx[p][t]];
+            }
+        }
+        extreme[t] = point;
+    }
+    __syncthreads();
+
+    // Hull reduction on wave 0.
+    if (wave == 0) {
+        // Ordered hull without repeats (:1172-1180), built by lane 0 and
+        // broadcast one vertex per lane.
+        uint32_t hv = 0xffffffffu;
+        int hc = 0;
+        if (lane == 0) {
+            uint32_t hull[kFitDirections];
+            for (int i = 0; i < kFitDirections; ++i) {
+                const uint32_t p = extreme[i];
+                if (p == 0xffffffffu) continue;
+                if (hc > 0 && hull[hc - 1] == p) continue;
+                hull[hc++] = p;
+            }
+            while (hc > 1 && hull[0] == hull[hc - 1]) --hc;
+            for (int i = 0; i < hc; ++i) extreme[i] = hull[i];
+        }
+        hc = __shfl(hc, 0, kWave);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        if (lane < hc) hv = extreme[lane];
+        int ok = hc >= 4 && hc <= kMaxHullVertices;
+        float vx = static_cast<float>(hv & 0xffffu);
+        float vy = static_cast<float>(hv >> 16);
+        int count = hc;
+        while (ok && count > 4) {
+            const int prev = (lane + count - 1) % count;
+            const int next = (lane + 1) % count;
+            const float pvx = __shfl(vx, prev, kWave), pvy = __shfl(vy, prev, kWave);
+            const float nvx = __shfl(vx, next, kWave), nvy = __shfl(vy, next, kWave);
+            const float ax = vx - pvx, ay = vy - pvy;
+            const float bx = nvx - vx, by = nvy - vy;
+            float area = fabsf(ax * by - ay * bx) * 0.5f;
+            int idx = lane;
+            if (lane >= count || !(area < FLT_MAX)) {
+                area = FLT_MAX;
+                idx = kWave;
+            }
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+                const float oa = __shfl_xor(area, off, kWave);
+                const int oi = __shfl_xor(idx, off, kWave);
+                if (oa < area || (oa == area && oi < idx)) {
+                    area = oa;
+                    idx = oi;
+                }
+            }
+            if (idx >= kWave) {
+                ok = 0;
+                break;
+            }
+            const float sx = __shfl(vx, min(lane + 1, kWave - 1), kWave);
+            const float sy = __shfl(vy, min(lane + 1, kWave - 1), kWave);
+            if (lane >= idx) {
+                vx = sx;
+                vy = sy;
+            }
+            --count;
+        }
+        float qx[4], qy[4];
+        for (int i = 0; i < 4; ++i) {
+            qx[i] = __shfl(vx, i, kWave);
+            qy[i] = __shfl(vy, i, kWave);
+        }
+        if (lane == 0) {
+            if (ok) {
+                float area2 = 0.0f;
+                for (int i = 0; i < 4; ++i) {
+                    const int j = (i + 1) & 3;
+                    area2 += qx[i] * qy[j] - qx[j] * qy[i];
+                }
+                if (fabsf(area2) < 24.0f) ok = 0;
+                if (ok && area2 < 0.0f) {
+                    for (int i = 0; i < 2; ++i) {
+                        const float tx = qx[i + 1], tyy = qy[i + 1];
+                        qx[i + 1] = qx[3 - i];
+                        qy[i + 1] = qy[3 - i];
+                        qx[3 - i] = tx;
+                        qy[3 - i] = tyy;
+                    }
+                }
+            }
+            if (ok) {
+                float min_side = FLT_MAX, max_side = 0.0f;
+                for (int i = 0; i < 4 && ok; ++i) {
+                    const int j = (i + 1) & 3;
+                    const float ex = qx[j] - qx[i];
+                    const float ey = qy[j] - qy[i];
+                    const float len = sqrtf(ex * ex + ey * ey);
+                    if (len < 4.0f) {
+                        ok = 0;
+                        break;
+                    }
+                    min_side = fminf(min_side, len);
+                    max_side = fmaxf(max_side, len);
+                    const int p = (i + 3) & 3;
+                    const float ax = qx[i] - qx[p];
+                    const float ay = qy[i] - qy[p];
+                    const float bx = qx[j] - qx[i];
+                    const float by = qy[j] - qy[i];
+                    if (ax * by - ay * bx <= 0.0f) {
+                        ok = 0;
+                        break;
+                    }
+                    const float dot = ax * bx + ay * by;
+                    const float n1 = sqrtf(ax * ax + ay * ay);
+                    const float n2 = sqrtf(bx * bx + by * by);
+                    if (n1 < 1e-3f || n2 < 1e-3f || fabsf(dot / (n1 * n2)) > 0.97f) {
+                        ok = 0;
+                        break;
+                    }
+                }
+                if (ok && max_side > 12.0f * min_side) ok = 0;
+            }
+            if (ok) {
+                const float cx = 0.25f * (qx[0] + qx[1] + qx[2] + qx[3]);
+                const float cy = 0.25f * (qy[0] + qy[1] + qy[2] + qy[3]);
+                for (int i = 0; i < 4; ++i) {
+                    const float dx = qx[i] - cx;
+                    const float dy = qy[i] - cy;
+                    const float len = sqrtf(dx * dx + dy * dy);
+                    if (len > 1e-3f) {
+                        qx[i] += dx / len * 0.65f;
+                        qy[i] += dy / len * 0.65f;
+                    }
+                }
+                for (int i = 0; i < 4; ++i) {
+                    fit_quad[i][0] = qx[i];
+                    fit_quad[i][1] = qy[i];
+                    hull_out[2 * i] = qx[i];
+                    hull_out[2 * i + 1] = qy[i];
+                }
+            }
+            quad_ok = ok;
+        }
+    }
+    __syncthreads();
+    if (!quad_ok) return;
+
+    if (d->refine) {
+        const uint8_t* gray = d->gray;
+        const int W = d->width, H = d->height, stride = d->stride;
+        for (int iteration = 0; iteration < kFitIterations; ++iteration) {
+            const bool first_pass = iteration == 0;
+            const EdgeGeom g = edge_geometry(fit_quad, wave, d->dim, first_pass);
+            if (g.ok) {
+                const int total = g.samples * g.probes;
+                for (int it = lane; it < total; it += kWave) {
+                    const int s = it / g.probes;
+                    const int i = it - s * g.probes;
+                    const float alpha = static_cast<float>(s + 1) / static_cast<float>(g.samples + 1);
+                    const float px = g.ax + alpha * g.ex;
+                    const float py = g.ay + alpha * g.ey;
+                    const float tt = g.step * static_cast<float>(i - g.half);
+                    mags[wave][s][i] = directional_rise(gray, W, H, stride, px + g.nx * tt, py + g.ny * tt, g.nx, g.ny);
+                }
+            }
+            __syncthreads();
+            if (lane < 32) {
+                float mx = 0.0f, my = 0.0f, mxx = 0.0f, mxy = 0.0f, myy = 0.0f;
+                int used = 0;
+                if (g.ok && lane < g.samples) {
+                    const float* mg = mags[wave][lane];
+                    const float alpha = static_cast<float>(lane + 1) / static_cast<float>(g.samples + 1);
+                    const float px = g.ax + alpha * g.ex;
+                    const float py = g.ay + alpha * g.ey;
+                    float best_mag = 0.0f;
+                    int best = -1;
+                    for (int i = 0; i < g.probes; ++i) {
+                        if (mg[i] > best_mag) {
+                            best_mag = mg[i];
+                            best = i;
+                        }
+                    }
+                    if (first_pass && best > 2) {
+                        const float accept = 0.70f * best_mag;
+                        for (int i = 1; i + 1 < best; ++i) {
+                            if (mg[i] < accept || mg[i] < mg[i - 1] || mg[i] < mg[i + 1]) continue;
+                            float valley = FLT_MAX;
+                            for (int k = i + 1; k < best; ++k) valley = fminf(valley, mg[k]);
+                            if (valley < 0.5f * mg[i]) {
+                                best = i;
+                                best_mag = mg[i];
+                                break;
+                            }
+                        }
+                    }
+                    if (best >= 0 && best_mag >= 6.0f) {
+                        float tt = g.step * static_cast<float>(best - g.half);
+                        if (best > 0 && best < g.probes - 1) {
+                            const float m0 = directional_rise(gray, W, H, stride, px + g.nx * (tt - g.step),
+                                                              py + g.ny * (tt - g.step), g.nx, g.ny);
+                            const float m2 = directional_rise(gray, W, H, stride, px + g.nx * (tt + g.step),
+                                                              py + g.ny * (tt + g.step), g.nx, g.ny);
+                            const float den = m0 - 2.0f * best_mag + m2;
+                            if (den < -1e-3f) {
+                                tt += fminf(fmaxf(0.5f * (m0 - m2) / den, -0.5f), 0.5f) * g.step;
+                            }
+                        }
+                        const float x = px + g.nx * tt;
+                        const float y = py + g.ny * tt;
+                        mx = x;
+                        my = y;
+                        mxx = x * x;
+                        mxy = x * y;
+                        myy = y * y;
+                        used = 1;
+                    }
+                }
+                red[wave][lane][0] = mx;
+                red[wave][lane][1] = my;
+                red[wave][lane][2] = mxx;
+                red[wave][lane][3] = mxy;
+                red[wave][lane][4] = myy;
+                red_used[wave][lane] = used;
+            }
+            __syncthreads();
+            if (lane == 0) {
+                int have = 0;
+                if (g.ok) {
+                    // The reference's shuffle-down tree over 32 lanes: after
+                    // offset `off`, lane i < off holds lane i + lane i+off.
+                    for (int off = 16; off > 0; off >>= 1) {
+                        for (int i = 0; i < off; ++i) {
+                            for (int k = 0; k < 5; ++k) red[wave][i][k] = red[wave][i][k] + red[wave][i + off][k];
+                            red_used[wave][i] += red_used[wave][i + off];
+                        }
+                    }
+                    const int used = red_used[wave][0];
+                    if (used >= 4) {
+                        const float inv = 1.0f / static_cast<float>(used);
+                        const float cx = red[wave][0][0] * inv;
+                        const float cy = red[wave][0][1] * inv;
+                        const float cxx = red[wave][0][2] * inv - cx * cx;
+                        const float cxy = red[wave][0][3] * inv - cx * cy;
+                        const float cyy = red[wave][0][4] * inv - cy * cy;
+                        float lnx, lny;
+                        axis_normal(cxx, cxy, cyy, &lnx, &lny);
+                        edge_lines[wave][0] = cx;
+                        edge_lines[wave][1] = cy;
+                        edge_lines[wave][2] = lnx;
+                        edge_lines[wave][3] = lny;
+                        have = 1;
+                    }
+                }
+                edge_have[wave] = have;
+            }
+            __syncthreads();
+            if (t == 0) {
+                float q[4][2];
+                float lines[4][4];
+                int have[4];
+                for (int i = 0; i < 4; ++i) {
+                    q[i][0] = fit_quad[i][0];
+                    q[i][1] = fit_quad[i][1];
+                    have[i] = edge_have[i];
+                    for (int k = 0; k < 4; ++k) lines[i][k] = edge_lines[i][k];
+                }
+                refine_moved = apply_refined_lines(q, lines, have);
+                for (int i = 0; i < 4; ++i) {
+                    fit_quad[i][0] = q[i][0];
+                    fit_quad[i][1] = q[i][1];
+                }
+            }
+            __syncthreads();
+            if (refine_moved < 0.15f) break;
+        }
+    }
+    if (t == 0) {
+        for (int i = 0; i < 4; ++i) {
+            out[2 * i] = fit_quad[i][0];
+            out[2 * i + 1] = fit_quad[i][1];
+        }
+        atomicAdd(&d->scal[S_COUNTS + 3], 1u);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// (e) decode
+// ---------------------------------------------------------------------------
+
+// reference homography_from_unit_quad (gpu_detector.cu:1249-1272).
+__device__ inline bool homography_from_unit_quad(const float (*q)[2], float h[9]) {
+    const float x0 = q[0][0], y0 = q[0][1];
+    const float x1 = q[1][0], y1 = q[1][1];
+    const float x2 = q[2][0], y2 = q[2][1];
+    const float x3 = q[3][0], y3 = q[3][1];
+    const float dx1 = x1 - x2, dx2 = x3 - x2, dx3 = x0 - x1 + x2 - x3;
+    const float dy1 = y1 - y2, dy2 = y3 - y2, dy3 = y0 - y1 + y2 - y3;
+    const float det = dx1 * dy2 - dx2 * dy1;
+    if (fabsf(det) < 1e-9f) return false;
+    const float g = (dx3 * dy2 - dx2 * dy3) / det;
+    const float k = (dx1 * dy3 - dx3 * dy1) / det;
+    h[0] = x1 - x0 + g * x1;
+    h[1] = x3 - x0 + k * x3;
+    h[2] = x0;
+    h[3] = y1 - y0 + g * y1;
+    h[4] = y3 - y0 + k * y3;
+    h[5] = y0;
+    h[6] = g;
+    h[7] = k;
+    h[8] = 1.0f;
+    return true;
+}
+
+// One workgroup of 8 waves per candidate: wave v decodes variant v (refined
+// quad for v < 4, hull quad otherwise) at rotation v & 3, one lane per grid
+// cell; the code is matched with one lane per group of codewords.  Lane 0
+// of wave 0 applies the reference's preference (refined first, highest
+// score, first on ties).  reference decode_quad :1291-1430 and
+// k_decode_quads :1438-1537.
+__global__ __launch_bounds__(512) void k_decode(Plan plan) {
+    int ci;
+    const FrameDesc* d = plan.frames + frame_of(plan, SH_CAND, blockIdx.x, &ci);
+    const int t = threadIdx.x;
+    const int wave = t / kWave, lane = t % kWave;
+    bd_detection* res = d->dec + ci;
+    if (ci >= d->max_candidates) return;
+    if (ci >= static_cast<int>(d->scal[S_NCAND])) {
+        if (t == 0) res->id = -1;
+        return;
+    }
+    const float* qsrc = d->quads + 8 * static_cast<size_t>(ci);
+    if (qsrc[0] != qsrc[0]) {
+        if (t == 0) res->id = -1;
+        return;
+    }
+    __shared__ float s_diff[8][64];
+    __shared__ float s_sharp[8][64];
+    __shared__ int r_id[8], r_ham[8];
+    __shared__ float r_margin[8];
+    __shared__ float r_corners[8][4][2];
+
+    const int dim = d->dim;
+    const int cells = dim * dim;
+    const float* base = (wave < 4 ? d->quads : d->hull_quads) + 8 * static_cast<size_t>(ci);
+    const int rot = wave & 3;
+    float corners[4][2];
+    for (int k = 0; k < 4; ++k) {
+        const int src = (rot + k) & 3;
+        corners[k][0] = base[2 * src];
+        corners[k][1] = base[2 * src + 1];
+    }
+    float h[9];
+    bool ok = homography_from_unit_quad(corners, h);
+    const int cx = lane % dim, cy = lane / dim;
+    float raw = 0.0f;
+    bool cell_ok = true;
+    if (ok && lane < cells) {
+        const float u = (static_cast<float>(cx) + 0.5f) / static_cast<float>(dim);
+        const float v = (static_cast<float>(cy) + 0.5f) / static_cast<float>(dim);
+        const float den = h[6] * u + h[7] * v + 1.0f;
+        if (fabsf(den) < 1e-9f) {
+            cell_ok = false;
+        } else {
+            const float x = (h[0] * u + h[1] * v + h[2]) / den;
+            const float y = (h[3] * u + h[4] * v + h[5]) / den;
+            raw = sample_bilinear(d->gray, d->width, d->height, d->stride, x - 0.0f, y - 0.0f);
+            if (raw < 0.0f) cell_ok = false;
+        }
+    }
+    const uint64_t bad_cells = __ballot(!cell_ok);
+    ok = ok && bad_cells == 0ull;
+    float mn = lane < cells ? raw : 255.0f;
+    float mx = lane < cells ? raw : 0.0f;
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        mn = fminf(mn, __shfl_xor(mn, off, kWave));
+        mx = fmaxf(mx, __shfl_xor(mx, off, kWave));
+    }
+    mn = fminf(255.0f, mn);
+    mx = fmaxf(0.0f, mx);
+    ok = ok && !(mx - mn < 10.0f);
+    const float thresh = 0.5f * (mn + mx);
+    const bool border = lane < cells && (cx == 0 || cy == 0 || cx == dim - 1 || cy == dim - 1);
+    const uint64_t bad_border = __ballot(border && raw >= thresh);
+    ok = ok && bad_border == 0ull;
+    const float diff = raw - thresh;
+    s_diff[wave][lane] = diff;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    float sharp = diff;
+    if (lane < cells && cx >= 1 && cx <= dim - 2 && cy >= 1 && cy <= dim - 2) {
+        const float* df = s_diff[wave];
+        const float lap = 4.0f * diff - df[lane - dim] - df[lane + dim] - df[lane - 1] - df[lane + 1];
+        sharp = diff + d->sharpening * lap;
+    }
+    s_sharp[wave][lane] = sharp;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+
+    // Code bits and class sums in the reference's row-major order.
+    uint64_t code = 0;
+    float margin = 0.0f;
+    if (lane == 0 && ok) {
+        float white_sum = 0.0f, black_sum = 0.0f;
+        int white_count = 0, black_count = 0;
+        for (int yy = 1; yy <= dim - 2; ++yy) {
+            for (int xx = 1; xx <= dim - 2; ++xx) {
+                const float dv = s_sharp[wave][yy * dim + xx];
+                code = (code << 1) | (dv > 0.0f ? 1ull : 0ull);
+                if (dv > 0.0f) {
+                    white_sum += dv;
+                    ++white_count;
+                } else {
+                    black_sum -= dv;
+                    ++black_count;
+                }
+            }
+        }
+        const float white_avg = white_sum / static_cast<float>(max(white_count, 1));
+        const float black_avg = black_sum / static_cast<float>(max(black_count, 1));
+        margin = fminf(white_avg, black_avg);
+    }
+    code = __shfl(code, 0, kWave);
+    // First codeword at the smallest distance, if within max_hamming.
+    int best_d = 1000, best_i = 1 << 30;
+    if (ok) {
+        const uint64_t* codes = d->codes;
+        for (int i = lane; i < d->code_count; i += kWave) {
+            const int dd = __popcll(codes[i] ^ code);
+            if (dd < best_d) {
+                best_d = dd;
+                best_i = i;
+            }
+        }
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const int od = __shfl_xor(best_d, off, kWave);
+        const int oi = __shfl_xor(best_i, off, kWave);
+        if (od < best_d || (od == best_d && oi < best_i)) {
+            best_d = od;
+            best_i = oi;
+        }
+    }
+    if (lane == 0) {
+        int id = -1;
+        if (ok && best_d <= d->max_hamming) {
+            if (!(margin < d->min_margin)) {
+                const float score = static_cast<float>(4 - best_d) * 1000.0f + margin;
+                if (score > -1.0f) id = best_i;
+            }
+        }
+        r_id[wave] = id;
+        r_ham[wave] = best_d;
+        r_margin[wave] = margin;
+        for (int k = 0; k < 4; ++k) {
+            r_corners[wave][k][0] = corners[k][0];
+            r_corners[wave][k][1] = corners[k][1];
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        int best = -1;
+        float best_score = -1.0f;
+        for (int variant = 0; variant < 2 && best < 0; ++variant) {
+            for (int rotation = 0; rotation < 4; ++rotation) {
+                const int r = variant * 4 + rotation;
+                if (r_id[r] < 0) continue;
+                const float score = static_cast<float>(4 - r_ham[r]) * 1000.0f + r_margin[r];
+                if (score > best_score) {
+                    best_score = score;
+                    best = r;
+                }
+            }
+        }
+        if (best < 0) {
+            res->id = -1;
+        } else {
+            res->id = r_id[best];
+            res->hamming = r_ham[best];
+            res->decision_margin = r_margin[best];
+            for (int k = 0; k < 4; ++k) {
+                res->corners[2 * k] = r_corners[best][k][0];
+                res->corners[2 * k + 1] = r_corners[best][k][1];
+            }
+            res->reserved = 0;
+            atomicAdd(&d->scal[S_DECODED], 1u);
+        }
+    }
+}
+
+// reference fill_homography (gpu_detector.cu:1739-1774), FP64 on the device.
+__device__ inline void fill_homography(const float* corners, float* H) {
+    const double x0 = corners[0], y0 = corners[1];
+    const double x1 = corners[2], y1 = corners[3];
+    const double x2 = corners[4], y2 = corners[5];
+    const double x3 = corners[6], y3 = corners[7];
+    const double dx1 = x1 - x2, dx2 = x3 - x2;
+    const double dx3 = x0 - x1 + x2 - x3;
+    const double dy1 = y1 - y2, dy2 = y3 - y2;
+    const double dy3 = y0 - y1 + y2 - y3;
+    double g = 0.0, hh = 0.0;
+    const double det = dx1 * dy2 - dx2 * dy1;
+    if (fabs(det) > 1e-12) {
+        g = (dx3 * dy2 - dx2 * dy3) / det;
+        hh = (dx1 * dy3 - dx3 * dy1) / det;
+    }
+    const double a = x1 - x0 + g * x1;
+    const double b = x3 - x0 + hh * x3;
+    const double dd = y1 - y0 + g * y1;
+    const double e = y3 - y0 + hh * y3;
+    const double w = g + hh + 2.0;
+    const double inv = (fabs(w) > 1e-12) ? 1.0 / w : 0.0;
+    H[0] = static_cast<float>(a * inv);
+    H[1] = static_cast<float>(b * inv);
+    H[2] = static_cast<float>((a + b + 2.0 * x0) * inv);
+    H[3] = static_cast<float>(dd * inv);
+    H[4] = static_cast<float>(e * inv);
+    H[5] = static_cast<float>((dd + e + 2.0 * y0) * inv);
+    H[6] = static_cast<float>(g * inv);
+    H[7] = static_cast<float>(hh * inv);
+    H[8] = static_cast<float>(w * inv);
+}
+
+__device__ inline void centre_side(const float* c, float* side, float* cx, float* cy) {
+    float s = 0.0f, x = 0.0f, y = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+        const int j = (k + 1) & 3;
+        s += hyp(c[j * 2] - c[k * 2], c[j * 2 + 1] - c[k * 2 + 1]);
+        x += c[k * 2];
+        y += c[k * 2 + 1];
+    }
+    *side = s * 0.25f;
+    *cx = x * 0.25f;
+    *cy = y * 0.25f;
+}
+
+// One workgroup per frame: decoded candidates in candidate order (the first
+// max_detections), de-duplicated as the reference's k_finalize (:1542-1601)
+// does it serially, then placed by a stable rank on id, each with its FP64
+// homography.
+__global__ __launch_bounds__(256) void k_finalize(Plan plan) {
+    const FrameDesc* d = plan.frames + blockIdx.x;
+    const int t = threadIdx.x;
+    const int maxd = d->max_detections;
+    const int ncand = static_cast<int>(d->scal[S_NCAND]);
+    __shared__ int list[256];
+    __shared__ int k_src[256];      // candidate index of each kept entry
+    __shared__ float k_cx[256], k_cy[256], k_margin[256];
+    __shared__ int k_id[256];
+    __shared__ int s_kept, s_dup, s_any_shared;
+    __shared__ uint32_t s_base;
+    if (t == 0) {
+        s_kept = 0;
+        s_any_shared = 0;
+        s_base = 0;
+    }
+    __syncthreads();
+    // Compaction in candidate order, chunks of 256.
+    for (int c0 = 0; c0 < ncand; c0 += blockDim.x) {
+        const int ci = c0 + t;
+        const bool valid = ci < ncand && d->dec[ci].id >= 0;
+        // block-wide exclusive scan of `valid`
+        __shared__ uint32_t wsum[4];
+        const int lane = t % kWave, wave = t / kWave;
+        const uint64_t bal = __ballot(valid);
+        const uint32_t before = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wave] = __popcll(bal);
+        __syncthreads();
+        uint32_t off = s_base;
+        for (int w = 0; w < wave; ++w) off += wsum[w];
+        if (valid) {
+            const uint32_t pos = off + before;
+            if (pos < static_cast<uint32_t>(maxd)) list[pos] = ci;
+        }
+        __syncthreads();
+        if (t == 0) s_base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+    const int n = min(static_cast<int>(s_base), maxd);
+    // Any id shared by two entries?  If not, de-duplication keeps every entry.
+    for (int i = t; i < n; i += blockDim.x) {
+        const int id = d->dec[list[i]].id;
+        for (int j = 0; j < n; ++j) {
+            if (j != i && d->dec[list[j]].id == id) {
+                s_any_shared = 1;
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    if (!s_any_shared) {
+        for (int i = t; i < n; i += blockDim.x) k_src[i] = list[i];
+        if (t == 0) s_kept = n;
+    } else {
+        for (int i = 0; i < n; ++i) {
+            const bd_detection& e = d->dec[list[i]];
+            float side, cx, cy;
+            centre_side(e.corners, &side, &cx, &cy);
+            if (t == 0) s_dup = 0x7fffffff;
+            __syncthreads();
+            const int kept = s_kept;
+            const float limit = 0.35f * side;
+            for (int k = t; k < kept; k += blockDim.x) {
+                if (k_id[k] == e.id && fabsf(k_cx[k] - cx) < limit && fabsf(k_cy[k] - cy) < limit) {
+                    atomicMin(&s_dup, k);
+                }
+            }
+            __syncthreads();
+            if (t == 0) {
+                const int dup = s_dup;
+                if (dup != 0x7fffffff) {
+                    if (e.decision_margin > k_margin[dup]) {
+                        float sd, ox, oy;
+                        centre_side(e.corners, &sd, &ox, &oy);
+                        k_src[dup] = list[i];
+                        k_id[dup] = e.id;
+                        k_margin[dup] = e.decision_margin;
+                        k_cx[dup] = ox;
+                        k_cy[dup] = oy;
+                    }
+                } else if (kept < maxd) {
+                    k_src[kept] = list[i];
+                    k_id[kept] = e.id;
+                    k_margin[kept] = e.decision_margin;
+                    k_cx[kept] = cx;
+                    k_cy[kept] = cy;
+                    s_kept = kept + 1;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    const int kept = s_kept;
+    // Stable placement by id (the reference's insertion sort).
+    for (int i = t; i < kept; i += blockDim.x) {
+        const int id = d->dec[k_src[i]].id;
+        int rank = 0;
+        for (int j = 0; j < kept; ++j) {
+            const int oj = d->dec[k_src[j]].id;
+            rank += (oj < id) || (oj == id && j < i);
+        }
+        bd_detection o = d->dec[k_src[i]];
+        o.reserved = 0;
+        fill_homography(o.corners, o.homography);
+        d->out[rank] = o;
+    }
+    if (t == 0) d->scal[S_COUNTS + 1] = static_cast<uint32_t>(kept);
+}
+
+}  // namespace
+
+hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
+    const int nf = plan.nframes;
+    switch (stage) {
+        case 0: hipLaunchKernelGGL(k_tile_stats, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
+        case 1: hipLaunchKernelGGL(k_frame_ref, dim3(nf), dim3(256), 0, stream, plan); break;
+        case 2: hipLaunchKernelGGL(k_binarize_runs, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
+        case 3: hipLaunchKernelGGL(k_merge, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
+        case 4: hipLaunchKernelGGL(k_stats, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
+        case 5: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
+        case 6: hipLaunchKernelGGL(k_select, dim3(nf), dim3(1024), 0, stream, plan); break;
+        case 7: hipLaunchKernelGGL(k_fit, dim3(plan.blocks[SH_CAND]), dim3(256), 0, stream, plan); break;
+        case 8: hipLaunchKernelGGL(k_decode, dim3(plan.blocks[SH_CAND]), dim3(512), 0, stream, plan); break;
+        case 9: hipLaunchKernelGGL(k_finalize, dim3(nf), dim3(256), 0, stream, plan); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pipeline(const Plan& plan, hipStream_t stream) {
+    for (int stage = 0; stage < kStageCount; ++stage) {
+        const hipError_t e = launch_stage(plan, stage, stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace bd

@@ -193,22 +193,14 @@ void render_tag_into_ref(uint8_t* image, int width, int height, const Family& fa
 }
 
 // detector/bench/scenes.hpp:131-148 (random_non_tag_code).  The reference
-// draws both halves from mix32 of the same state, so the candidate never
-// changes inside its loop; a bounded loop reports that case instead.
-bool random_non_tag_code_ref(uint32_t& rng, const Family& fam, uint64_t* out) {
+// draws both halves from mix32 of the same, never-advanced state, so its
+// retry loop either returns the first draw or spins without side effects;
+// compiled with optimisation it returns the first draw (the reference's own
+// exporter, built from its sources, does).  The port returns that draw.
+uint64_t random_non_tag_code_ref(uint32_t& rng, const Family& fam) {
     const int cells = fam.bits * fam.bits;
     const uint64_t mask = cells >= 64 ? ~0ull : ((1ull << cells) - 1ull);
-    const uint64_t code = ((static_cast<uint64_t>(mix32(rng)) << 32) ^ mix32(rng)) & mask;
-    int best = 1000;
-    for (int i = 0; i < fam.count; ++i) {
-        uint64_t r = fam.codes[i];
-        for (int k = 0; k < 4; ++k) {
-            best = std::min(best, popcount64(code ^ r));
-            r = rotate_code(r, fam.bits);
-        }
-    }
-    *out = code;
-    return best >= 8;
+    return ((static_cast<uint64_t>(mix32(rng)) << 32) ^ mix32(rng)) & mask;
 }
 
 // ---------------------------------------------------------------------------
@@ -496,7 +488,7 @@ __attribute__((visibility("default"))) uint64_t bds_rotate_code(uint64_t code, i
 
 // One scene of detector/bench/scenes.hpp:153-206 (make_scenes), by index.
 // Writes width*height pixels and the 8 ground-truth tags; returns the number
-// of tags or -1 if the reference's decoy loop would not terminate.
+// of tags.
 __attribute__((visibility("default"))) int32_t bds_bench_scene(int32_t width, int32_t height,
                                                               int32_t s, uint32_t seed,
                                                               uint8_t* image, bds_tag* tags) {
@@ -531,9 +523,7 @@ __attribute__((visibility("default"))) int32_t bds_bench_scene(int32_t width, in
         const double ds[2] = {base_size * 0.50, base_size * 0.40};
         for (int k = 0; k < 2; ++k) {
             const Mat3 pose = make_pose(dx[k], dy[k], ds[k], next_unit(rng) * 0.6, 0.0);
-            uint64_t code = 0;
-            if (!random_non_tag_code_ref(rng, fam, &code)) return -1;
-            render_tag_into_ref(image, width, height, fam, code, pose);
+            render_tag_into_ref(image, width, height, fam, random_non_tag_code_ref(rng, fam), pose);
         }
     }
     // detector/bench/scenes.hpp:80-89 (add_noise): delta = ((state >> 28) & 7) - 3.

@@ -115,7 +115,7 @@ def bench_scene(width, height, index, seed=6324):
     buf = (Tag * 8)()
     n = lib().bds_bench_scene(width, height, index, seed, img.ctypes.data, buf)
     if n < 0:
-        raise RuntimeError("reference decoy generator does not terminate for this scene")
+        raise ValueError("bds_bench_scene failed")
     return img, _tags(buf, n)
 
 

@@ -1,0 +1,142 @@
+"""Stage-by-stage parity of the HIP detector with the CPU oracle.
+
+Every stage is compared bit-exactly: tile statistics, references, ROI, the
+binary image, maximal runs, the component partition (canonical names),
+component area/bbox, the ordered candidate list, refined and hull quads,
+per-candidate decode results and the final detections with homographies.
+The GPU side is read through the C ABI (bd_batch_debug_copy).
+"""
+import numpy as np
+import pytest
+
+import bdcudatag
+import pyoracle
+import scenes
+from parity import compare_frame, gpu_stage_dump
+
+pytestmark = pytest.mark.gpu
+
+
+def run_both(img, family=0, **cfg):
+    H, W = img.shape
+    det = bdcudatag.Detector(family, W, H, **cfg)
+    dets = det.detect(img)
+    g = gpu_stage_dump(det.batch, 0)
+    o = pyoracle.Oracle(pyoracle.default_config(family, W, H, **cfg)).run(img)
+    return det, dets, g, o
+
+
+def check(img, family=0, **cfg):
+    det, dets, g, o = run_both(img, family, **cfg)
+    bad = compare_frame(g, o, dets, det.stats())
+    assert not bad, bad
+    return dets, o
+
+
+KA = dict(max_detections=32, min_decision_margin=2.0)
+
+
+@pytest.mark.parametrize("family,tag_id,size", [(0, 0, 170), (1, 0, 120), (0, 586, 150), (1, 29, 90)])
+def test_centered(family, tag_id, size):
+    pose = scenes.make_pose(320, 240, size, 0.0)
+    img = scenes.render_tag(family, tag_id, 640, 480, pose)
+    dets, _ = check(img, family, **KA)
+    assert [d["id"] for d in dets] == [tag_id]
+    assert dets[0]["hamming"] == 0
+
+
+@pytest.mark.parametrize("r", range(4))
+def test_rotations(r):
+    pose = scenes.make_pose(320, 240, 170, r * np.pi / 2)
+    img = scenes.render_tag(0, 0, 640, 480, pose)
+    dets, _ = check(img, 0, **KA)
+    assert [d["id"] for d in dets] == [0]
+
+
+@pytest.mark.parametrize("max_hamming,expect", [(-1, 1), (0, 0)])
+def test_flipped_bit(max_hamming, expect):
+    pose = scenes.make_pose(320, 240, 170, 0.15)
+    img = scenes.render_tag(0, 0, 640, 480, pose, flip_one_bit=True)
+    dets, _ = check(img, 0, max_hamming=max_hamming, **KA)
+    assert len(dets) == expect
+    if expect:
+        assert dets[0]["hamming"] == 1
+
+
+def test_small_and_perspective():
+    img = scenes.render_tag(0, 7, 640, 480, scenes.make_pose(320, 240, 38, 0.25))
+    dets, _ = check(img, 0, **KA)
+    assert [d["id"] for d in dets] == [7]
+    img = scenes.render_tag(0, 5, 640, 480, scenes.make_pose(320, 240, 170, 0.3, 0.00035))
+    dets, _ = check(img, 0, **KA)
+    assert [d["id"] for d in dets] == [5]
+
+
+def test_blank_frames():
+    for v in (255, 0, 128):
+        img = np.full((480, 640), v, dtype=np.uint8)
+        dets, _ = check(img, 0, **KA)
+        assert dets == []
+
+
+@pytest.mark.parametrize("res,index", [((640, 480), 0), ((640, 480), 3), ((1280, 800), 1),
+                                       ((1280, 800), 11), ((1920, 1080), 2)])
+def test_reference_bench_scenes(res, index):
+    img, tags = scenes.bench_scene(res[0], res[1], index)
+    dets, _ = check(img, 0, max_detections=32)
+    assert len(dets) >= 7
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 5])
+def test_workload_eight_tags_1280x800(seed):
+    img, tags = scenes.scene(0, 1280, 800, seed)
+    check(img, 0)
+
+
+@pytest.mark.parametrize("family", [0, 1])
+def test_dense_1920x1080(family):
+    img, tags = scenes.scene(1, 1920, 1080, 7)
+    det, dets, g, o = run_both(img, family)
+    bad = compare_frame(g, o, dets, det.stats())
+    assert not bad, bad
+    assert int(o["counts"][0]) > 0
+
+
+def test_dense_3840x2160_cap():
+    img, tags = scenes.scene(1, 3840, 2160, 3)
+    det, dets, g, o = run_both(img, 0)
+    bad = compare_frame(g, o, dets, det.stats())
+    assert not bad, bad
+
+
+def test_degraded():
+    img, tags = scenes.scene(3, 1280, 800, 4)
+    check(img, 0)
+
+
+@pytest.mark.parametrize("W,H", [(16, 16), (1001, 777), (333, 65), (65, 333)])
+def test_odd_sizes(W, H):
+    rng = np.random.default_rng(W * H)
+    img = scenes.render_tag(0, 11, W, H, scenes.make_pose(W * 0.5, H * 0.5, min(W, H) * 0.6, 0.2))
+    noise = rng.integers(-4, 5, size=img.shape)
+    img = np.clip(img.astype(int) + noise, 0, 255).astype(np.uint8)
+    check(img, 0, **KA)
+
+
+def test_candidate_cap_small():
+    img, tags = scenes.scene(1, 1920, 1080, 11)
+    det, dets, g, o = run_both(img, 0, max_candidates=16, max_detections=8)
+    bad = compare_frame(g, o, dets, det.stats())
+    assert not bad, bad
+
+
+def test_strided_host_frame():
+    img, tags = scenes.scene(0, 1280, 800, 9)
+    padded = np.zeros((800, 1344), dtype=np.uint8)
+    padded[:, :1280] = img
+    det = bdcudatag.Detector(0, 1280, 800)
+    a = det.detect(padded, stride=1344)
+    b = det.detect(img)
+    assert [d["id"] for d in a] == [d["id"] for d in b]
+    for x, y in zip(a, b):
+        assert np.array_equal(x["corners"], y["corners"])
