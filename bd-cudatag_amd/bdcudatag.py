@@ -93,7 +93,7 @@ SIGNATURES = {
     "bd_batch_frame_info": (_I, [_P, _I, ctypes.POINTER(_I)]),
     "bd_detector_batch": (_P, [_P]),
     "bd_batch_time_stages": (_I, [_P, _I, ctypes.POINTER(ctypes.c_float)]),
-    "bd_batch_replay": (_I, [_P, _I, ctypes.POINTER(ctypes.c_float)]),
+    "bd_batch_replay": (_I, [_P, _I, _I, ctypes.POINTER(ctypes.c_float)]),
 }
 
 _lib = None
@@ -234,9 +234,10 @@ class Batch:
         out["pipeline"] = float(ms[10])
         return out
 
-    def replay(self, reps: int) -> float:
+    def replay(self, reps: int, device: bool = False) -> float:
+        """Total device ms of `reps` back-to-back graph replays."""
         ms = ctypes.c_float()
-        rc = lib().bd_batch_replay(self.handle, reps, ctypes.byref(ms))
+        rc = lib().bd_batch_replay(self.handle, reps, 1 if device else 0, ctypes.byref(ms))
         if rc != BD_OK:
             raise RuntimeError(f"bd_batch_replay failed: {last_error()}")
         return float(ms.value)

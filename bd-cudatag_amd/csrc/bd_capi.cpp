@@ -348,13 +348,13 @@ int32_t bd_batch_time_stages(bd_batch* batch, int32_t reps, float* ms) {
     return BD_OK;
 }
 
-int32_t bd_batch_replay(bd_batch* batch, int32_t reps, float* ms) {
+int32_t bd_batch_replay(bd_batch* batch, int32_t reps, int32_t device_path, float* ms) {
     if (!batch || !batch->engine) {
         set_error("null batch");
         return BD_ERROR_INVALID_ARGUMENT;
     }
     std::string error;
-    if (!batch->engine->replay(reps, ms, &error)) {
+    if (!batch->engine->replay(reps, device_path != 0, ms, &error)) {
         set_error(error);
         return BD_ERROR_CUDA;
     }

@@ -499,7 +499,7 @@ bool Engine::time_stages(int reps, float* ms, std::string* error) {
     return true;
 }
 
-bool Engine::replay(int reps, float* ms, std::string* error) {
+bool Engine::replay(int reps, bool device_path, float* ms, std::string* error) {
     const std::lock_guard<std::mutex> lock(mutex_);
     if (reps < 1) {
         if (error) *error = "invalid repetition count";
@@ -507,7 +507,8 @@ bool Engine::replay(int reps, float* ms, std::string* error) {
     }
     hipError_t st = hipEventRecord(ev_start_, stream_);
     for (int r = 0; r < reps && st == hipSuccess; ++r) {
-        st = graph_host_ ? hipGraphLaunch(graph_host_, stream_) : enqueue(false);
+        hipGraphExec_t g = device_path ? graph_dev_ : graph_host_;
+        st = g ? hipGraphLaunch(g, stream_) : enqueue(device_path);
     }
     if (st == hipSuccess) st = hipEventRecord(ev_stop_, stream_);
     if (st == hipSuccess) st = hipEventSynchronize(ev_stop_);

@@ -63,9 +63,9 @@ public:
     // the frames already resident in the engine: HIP events on the engine
     // stream between consecutive kernels.  ms[0..9] per kernel, ms[10] whole.
     bool time_stages(int reps, float* ms, std::string* error);
-    // Replays the host-path graph `reps` times back to back (H2D of the
-    // staged frames, kernels, D2H) and returns the elapsed device time.
-    bool replay(int reps, float* ms, std::string* error);
+    // Replays the host-path or device-path graph `reps` times back to back
+    // and returns the elapsed device time.
+    bool replay(int reps, bool device_path, float* ms, std::string* error);
     hipStream_t stream() const { return stream_; }
 
 private:

@@ -221,11 +221,13 @@ BD_API bd_batch* bd_detector_batch(bd_detector* detector);
  * consecutive kernels on the detector's stream; ms[0..9] = mean time per
  * kernel (tile_stats, frame_ref, binarize_runs, merge, stats, collect,
  * select, fit, decode, finalize), ms[10] = mean whole pipeline.
- * bd_batch_replay: replays the host-path graph (H2D, kernels, D2H) `reps`
- * times back to back; *ms = total device time.
+ * bd_batch_replay: replays the captured graph `reps` times back to back
+ * without host synchronisation and returns the total device time in *ms;
+ * device_path = 0 replays the host path (H2D of the staged frames, kernels,
+ * D2H), 1 the device path on the frames of the last device call.
  */
 BD_API int32_t bd_batch_time_stages(bd_batch* batch, int32_t reps, float* ms);
-BD_API int32_t bd_batch_replay(bd_batch* batch, int32_t reps, float* ms);
+BD_API int32_t bd_batch_replay(bd_batch* batch, int32_t reps, int32_t device_path, float* ms);
 
 #ifdef __cplusplus
 }
