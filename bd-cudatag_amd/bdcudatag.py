@@ -25,8 +25,8 @@ STAGES = {
     "bits": 6, "row_count": 7, "runs": 8, "labels": 9, "area": 10, "bbox": 11,
     "candidates": 12, "quads": 13, "hull_quads": 14, "decode": 15, "tile_thresh": 16,
 }
-KERNELS = ["tile_stats", "frame_ref", "binarize_runs", "merge", "stats", "collect", "select",
-           "fit", "decode", "finalize"]
+KERNELS = ["tile_stats", "frame_ref", "binarize_runs", "merge", "flatten", "stats", "collect",
+           "select", "fit", "decode", "finalize"]
 
 
 class Config(ctypes.Structure):
@@ -63,6 +63,30 @@ class Frame(ctypes.Structure):
     ]
 
 
+class ServerClass(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("lanes", ctypes.c_int32),
+                ("lane_frames", ctypes.c_int32), ("queue_depth", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [("request", ctypes.c_int64), ("class_index", ctypes.c_int32),
+                ("count", ctypes.c_int32), ("written", ctypes.c_int32),
+                ("batch_frames", ctypes.c_int32), ("latency_ms", ctypes.c_double)]
+
+
+class StreamSpec(ctypes.Structure):
+    _fields_ = [("class_index", ctypes.c_int32), ("frame_count", ctypes.c_int32),
+                ("fps", ctypes.c_double), ("frames", ctypes.POINTER(ctypes.c_void_p)),
+                ("stride", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class ServeReport(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in ("offered", "completed", "dropped", "launches")] + \
+               [(k, ctypes.c_double) for k in ("seconds", "fps", "mpix_s", "p50_ms", "p99_ms",
+                                                "p999_ms", "max_ms", "mean_ms", "mean_batch")]
+
+
 # Every symbol the public header declares, with (restype, argtypes).
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
@@ -94,6 +118,15 @@ SIGNATURES = {
     "bd_detector_batch": (_P, [_P]),
     "bd_batch_time_stages": (_I, [_P, _I, ctypes.POINTER(ctypes.c_float)]),
     "bd_batch_replay": (_I, [_P, _I, _I, ctypes.POINTER(ctypes.c_float)]),
+    "bd_server_create": (_P, [_I, _I, ctypes.POINTER(ServerClass), ctypes.POINTER(Config),
+                              ctypes.POINTER(_I)]),
+    "bd_server_destroy": (None, [_P]),
+    "bd_server_submit": (ctypes.c_int64, [_P, _I, _P, _I]),
+    "bd_server_poll": (_I, [_P, ctypes.POINTER(ServerResult), _I, ctypes.POINTER(Detection), _I]),
+    "bd_server_wait_idle": (_I, [_P, _I]),
+    "bd_server_simulate": (_I, [_P, ctypes.POINTER(StreamSpec), _I, ctypes.c_double, ctypes.c_uint64,
+                                ctypes.POINTER(ServeReport)]),
+    "bd_server_last_error": (ctypes.c_char_p, []),
 }
 
 _lib = None
@@ -226,12 +259,12 @@ class Batch:
         return buf
 
     def time_stages(self, reps: int = 10) -> dict:
-        ms = (ctypes.c_float * 11)()
+        ms = (ctypes.c_float * (len(KERNELS) + 1))()
         rc = lib().bd_batch_time_stages(self.handle, reps, ms)
         if rc != BD_OK:
             raise RuntimeError(f"bd_batch_time_stages failed: {last_error()}")
         out = {k: float(ms[i]) for i, k in enumerate(KERNELS)}
-        out["pipeline"] = float(ms[10])
+        out["pipeline"] = float(ms[len(KERNELS)])
         return out
 
     def replay(self, reps: int, device: bool = False) -> float:
@@ -287,3 +320,71 @@ class Detector:
         s = Stats()
         lib().bd_detector_get_stats(self.handle, ctypes.byref(s))
         return _stats_dict(s)
+
+
+class Server:
+    """Continuous-batching server over device-resident frames.
+
+    classes: list of dicts {width, height, lanes, lane_frames, queue_depth}.
+    """
+
+    def __init__(self, family: int, classes, **config):
+        arr = (ServerClass * len(classes))()
+        for i, c in enumerate(classes):
+            arr[i] = ServerClass(c["width"], c["height"], c["lanes"], c["lane_frames"],
+                                 c["queue_depth"], 0)
+        self.classes = [dict(c) for c in classes]
+        status = ctypes.c_int32(0)
+        cfg = default_config(**config)
+        self.max_detections = max(1, min(cfg.max_detections, 256))
+        self.handle = lib().bd_server_create(family, len(classes), arr, ctypes.byref(cfg),
+                                             ctypes.byref(status))
+        if not self.handle:
+            msg = lib().bd_server_last_error()
+            raise RuntimeError(f"bd_server_create failed ({status.value}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().bd_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, class_index: int, ptr: int, stride: int) -> int:
+        """Request id (>= 0), or a negative status when dropped."""
+        return int(lib().bd_server_submit(self.handle, class_index, ptr, stride))
+
+    def poll(self, max_results: int = 256):
+        res = (ServerResult * max_results)()
+        md = self.max_detections
+        dets = (Detection * (max_results * md))()
+        n = lib().bd_server_poll(self.handle, res, max_results, dets, md)
+        if n < 0:
+            raise RuntimeError(f"bd_server_poll failed ({n})")
+        out = []
+        for i in range(n):
+            r = res[i]
+            out.append({"request": int(r.request), "class_index": int(r.class_index),
+                        "count": int(r.count), "batch_frames": int(r.batch_frames),
+                        "latency_ms": float(r.latency_ms),
+                        "detections": [detection_to_dict(dets[i * md + k]) for k in range(r.written)]})
+        return out
+
+    def wait_idle(self, timeout_ms: int = 10000) -> bool:
+        return lib().bd_server_wait_idle(self.handle, timeout_ms) == BD_OK
+
+    def simulate(self, streams, seconds: float, seed: int = 0) -> dict:
+        """streams: list of dicts {class_index, fps, frames: [device ptrs], stride}."""
+        arr = (StreamSpec * len(streams))()
+        keep = []
+        for i, st in enumerate(streams):
+            ptrs = (ctypes.c_void_p * len(st["frames"]))(*st["frames"])
+            keep.append(ptrs)
+            arr[i] = StreamSpec(st["class_index"], len(st["frames"]), float(st["fps"]),
+                                ctypes.cast(ptrs, ctypes.POINTER(ctypes.c_void_p)), st["stride"], 0)
+        rep = ServeReport()
+        rc = lib().bd_server_simulate(self.handle, arr, len(streams), seconds, seed, ctypes.byref(rep))
+        if rc != BD_OK:
+            raise RuntimeError(f"bd_server_simulate failed ({rc})")
+        return {k: getattr(rep, k) for k, _ in ServeReport._fields_}

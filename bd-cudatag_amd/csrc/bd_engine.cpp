@@ -122,7 +122,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         o.tthr = take(tiles);
         o.trlo = take(th * 4);
         o.trhi = take(th * 4);
-        o.roi = take(th * 12);
+        o.roi = take(th * 16);
         o.hist = take(320 * 4);
         o.bits = take(static_cast<size_t>(H) * w64 * 8);
         o.rowc = take(static_cast<size_t>(H) * 4);
@@ -178,6 +178,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         FrameDesc& d = desc_[f];
         const Off& o = off[f];
         d.gray = d_input_ + input_off_[f];
+        d.active = 1;
         d.width = W;
         d.height = H;
         d.stride = W;
@@ -221,6 +222,10 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         prefix[SH_ROW4 * (nf + 1) + f + 1] = prefix[SH_ROW4 * (nf + 1) + f] + (H + 3) / 4;
         prefix[SH_CAND * (nf + 1) + f + 1] = prefix[SH_CAND * (nf + 1) + f] + maxc[f];
         prefix[SH_FRAME * (nf + 1) + f + 1] = f + 1;
+        const int pairs = (d.tile_w + 1) / 2;
+        const int rows_per_wg = pairs >= 256 ? 1 : 256 / pairs;
+        prefix[SH_TSTAT * (nf + 1) + f + 1] =
+            prefix[SH_TSTAT * (nf + 1) + f] + (d.tile_h + rows_per_wg - 1) / rows_per_wg;
     }
     std::memcpy(h_desc_dev_, desc_.data(), desc_.size() * sizeof(FrameDesc));
 
@@ -325,6 +330,7 @@ int Engine::detect(const bd_frame* frames, bool device, bd_detection* out, int o
         for (int f = 0; f < nf; ++f) {
             h_desc_dev_[f].gray = frames[f].gray;
             h_desc_dev_[f].stride = frames[f].stride;
+            h_desc_dev_[f].active = 1;
         }
     } else {
         // Repack into pinned staging (gpu_detector.cu:2169-2176).
@@ -429,7 +435,7 @@ int64_t Engine::debug_copy(int index, int stage, void* dst, int64_t capacity, st
         case BD_STAGE_TILE_THRESH: src = d.tile_thresh; bytes = tiles; break;
         case BD_STAGE_HIST: src = d.hist; bytes = 320 * 4; break;
         case BD_STAGE_SCALARS: src = d.scal; bytes = S_COUNT * 4; break;
-        case BD_STAGE_ROW_ROI: src = d.row_roi; bytes = static_cast<size_t>(d.tile_h) * 12; break;
+        case BD_STAGE_ROW_ROI: src = d.row_roi; bytes = static_cast<size_t>(d.tile_h) * 16; break;
         case BD_STAGE_BITS: src = d.bits; bytes = static_cast<size_t>(d.height) * d.words64 * 8; break;
         case BD_STAGE_ROW_COUNT: src = d.row_count; bytes = static_cast<size_t>(d.height) * 4; break;
         case BD_STAGE_RUNS: src = d.runs; bytes = slots * 4; break;
@@ -453,6 +459,49 @@ int64_t Engine::debug_copy(int index, int stage, void* dst, int64_t capacity, st
         }
     }
     return static_cast<int64_t>(bytes);
+}
+
+bool Engine::launch_async(const uint8_t* const* gray, const int32_t* stride, int n,
+                          std::string* error) {
+    const std::lock_guard<std::mutex> lock(mutex_);
+    const int nf = frame_count();
+    if (n < 1 || n > nf) {
+        if (error) *error = "invalid frame count";
+        return false;
+    }
+    for (int f = 0; f < nf; ++f) {
+        h_desc_dev_[f].active = f < n ? 1 : 0;
+        if (f < n) {
+            h_desc_dev_[f].gray = gray[f];
+            h_desc_dev_[f].stride = stride[f];
+        }
+    }
+    hipError_t st = graph_dev_ ? hipGraphLaunch(graph_dev_, stream_) : enqueue(true);
+    if (st == hipSuccess) st = hipEventRecord(ev_stop_, stream_);
+    if (st != hipSuccess) {
+        if (error) *error = hip_error("async launch", st);
+        return false;
+    }
+    async_frames_ = n;
+    return true;
+}
+
+int Engine::query_async() {
+    const hipError_t st = hipEventQuery(ev_stop_);
+    if (st == hipSuccess) return 1;
+    if (st == hipErrorNotReady) return 0;
+    return -1;
+}
+
+int Engine::collect_async(int index, bd_detection* out, int capacity) const {
+    const uint32_t* c = h_scal_ + static_cast<size_t>(index) * S_COUNT + S_COUNTS;
+    const int found = std::min(static_cast<int>(c[1]), max_det_);
+    const int written = std::min(found, std::max(capacity, 0));
+    if (written > 0 && out) {
+        std::memcpy(out, h_out_ + static_cast<size_t>(index) * max_det_,
+                    static_cast<size_t>(written) * sizeof(bd_detection));
+    }
+    return found;
 }
 
 bool Engine::time_stages(int reps, float* ms, std::string* error) {

@@ -68,6 +68,15 @@ public:
     bool replay(int reps, bool device_path, float* ms, std::string* error);
     hipStream_t stream() const { return stream_; }
 
+    // Asynchronous device-path launch of positions [0, n) (the rest are
+    // inactive for this launch).  The caller must not launch again before
+    // query_async() reports completion.
+    bool launch_async(const uint8_t* const* gray, const int32_t* stride, int n, std::string* error);
+    // 1 when the last async launch has finished, 0 while running, -1 on error.
+    int query_async();
+    // Detections of position `index` of the finished async launch.
+    int collect_async(int index, bd_detection* out, int capacity) const;
+
 private:
     Engine() = default;
     bool init(const Params& params, const std::vector<int>& widths,
@@ -105,6 +114,7 @@ private:
     hipGraphExec_t graph_host_ = nullptr, graph_dev_ = nullptr;
 
     std::vector<FrameStats> stats_;
+    int async_frames_ = 0;
     mutable std::mutex mutex_;
 };
 

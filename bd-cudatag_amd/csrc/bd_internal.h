@@ -28,6 +28,9 @@ constexpr int kMaxSamples = 16;
 // Candidate keys sorted inside one workgroup's LDS before falling back to a
 // radix select over global memory.
 constexpr int kSortCap = 8192;
+// Runs of one 8-row band joined in LDS by k_binarize_runs; a band with more
+// runs is joined row by row in global memory by k_merge instead.
+constexpr int kBandRuns = 8192;
 
 // Per-frame scalar block (u32 each).  Accumulators are cleared by the kernel
 // that consumes them, so a replayed graph needs no memset nodes.
@@ -46,6 +49,7 @@ enum Scalar : int {
 
 struct FrameDesc {
     const uint8_t* gray;
+    int32_t active;  // 0: this position carries no frame in the current launch
     int32_t width, height, stride;
     int32_t tile_w, tile_h, words64, row_cap;
     int32_t max_candidates, max_detections;
@@ -59,7 +63,7 @@ struct FrameDesc {
     uint8_t* tile_thresh;
     int32_t* trow_lo;     // per tile row: leftmost active tile or tile_w
     int32_t* trow_hi;     // per tile row: rightmost active tile or -1
-    int32_t* row_roi;     // 3 * tile_h: active, start, end
+    int32_t* row_roi;     // 4 * tile_h: active, start, end, band joined locally
     uint32_t* hist;       // 256 span bins + 64 mean bins
     uint32_t* scal;       // S_COUNT scalars
     uint64_t* bits;       // height * words64
@@ -78,7 +82,7 @@ struct FrameDesc {
 
 // Prefix tables over frames: prefix[k][f] = first block of frame f for
 // kernel shape k; prefix[k][nframes] = total blocks.
-enum Shape : int { SH_TILE_ROW = 0, SH_ROW4 = 1, SH_CAND = 2, SH_FRAME = 3, SH_COUNT = 4 };
+enum Shape : int { SH_TILE_ROW = 0, SH_ROW4 = 1, SH_CAND = 2, SH_FRAME = 3, SH_TSTAT = 4, SH_COUNT = 5 };
 
 struct Plan {
     const FrameDesc* frames;
@@ -88,8 +92,8 @@ struct Plan {
 };
 
 // Kernels in pipeline order: tile_stats, frame_ref, binarize_runs, merge,
-// stats, collect, select, fit, decode, finalize.
-constexpr int kStageCount = 10;
+// flatten, stats, collect, select, fit, decode, finalize.
+constexpr int kStageCount = 11;
 
 // Enqueues one kernel (0..kStageCount-1) for every frame of the plan.
 hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream);

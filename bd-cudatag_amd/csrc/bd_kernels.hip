@@ -1,11 +1,12 @@
 // HIP kernels of the detection pipeline for gfx950 (CDNA4, wave64).
 //
-// Ten kernels, each with a grid that spans every frame of the plan:
+// Eleven kernels, each with a grid that spans every frame of the plan:
 //   k_tile_stats      8x8 tile min/max/mean, histograms, active extents   (a)
 //   k_frame_ref       quartile span + Otsu split, row ROI                 (a)
-//   k_binarize_runs   pooled threshold, 64-px ballot-free bit packing,
-//                     maximal black runs per row                          (b)
-//   k_merge           8-connected union of runs in adjacent rows          (b)
+//   k_binarize_runs   pooled threshold, 64-px bit packing, black runs per
+//                     row, band-local union-find in LDS                    (b)
+//   k_merge           8-connected union across band boundaries            (b)
+//   k_flatten         band roots resolved to component roots              (b)
 //   k_stats           root labels, component area and bounding box        (b)
 //   k_collect         tag-shaped component filter                         (c)
 //   k_select          largest-first candidate order (LDS bitonic sort)    (c)
@@ -91,98 +92,117 @@ __device__ inline void acc_word(uint32_t w, uint32_t& mn, uint32_t& mx, uint32_t
     sum = __builtin_amdgcn_sad_u8(w, 0u, sum);
 }
 
-// One workgroup per tile row (8 pixel rows).  Each thread owns a 16x8 pixel
-// block (two tiles) read as 16-byte rows, so a wave reads 1 KiB contiguous
-// per row.  reference gpu_detector.cu:104-166 (k_tile_stats, k_tile_histograms)
-// plus the per-row active extents that k_build_row_roi (:282-357) needs.
+// Tile rows per k_tile_stats workgroup: as many as fit 256 threads at one
+// 16-pixel-wide column pair per thread (shared with the host's grid sizing).
+__host__ __device__ inline int tile_rows_per_wg(int tile_w) {
+    const int pairs = (tile_w + 1) >> 1;
+    return pairs >= 256 ? 1 : 256 / pairs;
+}
+
+// Each thread owns a 16x8 pixel block (two tiles) read as 16-byte rows, so a
+// wave reads 1 KiB contiguous per pixel row; a workgroup covers
+// tile_rows_per_wg() tile rows.  reference gpu_detector.cu:104-166
+// (k_tile_stats, k_tile_histograms) plus the per-row active extents that
+// k_build_row_roi (:282-357) needs.
 __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
-    int ty;
-    const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
-    const int W = d->width, H = d->height, stride = d->stride, tw = d->tile_w;
+    int blk;
+    const FrameDesc* d = plan.frames + frame_of(plan, SH_TSTAT, blockIdx.x, &blk);
+    if (!d->active) return;
+    const int W = d->width, H = d->height, stride = d->stride, tw = d->tile_w, th = d->tile_h;
     const uint8_t* gray = d->gray;
+    const int t = threadIdx.x;
+    const int pairs = (tw + 1) >> 1;
+    const int R = tile_rows_per_wg(tw);
+    const int ty0 = blk * R;
     __shared__ uint32_t hs[256 + 64];
-    __shared__ int lo_s, hi_s;
-    for (int i = threadIdx.x; i < 320; i += blockDim.x) hs[i] = 0;
-    if (threadIdx.x == 0) {
-        lo_s = tw;
-        hi_s = -1;
+    __shared__ int lo_s[256], hi_s[256];
+    for (int i = t; i < 320; i += blockDim.x) hs[i] = 0;
+    if (t < R) {
+        lo_s[t] = tw;
+        hi_s[t] = -1;
     }
     __syncthreads();
 
-    const int y0 = ty * kTile;
-    const int y1 = min(y0 + kTile, H);
-    const int rows = y1 - y0;
-    const bool vec = ((reinterpret_cast<uintptr_t>(gray) | static_cast<uintptr_t>(stride)) & 15u) == 0;
-    const int pairs = (tw + 1) >> 1;
-    for (int p = threadIdx.x; p < pairs; p += blockDim.x) {
-        const int x0 = p * 16;
-        uint32_t mnA = 255, mxA = 0, sumA = 0, mnB = 255, mxB = 0, sumB = 0;
-        const int nA = min(8, W - x0);
-        const int nB = min(8, W - x0 - 8);
-        if (vec && x0 + 16 <= W) {
-            for (int y = y0; y < y1; ++y) {
-                const uint4 v = *reinterpret_cast<const uint4*>(gray + static_cast<size_t>(y) * stride + x0);
-                acc_word(v.x, mnA, mxA, sumA);
-                acc_word(v.y, mnA, mxA, sumA);
-                acc_word(v.z, mnB, mxB, sumB);
-                acc_word(v.w, mnB, mxB, sumB);
-            }
-        } else {
-            for (int y = y0; y < y1; ++y) {
-                const uint8_t* row = gray + static_cast<size_t>(y) * stride + x0;
-                for (int x = 0; x < nA; ++x) {
-                    const uint32_t v = row[x];
-                    mnA = min(mnA, v);
-                    mxA = max(mxA, v);
-                    sumA += v;
+    const int sub = pairs >= 256 ? 0 : t / pairs;
+    const int p0 = pairs >= 256 ? t : t - sub * pairs;
+    const int pstep = pairs >= 256 ? 256 : pairs;
+    const int ty = ty0 + sub;
+    if (sub < R && ty < th) {
+        const int y0 = ty * kTile;
+        const int y1 = min(y0 + kTile, H);
+        const int rows = y1 - y0;
+        const bool vec = ((reinterpret_cast<uintptr_t>(gray) | static_cast<uintptr_t>(stride)) & 15u) == 0;
+        for (int p = p0; p < pairs; p += pstep) {
+            const int x0 = p * 16;
+            uint32_t mnA = 255, mxA = 0, sumA = 0, mnB = 255, mxB = 0, sumB = 0;
+            const int nA = min(8, W - x0);
+            const int nB = min(8, W - x0 - 8);
+            if (vec && x0 + 16 <= W) {
+                for (int y = y0; y < y1; ++y) {
+                    const uint4 v = *reinterpret_cast<const uint4*>(gray + static_cast<size_t>(y) * stride + x0);
+                    acc_word(v.x, mnA, mxA, sumA);
+                    acc_word(v.y, mnA, mxA, sumA);
+                    acc_word(v.z, mnB, mxB, sumB);
+                    acc_word(v.w, mnB, mxB, sumB);
                 }
-                for (int x = 0; x < nB; ++x) {
-                    const uint32_t v = row[8 + x];
-                    mnB = min(mnB, v);
-                    mxB = max(mxB, v);
-                    sumB += v;
+            } else {
+                for (int y = y0; y < y1; ++y) {
+                    const uint8_t* row = gray + static_cast<size_t>(y) * stride + x0;
+                    for (int x = 0; x < nA; ++x) {
+                        const uint32_t v = row[x];
+                        mnA = min(mnA, v);
+                        mxA = max(mxA, v);
+                        sumA += v;
+                    }
+                    for (int x = 0; x < nB; ++x) {
+                        const uint32_t v = row[8 + x];
+                        mnB = min(mnB, v);
+                        mxB = max(mxB, v);
+                        sumB += v;
+                    }
                 }
             }
-        }
-        const int ta = 2 * p;
-        const uint32_t cA = static_cast<uint32_t>(nA * rows);
-        const uint32_t meanA = (sumA + cA / 2) / cA;
-        const size_t ia = static_cast<size_t>(ty) * tw + ta;
-        d->tile_min[ia] = static_cast<uint8_t>(mnA);
-        d->tile_max[ia] = static_cast<uint8_t>(mxA);
-        d->tile_mean[ia] = static_cast<uint8_t>(meanA);
-        atomicAdd(&hs[(mxA - mnA) & 0xffu], 1u);
-        atomicAdd(&hs[256 + (meanA >> 2)], 1u);
-        int lo = tw, hi = -1;
-        if (static_cast<int>(mxA - mnA) >= kMinTileContrast) lo = hi = ta;
-        if (nB > 0) {
-            const uint32_t cB = static_cast<uint32_t>(nB * rows);
-            const uint32_t meanB = (sumB + cB / 2) / cB;
-            d->tile_min[ia + 1] = static_cast<uint8_t>(mnB);
-            d->tile_max[ia + 1] = static_cast<uint8_t>(mxB);
-            d->tile_mean[ia + 1] = static_cast<uint8_t>(meanB);
-            atomicAdd(&hs[(mxB - mnB) & 0xffu], 1u);
-            atomicAdd(&hs[256 + (meanB >> 2)], 1u);
-            if (static_cast<int>(mxB - mnB) >= kMinTileContrast) {
-                lo = min(lo, ta + 1);
-                hi = ta + 1;
+            const int ta = 2 * p;
+            const uint32_t cA = static_cast<uint32_t>(nA * rows);
+            const uint32_t meanA = (sumA + cA / 2) / cA;
+            const size_t ia = static_cast<size_t>(ty) * tw + ta;
+            d->tile_min[ia] = static_cast<uint8_t>(mnA);
+            d->tile_max[ia] = static_cast<uint8_t>(mxA);
+            d->tile_mean[ia] = static_cast<uint8_t>(meanA);
+            atomicAdd(&hs[(mxA - mnA) & 0xffu], 1u);
+            atomicAdd(&hs[256 + (meanA >> 2)], 1u);
+            int lo = tw, hi = -1;
+            if (static_cast<int>(mxA - mnA) >= kMinTileContrast) lo = hi = ta;
+            if (nB > 0) {
+                const uint32_t cB = static_cast<uint32_t>(nB * rows);
+                const uint32_t meanB = (sumB + cB / 2) / cB;
+                d->tile_min[ia + 1] = static_cast<uint8_t>(mnB);
+                d->tile_max[ia + 1] = static_cast<uint8_t>(mxB);
+                d->tile_mean[ia + 1] = static_cast<uint8_t>(meanB);
+                atomicAdd(&hs[(mxB - mnB) & 0xffu], 1u);
+                atomicAdd(&hs[256 + (meanB >> 2)], 1u);
+                if (static_cast<int>(mxB - mnB) >= kMinTileContrast) {
+                    lo = min(lo, ta + 1);
+                    hi = ta + 1;
+                }
             }
-        }
-        if (hi >= 0) {
-            atomicMin(&lo_s, lo);
-            atomicMax(&hi_s, hi);
+            if (hi >= 0) {
+                atomicMin(&lo_s[sub], lo);
+                atomicMax(&hi_s[sub], hi);
+            }
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < 320; i += blockDim.x) {
+    for (int i = t; i < 320; i += blockDim.x) {
         if (hs[i]) atomicAdd(&d->hist[i], hs[i]);
     }
-    if (threadIdx.x == 0) {
-        d->trow_lo[ty] = lo_s;
-        d->trow_hi[ty] = hi_s;
-        if (hi_s >= 0) {
-            atomicMin(&d->scal[S_TOP], static_cast<uint32_t>(ty));
-            atomicMax(&d->scal[S_BOTTOM1], static_cast<uint32_t>(ty + 1));
+    if (t < R && ty0 + t < th) {
+        const int row = ty0 + t;
+        d->trow_lo[row] = lo_s[t];
+        d->trow_hi[row] = hi_s[t];
+        if (hi_s[t] >= 0) {
+            atomicMin(&d->scal[S_TOP], static_cast<uint32_t>(row));
+            atomicMax(&d->scal[S_BOTTOM1], static_cast<uint32_t>(row + 1));
         }
     }
 }
@@ -192,6 +212,7 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
 // reset of the accumulators the next frame will use.
 __global__ __launch_bounds__(256) void k_frame_ref(Plan plan) {
     const FrameDesc* d = plan.frames + blockIdx.x;
+    if (!d->active) return;
     const int t = threadIdx.x;
     const int tw = d->tile_w, th = d->tile_h;
     __shared__ uint32_t span[256];
@@ -299,7 +320,8 @@ __global__ __launch_bounds__(256) void k_frame_ref(Plan plan) {
     // none (k_build_row_roi :314-356).
     const int top = s_top, bottom = s_bottom;
     for (int ty = t; ty < th; ty += blockDim.x) {
-        int32_t* roi = d->row_roi + 3 * ty;
+        int32_t* roi = d->row_roi + 4 * ty;
+        roi[3] = 0;
         if (bottom < 0 || ty < top - 1 || ty > bottom + 1) {
             roi[0] = 0;
             roi[1] = 0;
@@ -340,26 +362,71 @@ __global__ __launch_bounds__(256) void k_frame_ref(Plan plan) {
 // (b) binarization and runs
 // ---------------------------------------------------------------------------
 
-// One workgroup per tile row.  The pooled thresholds of the row's tiles are
-// built once in LDS; each thread packs 64 pixels of one pixel row into a u64
-// (1 = white) from two 16-byte loads per 32-pixel group; then one wave per
-// pixel row turns the masks into maximal black runs (start/end bit masks,
-// popcount, wave prefix sums), so no fragment joining is needed later.
-// reference gpu_detector.cu:237-276 (k_tile_threshold), :362-391
-// (k_binarize), :427-504 (k_find_runs).
+// Run words are x0 | x1 << 16; bit 31 marks a band root (see
+// k_binarize_runs).  x0, x1 < 8192.
+__device__ inline int run_x0(uint32_t r) { return static_cast<int>(r & 0xffffu); }
+__device__ inline int run_x1(uint32_t r) { return static_cast<int>((r >> 16) & 0x7fffu); }
+constexpr uint32_t kRootFlag = 0x80000000u;
+
+__device__ inline uint32_t lds_find(const uint32_t* lab, uint32_t x) {
+    uint32_t p = lab[x];
+    while (p != x) {
+        x = p;
+        p = lab[x];
+    }
+    return x;
+}
+
+__device__ inline void lds_union(uint32_t* lab, uint32_t a, uint32_t b) {
+    for (;;) {
+        a = lds_find(lab, a);
+        b = lds_find(lab, b);
+        if (a == b) return;
+        if (a > b) {
+            const uint32_t tmp = a;
+            a = b;
+            b = tmp;
+        }
+        const uint32_t old = atomicMin(&lab[b], a);
+        if (old == b) return;
+        b = old;
+    }
+}
+
+// One workgroup per tile row (8 pixel rows).
+//  1. Pooled 3x3 thresholds of the row's tiles in LDS (k_tile_threshold,
+//     reference gpu_detector.cu:237-276).
+//  2. Each thread packs 64 pixels of one pixel row into a u64 (1 = white)
+//     from two 16-byte loads per 32-pixel group, with the reference's
+//     per-32-pixel ROI test (k_binarize :362-391).
+//  3. One wave per pixel row turns the masks into black runs with start/end
+//     bit masks, popcounts and wave prefix sums: maximal runs, except in
+//     image row 0 where runs split at every 32-pixel word as the reference's
+//     fragments there are never rejoined (k_find_runs :427-504 emits
+//     fragments; k_merge_runs :514 skips row 0, so its same-row joins never
+//     run for that row).
+//  4. Band-local union-find in LDS over the 8 rows, processed top to bottom
+//     with each row flattened before the next, so every run ends one hop
+//     from its band root; band roots are flagged in the run word.  A band
+//     with more runs than the LDS buffer leaves every run a root and lets
+//     k_merge join all of its rows.
 __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
     int ty;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
+    if (!d->active) return;
     const int W = d->width, H = d->height, tw = d->tile_w, th = d->tile_h;
     const int w64 = d->words64, row_cap = d->row_cap, stride = d->stride;
     const int y0 = ty * kTile;
     const int nrows = min(kTile, H - y0);
     const int t = threadIdx.x;
-    const int32_t* roi = d->row_roi + 3 * ty;
+    int32_t* roi = d->row_roi + 4 * ty;
     const int row_on = roi[0], rs = roi[1], re = roi[2];
 
     __shared__ uint8_t thr[1024];
     __shared__ uint64_t masks[kTile][128];
+    __shared__ uint32_t s_off[kTile + 1];
+    __shared__ uint32_t b_run[kBandRuns];
+    __shared__ uint32_t b_lab[kBandRuns];
 
     // Pooled 3x3 thresholds for this tile row (k_tile_threshold).
     const int ref0 = static_cast<int>(d->scal[S_REF0]);
@@ -437,38 +504,84 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
     __syncthreads();
 
     const int wave = t / kWave, lane = t % kWave;
+    const int nwaves = static_cast<int>(blockDim.x) / kWave;
+    // Start/end masks of word w of band row r.
+    auto edges = [&](int r, int w, uint64_t* starts, uint64_t* ends) {
+        const uint64_t m = masks[r][w];
+        uint64_t prev = w > 0 ? (masks[r][w - 1] >> 63) : 1ull;
+        uint64_t next = w + 1 < w64 ? (masks[r][w + 1] & 1ull) : 1ull;
+        if (y0 + r == 0) {
+            // Row 0: a fragment starts at bit 0 of every 32-px word and ends
+            // at its last bit (reference k_find_runs :454, :485-487).
+            prev = 1ull | (1ull << 32);
+            *ends = ~m & ((m >> 1) | (1ull << 63) | (1ull << 31));
+        } else {
+            *ends = ~m & ((m >> 1) | (next << 63));
+        }
+        *starts = ~m & ((m << 1) | prev);
+    };
+
+    // Pass A: runs per row, and the reference's fragment count (:454-468).
+    uint32_t block_frags = 0;
+    for (int r = wave; r < nrows; r += nwaves) {
+        uint32_t n = 0, frags = 0;
+        for (int w = lane; w < w64; w += kWave) {
+            uint64_t st, en;
+            edges(r, w, &st, &en);
+            n += __popcll(st);
+            const uint64_t m = masks[r][w];
+            for (int h = 0; h < 2; ++h) {
+                if ((2 * w + h) * 32 >= W) break;
+                const uint32_t m32 = static_cast<uint32_t>(m >> (32 * h));
+                frags += __popc(~m32 & ((m32 << 1) | 1u));
+            }
+        }
+        n = wave_sum(n);
+        frags = wave_sum(frags);
+        if (lane == 0) {
+            s_off[r + 1] = n;
+            d->row_count[y0 + r] = n;
+        }
+        block_frags += frags;
+    }
+    if (t == 0) s_off[0] = 0;
+    __syncthreads();
+    if (t == 0) {
+        for (int r = 0; r < nrows; ++r) s_off[r + 1] += s_off[r];
+    }
+    __syncthreads();
+    const uint32_t nb = s_off[nrows];
+    const bool local = nb <= static_cast<uint32_t>(kBandRuns);
+    if (lane == 0 && block_frags) atomicAdd(&d->scal[S_COUNTS + 2], block_frags);
+    if (t == 0 && nb) atomicAdd(&d->scal[S_MAXRUNS], nb);
+
+    // Pass B: write the runs (LDS when the band fits, else global with every
+    // run a root) and initialise the component statistics.
     uint16_t* runs16 = reinterpret_cast<uint16_t*>(d->runs);
-    uint32_t block_frags = 0, block_runs = 0;
-    for (int r = wave; r < nrows; r += static_cast<int>(blockDim.x) / kWave) {
+    uint16_t* lds16 = reinterpret_cast<uint16_t*>(b_run);
+    for (int r = wave; r < nrows; r += nwaves) {
         const int y = y0 + r;
         const uint32_t base = static_cast<uint32_t>(y) * static_cast<uint32_t>(row_cap);
-        uint32_t carry_s = 0, carry_e = 0, frags = 0;
+        const uint32_t lbase = s_off[r];
+        uint32_t carry_s = 0, carry_e = 0;
         for (int w0 = 0; w0 < w64; w0 += kWave) {
             const int w = w0 + lane;
             uint64_t starts = 0, ends = 0;
-            if (w < w64) {
-                const uint64_t m = masks[r][w];
-                const uint64_t prev = w > 0 ? (masks[r][w - 1] >> 63) : 1ull;
-                const uint64_t next = w + 1 < w64 ? (masks[r][w + 1] & 1ull) : 1ull;
-                starts = ~m & ((m << 1) | prev);
-                ends = ~m & ((m >> 1) | (next << 63));
-                // Fragment count of the reference's 32-px words (:454-468),
-                // kept for the runs statistic.
-                for (int h = 0; h < 2; ++h) {
-                    if ((2 * w + h) * 32 >= W) break;
-                    const uint32_t m32 = static_cast<uint32_t>(m >> (32 * h));
-                    frags += __popc(~m32 & ((m32 << 1) | 1u));
-                }
-            }
+            if (w < w64) edges(r, w, &starts, &ends);
             uint32_t tot_s, tot_e;
             uint32_t ks = carry_s + wave_excl_scan(__popcll(starts), lane, &tot_s);
             uint32_t ke = carry_e + wave_excl_scan(__popcll(ends), lane, &tot_e);
             while (starts) {
                 const int b = __ffsll(static_cast<unsigned long long>(starts)) - 1;
                 starts &= starts - 1;
-                const uint32_t slot = base + ks++;
-                runs16[2 * slot] = static_cast<uint16_t>(w * 64 + b);
-                d->label[slot] = slot;
+                const uint32_t k = ks++;
+                const uint32_t slot = base + k;
+                if (local) {
+                    lds16[2 * (lbase + k)] = static_cast<uint16_t>(w * 64 + b);
+                } else {
+                    runs16[2 * slot] = static_cast<uint16_t>(w * 64 + b);
+                    d->label[slot] = slot;
+                }
                 d->area[slot] = 0;
                 uint4* bb = reinterpret_cast<uint4*>(d->bbox + 4 * static_cast<size_t>(slot));
                 *bb = make_uint4(0xffffffffu, 0xffffffffu, 0u, 0u);
@@ -476,19 +589,62 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
             while (ends) {
                 const int b = __ffsll(static_cast<unsigned long long>(ends)) - 1;
                 ends &= ends - 1;
-                runs16[2 * (base + ke++) + 1] = static_cast<uint16_t>(w * 64 + b);
+                const uint32_t k = ke++;
+                if (local) {
+                    lds16[2 * (lbase + k) + 1] = static_cast<uint16_t>(w * 64 + b);
+                } else {
+                    runs16[2 * (base + k) + 1] = static_cast<uint16_t>((w * 64 + b) | 0x8000);
+                }
             }
             carry_s += tot_s;
             carry_e += tot_e;
         }
-        frags = wave_sum(frags);
-        if (lane == 0) d->row_count[y] = carry_s;
-        block_frags += frags;
-        block_runs += carry_s;
     }
-    if (lane == 0 && block_runs) {
-        atomicAdd(&d->scal[S_COUNTS + 2], block_frags);
-        atomicAdd(&d->scal[S_MAXRUNS], block_runs);
+    if (t == 0) roi[3] = local ? 1 : 0;
+    if (!local) return;
+    __syncthreads();
+
+    // Band-local union-find, rows in order (k_merge_runs :525-556 on the band).
+    for (uint32_t i = t; i < nb; i += blockDim.x) b_lab[i] = i;
+    __syncthreads();
+    for (int r = 1; r < nrows; ++r) {
+        const uint32_t beg = s_off[r], end = s_off[r + 1], pbeg = s_off[r - 1];
+        if (beg < end && pbeg < beg) {
+            for (uint32_t i = beg + t; i < end; i += blockDim.x) {
+                const uint32_t run = b_run[i];
+                const int x0 = run_x0(run), x1 = run_x1(run);
+                int lo = static_cast<int>(pbeg), hi = static_cast<int>(beg) - 1, first = static_cast<int>(beg);
+                while (lo <= hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (run_x1(b_run[mid]) >= x0 - 1) {
+                        first = mid;
+                        hi = mid - 1;
+                    } else {
+                        lo = mid + 1;
+                    }
+                }
+                for (int k = first; k < static_cast<int>(beg); ++k) {
+                    if (run_x0(b_run[k]) > x1 + 1) break;
+                    lds_union(b_lab, i, static_cast<uint32_t>(k));
+                }
+            }
+            __syncthreads();
+            for (uint32_t i = beg + t; i < end; i += blockDim.x) b_lab[i] = lds_find(b_lab, i);
+        }
+        __syncthreads();
+    }
+    for (uint32_t i = t; i < nb; i += blockDim.x) b_lab[i] = lds_find(b_lab, i);
+    __syncthreads();
+    for (uint32_t i = t; i < nb; i += blockDim.x) {
+        int r = 0;
+        while (i >= s_off[r + 1]) ++r;
+        const uint32_t slot = static_cast<uint32_t>(y0 + r) * static_cast<uint32_t>(row_cap) + (i - s_off[r]);
+        const uint32_t root = b_lab[i];
+        int rr = 0;
+        while (root >= s_off[rr + 1]) ++rr;
+        const uint32_t rslot = static_cast<uint32_t>(y0 + rr) * static_cast<uint32_t>(row_cap) + (root - s_off[rr]);
+        d->runs[slot] = b_run[i] | (root == i ? kRootFlag : 0u);
+        d->label[slot] = rslot;
     }
 }
 
@@ -526,15 +682,19 @@ __device__ inline void uf_union(uint32_t* label, uint32_t a, uint32_t b) {
     }
 }
 
-// One wave per row y: every run unions with the runs of row y-1 that touch
-// [x0-1, x1+1] (8-connectivity; reference k_merge_runs :509-557).  Maximal
-// runs of one row never touch each other, so no same-row joins are needed.
+// One wave per row y that starts a band (or lies in a band that was not
+// joined locally): every run unions with the runs of row y-1 that touch
+// [x0-1, x1+1] (8-connectivity; reference k_merge_runs :509-557).  Runs of
+// one row never need same-row joins: they are maximal, or in row 0 they are
+// fragments the reference leaves unjoined.
 __global__ __launch_bounds__(256) void k_merge(Plan plan) {
     int blk;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_ROW4, blockIdx.x, &blk);
+    if (!d->active) return;
     const int y = blk * 4 + static_cast<int>(threadIdx.x) / kWave;
     const int lane = threadIdx.x % kWave;
     if (y < 1 || y >= d->height) return;
+    if ((y & (kTile - 1)) != 0 && d->row_roi[4 * (y >> 3) + 3]) return;  // joined in its band
     const int n = static_cast<int>(d->row_count[y]);
     const int pn = static_cast<int>(d->row_count[y - 1]);
     if (n == 0 || pn == 0) return;
@@ -545,12 +705,12 @@ __global__ __launch_bounds__(256) void k_merge(Plan plan) {
     const uint32_t prev_base = static_cast<uint32_t>(y - 1) * row_cap;
     for (int i = lane; i < n; i += kWave) {
         const uint32_t run = cur[i];
-        const int x0 = static_cast<int>(run & 0xffffu);
-        const int x1 = static_cast<int>(run >> 16);
+        const int x0 = run_x0(run);
+        const int x1 = run_x1(run);
         int lo = 0, hi = pn - 1, first = pn;
         while (lo <= hi) {
             const int mid = (lo + hi) >> 1;
-            if (static_cast<int>(prev[mid] >> 16) >= x0 - 1) {
+            if (run_x1(prev[mid]) >= x0 - 1) {
                 first = mid;
                 hi = mid - 1;
             } else {
@@ -558,33 +718,114 @@ __global__ __launch_bounds__(256) void k_merge(Plan plan) {
             }
         }
         for (int k = first; k < pn; ++k) {
-            if (static_cast<int>(prev[k] & 0xffffu) > x1 + 1) break;
+            if (run_x0(prev[k]) > x1 + 1) break;
             uf_union(d->label, cur_base + i, prev_base + k);
         }
     }
 }
 
-// Every run learns its root (written back, so later stages read one hop) and
-// adds its length and extent to the root (reference k_region_stats :562-586).
-__global__ __launch_bounds__(256) void k_stats(Plan plan) {
+// Band roots (flagged runs) walk to their component root; only band roots
+// carry global links, so this touches a small fraction of the runs.  Paths
+// are split with atomicMin, which can only lower a label towards the root,
+// so a concurrent walker never undoes another's final write.
+__global__ __launch_bounds__(256) void k_flatten(Plan plan) {
     int blk;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_ROW4, blockIdx.x, &blk);
+    if (!d->active) return;
     const int y = blk * 4 + static_cast<int>(threadIdx.x) / kWave;
     const int lane = threadIdx.x % kWave;
     if (y >= d->height) return;
     const int n = static_cast<int>(d->row_count[y]);
     const uint32_t base = static_cast<uint32_t>(y) * static_cast<uint32_t>(d->row_cap);
+    uint32_t* label = d->label;
     for (int i = lane; i < n; i += kWave) {
         const uint32_t slot = base + static_cast<uint32_t>(i);
-        const uint32_t root = uf_root(d->label, slot);
-        d->label[slot] = root;
-        const uint32_t run = d->runs[slot];
-        const uint32_t x0 = run & 0xffffu, x1 = run >> 16;
-        atomicAdd(&d->area[root], x1 - x0 + 1);
-        atomicMin(&d->bbox[4 * static_cast<size_t>(root) + 0], x0);
-        atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 2], x1);
-        atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 3], static_cast<uint32_t>(y));
-        if (root == slot) d->bbox[4 * static_cast<size_t>(root) + 1] = static_cast<uint32_t>(y);
+        if (!(d->runs[slot] & kRootFlag)) continue;
+        uint32_t x = slot;
+        uint32_t p = label[x];
+        while (p != x) {
+            const uint32_t gp = label[p];
+            if (gp != p) atomicMin(&label[x], gp);
+            x = p;
+            p = gp;
+        }
+        atomicMin(&label[slot], x);
+    }
+}
+
+constexpr int kStatSlots = 512;
+constexpr uint32_t kEmpty = 0xffffffffu;
+
+// One workgroup per band of 8 rows.  Every run takes its root in two hops
+// (run -> band root -> root; a band root's label is already final, so the
+// write-back races with no reader) and adds its length and extent into an
+// LDS table keyed by root; the table is then flushed with one set of global
+// atomics per (band, root), so a component with thousands of runs does not
+// serialise on its root's counters (reference k_region_stats :562-586 does
+// one atomic per run).
+__global__ __launch_bounds__(256) void k_stats(Plan plan) {
+    int ty;
+    const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
+    if (!d->active) return;
+    if (!d->row_roi[4 * ty]) return;
+    const int H = d->height;
+    const int y0 = ty * kTile;
+    const int nrows = min(kTile, H - y0);
+    const int t = threadIdx.x;
+    const int wave = t / kWave, lane = t % kWave;
+    __shared__ uint32_t hkey[kStatSlots], harea[kStatSlots], hx0[kStatSlots], hx1[kStatSlots], hy1[kStatSlots];
+    for (int e = t; e < kStatSlots; e += blockDim.x) {
+        hkey[e] = kEmpty;
+        harea[e] = 0;
+        hx0[e] = 0xffffffffu;
+        hx1[e] = 0;
+        hy1[e] = 0;
+    }
+    __syncthreads();
+    uint32_t* label = d->label;
+    for (int r = wave; r < nrows; r += static_cast<int>(blockDim.x) / kWave) {
+        const int y = y0 + r;
+        const int n = static_cast<int>(d->row_count[y]);
+        const uint32_t base = static_cast<uint32_t>(y) * static_cast<uint32_t>(d->row_cap);
+        for (int i = lane; i < n; i += kWave) {
+            const uint32_t slot = base + static_cast<uint32_t>(i);
+            const uint32_t root = label[label[slot]];
+            label[slot] = root;
+            if (root == slot) d->bbox[4 * static_cast<size_t>(root) + 1] = static_cast<uint32_t>(y);
+            const uint32_t run = d->runs[slot];
+            const uint32_t x0 = static_cast<uint32_t>(run_x0(run)), x1 = static_cast<uint32_t>(run_x1(run));
+            int found = -1;
+            const uint32_t h = (root * 2654435761u) >> 23;
+            for (int probe = 0; probe < 32; ++probe) {
+                const int e = static_cast<int>((h + static_cast<uint32_t>(probe)) & (kStatSlots - 1));
+                uint32_t k = hkey[e];
+                if (k == kEmpty) k = atomicCAS(&hkey[e], kEmpty, root);
+                if (k == kEmpty || k == root) {
+                    found = e;
+                    break;
+                }
+            }
+            if (found >= 0) {
+                atomicAdd(&harea[found], x1 - x0 + 1);
+                atomicMin(&hx0[found], x0);
+                atomicMax(&hx1[found], x1);
+                atomicMax(&hy1[found], static_cast<uint32_t>(y));
+            } else {
+                atomicAdd(&d->area[root], x1 - x0 + 1);
+                atomicMin(&d->bbox[4 * static_cast<size_t>(root) + 0], x0);
+                atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 2], x1);
+                atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 3], static_cast<uint32_t>(y));
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = t; e < kStatSlots; e += blockDim.x) {
+        const uint32_t root = hkey[e];
+        if (root == kEmpty) continue;
+        atomicAdd(&d->area[root], harea[e]);
+        atomicMin(&d->bbox[4 * static_cast<size_t>(root) + 0], hx0[e]);
+        atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 2], hx1[e]);
+        atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 3], hy1[e]);
     }
 }
 
@@ -598,6 +839,7 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
 __global__ __launch_bounds__(256) void k_collect(Plan plan) {
     int blk;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_ROW4, blockIdx.x, &blk);
+    if (!d->active) return;
     const int y = blk * 4 + static_cast<int>(threadIdx.x) / kWave;
     const int lane = threadIdx.x % kWave;
     if (y >= d->height) return;
@@ -649,6 +891,7 @@ __device__ inline void bitonic_sort_lds(uint64_t* keys, int n) {
 // buffer first find the key of rank max_candidates by an 8-bit radix select.
 __global__ __launch_bounds__(1024) void k_select(Plan plan) {
     const FrameDesc* d = plan.frames + blockIdx.x;
+    if (!d->active) return;
     __shared__ uint64_t keys[kSortCap];
     __shared__ uint32_t hist[256];
     __shared__ uint64_t s_prefix, s_mask;
@@ -868,6 +1111,7 @@ __device__ inline EdgeGeom edge_geometry(const float (*quad)[2], int edge, int d
 __global__ __launch_bounds__(256) void k_fit(Plan plan) {
     int ci;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_CAND, blockIdx.x, &ci);
+    if (!d->active) return;
     const int t = threadIdx.x;
     const int wave = t / kWave, lane = t % kWave;
     float* out = d->quads + 8 * static_cast<size_t>(ci);
@@ -908,7 +1152,7 @@ __global__ __launch_bounds__(256) void k_fit(Plan plan) {
         int lo = 0, hi = n - 1, first = n;
         while (lo <= hi) {
             const int mid = (lo + hi) >> 1;
-            if (static_cast<int>(rr[mid] >> 16) >= bx0) {
+            if (run_x1(rr[mid]) >= bx0) {
                 first = mid;
                 hi = mid - 1;
             } else {
@@ -918,8 +1162,8 @@ __global__ __launch_bounds__(256) void k_fit(Plan plan) {
         int left = -1, right = -1;
         for (int k = first; k < n; ++k) {
             const uint32_t run = rr[k];
-            const int x0 = static_cast<int>(run & 0xffffu);
-            const int x1 = static_cast<int>(run >> 16);
+            const int x0 = run_x0(run);
+            const int x1 = run_x1(run);
             if (x0 > bx1) break;
             if (lab[k] != root) continue;
             left = left < 0 ? x0 : min(left, x0);
@@ -1277,6 +1521,7 @@ __device__ inline bool homography_from_unit_quad(const float (*q)[2], float h[9]
 __global__ __launch_bounds__(512) void k_decode(Plan plan) {
     int ci;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_CAND, blockIdx.x, &ci);
+    if (!d->active) return;
     const int t = threadIdx.x;
     const int wave = t / kWave, lane = t % kWave;
     bd_detection* res = d->dec + ci;
@@ -1496,6 +1741,7 @@ __device__ inline void centre_side(const float* c, float* side, float* cx, float
 // homography.
 __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
     const FrameDesc* d = plan.frames + blockIdx.x;
+    if (!d->active) return;
     const int t = threadIdx.x;
     const int maxd = d->max_detections;
     const int ncand = static_cast<int>(d->scal[S_NCAND]);
@@ -1609,16 +1855,17 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
 hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
     const int nf = plan.nframes;
     switch (stage) {
-        case 0: hipLaunchKernelGGL(k_tile_stats, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
+        case 0: hipLaunchKernelGGL(k_tile_stats, dim3(plan.blocks[SH_TSTAT]), dim3(256), 0, stream, plan); break;
         case 1: hipLaunchKernelGGL(k_frame_ref, dim3(nf), dim3(256), 0, stream, plan); break;
         case 2: hipLaunchKernelGGL(k_binarize_runs, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
         case 3: hipLaunchKernelGGL(k_merge, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
-        case 4: hipLaunchKernelGGL(k_stats, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
-        case 5: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
-        case 6: hipLaunchKernelGGL(k_select, dim3(nf), dim3(1024), 0, stream, plan); break;
-        case 7: hipLaunchKernelGGL(k_fit, dim3(plan.blocks[SH_CAND]), dim3(256), 0, stream, plan); break;
-        case 8: hipLaunchKernelGGL(k_decode, dim3(plan.blocks[SH_CAND]), dim3(512), 0, stream, plan); break;
-        case 9: hipLaunchKernelGGL(k_finalize, dim3(nf), dim3(256), 0, stream, plan); break;
+        case 4: hipLaunchKernelGGL(k_flatten, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
+        case 5: hipLaunchKernelGGL(k_stats, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
+        case 6: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
+        case 7: hipLaunchKernelGGL(k_select, dim3(nf), dim3(1024), 0, stream, plan); break;
+        case 8: hipLaunchKernelGGL(k_fit, dim3(plan.blocks[SH_CAND]), dim3(256), 0, stream, plan); break;
+        case 9: hipLaunchKernelGGL(k_decode, dim3(plan.blocks[SH_CAND]), dim3(512), 0, stream, plan); break;
+        case 10: hipLaunchKernelGGL(k_finalize, dim3(nf), dim3(256), 0, stream, plan); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

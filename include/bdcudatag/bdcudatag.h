@@ -194,10 +194,10 @@ enum {
     BD_STAGE_TILE_MEAN = 2,    /* u8[tile_h * tile_w] */
     BD_STAGE_HIST = 3,         /* u32[256 span + 64 mean] of the last frame */
     BD_STAGE_SCALARS = 4,      /* u32[16], see bd_internal.h */
-    BD_STAGE_ROW_ROI = 5,      /* i32[3 * tile_h]: active, start, end */
+    BD_STAGE_ROW_ROI = 5,      /* i32[4 * tile_h]: active, start, end, band joined locally */
     BD_STAGE_BITS = 6,         /* u64[height * ceil(width/64)], bit x&63, 1 = white */
-    BD_STAGE_ROW_COUNT = 7,    /* u32[height]: maximal black runs per row */
-    BD_STAGE_RUNS = 8,         /* u32[height * row_cap]: x0 | x1 << 16 */
+    BD_STAGE_ROW_COUNT = 7,    /* u32[height]: black runs per row (maximal; 32-px fragments in row 0) */
+    BD_STAGE_RUNS = 8,         /* u32[height * row_cap]: x0 | x1 << 16, bit 31 = band root */
     BD_STAGE_LABELS = 9,       /* u32[height * row_cap]: root slot of each run */
     BD_STAGE_AREA = 10,        /* u32[height * row_cap], valid at roots */
     BD_STAGE_BBOX = 11,        /* u32[height * row_cap * 4]: x0, y0, x1, y1 at roots */
@@ -218,9 +218,9 @@ BD_API bd_batch* bd_detector_batch(bd_detector* detector);
 /*
  * Benchmark hooks on the frames staged by the last host call.
  * bd_batch_time_stages: runs the kernels `reps` times with HIP events between
- * consecutive kernels on the detector's stream; ms[0..9] = mean time per
- * kernel (tile_stats, frame_ref, binarize_runs, merge, stats, collect,
- * select, fit, decode, finalize), ms[10] = mean whole pipeline.
+ * consecutive kernels on the detector's stream; ms[0..10] = mean time per
+ * kernel (tile_stats, frame_ref, binarize_runs, merge, flatten, stats,
+ * collect, select, fit, decode, finalize), ms[11] = mean whole pipeline.
  * bd_batch_replay: replays the captured graph `reps` times back to back
  * without host synchronisation and returns the total device time in *ms;
  * device_path = 0 replays the host path (H2D of the staged frames, kernels,
@@ -228,6 +228,80 @@ BD_API bd_batch* bd_detector_batch(bd_detector* detector);
  */
 BD_API int32_t bd_batch_time_stages(bd_batch* batch, int32_t reps, float* ms);
 BD_API int32_t bd_batch_replay(bd_batch* batch, int32_t reps, int32_t device_path, float* ms);
+
+/* ---------------------------------------------------------------------- */
+/* Serving: continuous batching over a preallocated arena (version 3).    */
+/* ---------------------------------------------------------------------- */
+
+typedef struct bd_server bd_server;
+
+/* One frame-size class of a server: `lanes` engines (one HIP stream each)
+ * of `lane_frames` positions; at most `queue_depth` admitted requests wait
+ * for a free lane, later submissions are dropped. */
+typedef struct bd_server_class {
+    int32_t width;
+    int32_t height;
+    int32_t lanes;
+    int32_t lane_frames;
+    int32_t queue_depth;
+    int32_t reserved;
+} bd_server_class;
+
+typedef struct bd_server_result {
+    int64_t request;      /* id returned by bd_server_submit */
+    int32_t class_index;
+    int32_t count;        /* detections found (as bd_detector_detect_device returns) */
+    int32_t written;      /* records copied to the detection buffer by bd_server_poll */
+    int32_t batch_frames; /* frames in the launch that carried the request */
+    double latency_ms;    /* admission -> results on the host */
+} bd_server_result;
+
+typedef struct bd_stream_spec {
+    int32_t class_index;
+    int32_t frame_count;          /* device frames cycled by the stream */
+    double fps;                   /* Poisson arrival rate */
+    const uint8_t* const* frames; /* HIP device pointers */
+    int32_t stride;
+    int32_t reserved;
+} bd_stream_spec;
+
+typedef struct bd_serve_report {
+    int64_t offered;
+    int64_t completed;
+    int64_t dropped;
+    int64_t launches;
+    double seconds;
+    double fps;
+    double mpix_s;
+    double p50_ms;
+    double p99_ms;
+    double p999_ms;
+    double max_ms;
+    double mean_ms;
+    double mean_batch;
+} bd_serve_report;
+
+BD_API bd_server* bd_server_create(int32_t family, int32_t nclasses,
+                                   const bd_server_class* classes, const bd_config* config,
+                                   int32_t* status_out);
+BD_API void bd_server_destroy(bd_server* server);
+/* Non-blocking admission of a device-resident frame; returns the request id
+ * (>= 0) or BD_ERROR_OUT_OF_MEMORY when the class queue is full (dropped). */
+BD_API int64_t bd_server_submit(bd_server* server, int32_t class_index,
+                                const uint8_t* gray_device, int32_t stride);
+/* Non-blocking: moves up to max_results completed requests out; request i's
+ * detections go to dets + i * dets_per_result. */
+BD_API int32_t bd_server_poll(bd_server* server, bd_server_result* results, int32_t max_results,
+                              bd_detection* dets, int32_t dets_per_result);
+/* Waits until every admitted request has completed (BD_OK) or timeout. */
+BD_API int32_t bd_server_wait_idle(bd_server* server, int32_t timeout_ms);
+/* Drives the server with Poisson arrivals from `nstreams` simulated
+ * cameras for `seconds`; latency is measured from each scheduled arrival
+ * to its results on the host. */
+BD_API int32_t bd_server_simulate(bd_server* server, const bd_stream_spec* streams,
+                                  int32_t nstreams, double seconds, uint64_t seed,
+                                  bd_serve_report* report);
+BD_API const char* bd_server_last_error(void);
 
 #ifdef __cplusplus
 }

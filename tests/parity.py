@@ -29,27 +29,33 @@ def bits_from_u64(words: np.ndarray, width: int) -> np.ndarray:
     return b[:, :width].astype(bool)
 
 
+def merge_touching(row):
+    """Joins consecutive runs that touch and belong to one component.  Runs
+    that touch but stay in different components (the reference never joins
+    the 32-pixel fragments of image row 0) are kept apart."""
+    out = []
+    for x0, x1, name in row:
+        if out and out[-1][1] + 1 == x0 and out[-1][2] == name:
+            out[-1] = (out[-1][0], x1, name)
+        else:
+            out.append((x0, x1, name))
+    return out
+
+
 def oracle_components(o: dict):
-    """Maximal runs per row with component names, from the oracle's fragments."""
+    """Runs per row with component names, from the oracle's fragments."""
     runs, parent, rc = o["runs"], o["parent"], o["row_count"]
     row_cap = o["row_cap"]
     H = runs.shape[0]
     rows = []
     for y in range(H):
-        n = int(rc[y])
         out = []
-        for i in range(n):
+        for i in range(int(rc[y])):
             r = int(runs[y, i])
-            x0, x1 = r & 0xFFFF, r >> 16
             root = int(parent[y, i])
             ry, ri = divmod(root, row_cap)
-            name = (ry, int(runs[ry, ri]) & 0xFFFF)
-            if out and out[-1][1] + 1 == x0:
-                assert out[-1][2] == name, "touching fragments in different components"
-                out[-1] = (out[-1][0], x1, name)
-            else:
-                out.append((x0, x1, name))
-        rows.append(out)
+            out.append((r & 0xFFFF, r >> 16, (ry, int(runs[ry, ri]) & 0xFFFF)))
+        rows.append(merge_touching(out))
     return rows
 
 
@@ -62,14 +68,13 @@ def gpu_components(runs: np.ndarray, labels: np.ndarray, row_count: np.ndarray, 
     H = runs.shape[0]
     rows = []
     for y in range(H):
-        n = int(row_count[y])
         out = []
-        for i in range(n):
+        for i in range(int(row_count[y])):
             r = int(runs[y, i])
             root = int(labels[y, i])
             ry, ri = divmod(root, row_cap)
-            out.append((r & 0xFFFF, r >> 16, (ry, int(runs[ry, ri]) & 0xFFFF)))
-        rows.append(out)
+            out.append((r & 0xFFFF, (r >> 16) & 0x7FFF, (ry, int(runs[ry, ri]) & 0xFFFF)))
+        rows.append(merge_touching(out))
     return rows
 
 
@@ -113,7 +118,7 @@ def gpu_stage_dump(batch, index: int = 0) -> dict:
         "tile_min": g["tile_min"], "tile_max": g["tile_max"], "tile_mean": g["tile_mean"],
         "tile_thresh": g["tile_thresh"],
         "scalars": g["scalars"].view(np.uint32),
-        "row_roi": g["row_roi"].view(np.int32).reshape(th, 3),
+        "row_roi": g["row_roi"].view(np.int32).reshape(th, 4),
         "bits": g["bits"].view(np.uint64).reshape(H, w64),
         "row_count": g["row_count"].view(np.uint32),
         "runs": g["runs"].view(np.uint32).reshape(H, row_cap),

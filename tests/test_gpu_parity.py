@@ -140,3 +140,60 @@ def test_strided_host_frame():
     assert [d["id"] for d in a] == [d["id"] for d in b]
     for x, y in zip(a, b):
         assert np.array_equal(x["corners"], y["corners"])
+
+
+def test_device_pointer_path():
+    # detector/tests/test_detector.cpp:259-281: id 11 through a device pointer
+    import devmem
+    img = scenes.render_tag(0, 11, 640, 480, scenes.make_pose(320, 240, 150, 0.1))
+    buf = devmem.DeviceBuffer(img)
+    det = bdcudatag.Detector(0, 640, 480, **KA)
+    a = det.detect_device(buf.ptr, 640)
+    b = det.detect(img)
+    assert [d["id"] for d in a] == [11]
+    assert np.array_equal(a[0]["corners"], b[0]["corners"])
+    assert np.array_equal(a[0]["homography"], b[0]["homography"])
+    # padded device rows
+    padded = np.full((480, 704), 7, dtype=np.uint8)
+    padded[:, :640] = img
+    pbuf = devmem.DeviceBuffer(padded)
+    c = det.detect_device(pbuf.ptr, 704)
+    assert [d["id"] for d in c] == [11] and np.array_equal(c[0]["corners"], b[0]["corners"])
+
+
+def test_stats_and_thread_safety():
+    # detector/tests/test_detector.cpp:316-340: 4 threads x 5 frames, one handle
+    import threading
+    img = scenes.render_tag(0, 0, 640, 480, scenes.make_pose(320, 240, 170, 0.0))
+    det = bdcudatag.Detector(0, 640, 480, **KA)
+    bad = []
+
+    def work():
+        for _ in range(5):
+            d = det.detect(img)
+            if len(d) != 1 or d[0]["id"] != 0:
+                bad.append(d)
+
+    threads = [threading.Thread(target=work) for _ in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    st = det.stats()
+    assert not bad
+    assert st["frames"] == 20 and st["detections"] == 1 and st["candidates"] >= 1
+    assert st["last_gpu_ms"] > 0 and st["average_gpu_ms"] > 0
+
+
+def test_batch_of_uneven_frames_matches_single():
+    sizes = [(640, 480), (1280, 800), (1920, 1200), (1001, 777)]
+    imgs = [scenes.scene(2, w, h, 5 + i)[0] for i, (w, h) in enumerate(sizes)]
+    batch = bdcudatag.Batch(0, sizes)
+    got = batch.detect(imgs)
+    for i, (w, h) in enumerate(sizes):
+        single = bdcudatag.Detector(0, w, h).detect(imgs[i])
+        assert [d["id"] for d in got[i]] == [d["id"] for d in single]
+        for a, b in zip(got[i], single):
+            assert np.array_equal(a["corners"], b["corners"])
+        o = pyoracle.Oracle(pyoracle.default_config(0, w, h)).run(imgs[i])
+        assert [d["id"] for d in o["detections"]] == [d["id"] for d in got[i]]
