@@ -238,6 +238,28 @@ class Batch:
     def detect(self, images):
         return self._run(images, False)
 
+    def prepare_device(self, ptr_strides):
+        """Pre-builds the frame table and output buffers for detect_raw()."""
+        n = len(self.frames)
+        arr = (Frame * n)()
+        for i, (ptr, stride) in enumerate(ptr_strides):
+            arr[i].gray = ptr
+            arr[i].width = self.frames[i]["width"]
+            arr[i].height = self.frames[i]["height"]
+            arr[i].stride = stride
+        md = self.max_detections
+        self._raw = (arr, (Detection * (n * md))(), (ctypes.c_int32 * n)(), n, md)
+
+    def detect_raw(self) -> int:
+        """One bd_batch_detect_device call on the prepared frames; results stay
+        in the prepared C buffers (no Python conversion).  Returns the total
+        number of records written."""
+        arr, out, counts, n, md = self._raw
+        rc = lib().bd_batch_detect_device(self.handle, arr, n, out, md, counts)
+        if rc < 0:
+            raise RuntimeError(f"batch detect failed ({rc}): {last_error()}")
+        return rc
+
     def detect_device(self, ptr_strides):
         return self._run(ptr_strides, True)
 

@@ -405,11 +405,11 @@ __device__ inline void lds_union(uint32_t* lab, uint32_t a, uint32_t b) {
 //     fragments there are never rejoined (k_find_runs :427-504 emits
 //     fragments; k_merge_runs :514 skips row 0, so its same-row joins never
 //     run for that row).
-//  4. Band-local union-find in LDS over the 8 rows, processed top to bottom
-//     with each row flattened before the next, so every run ends one hop
-//     from its band root; band roots are flagged in the run word.  A band
-//     with more runs than the LDS buffer leaves every run a root and lets
-//     k_merge join all of its rows.
+//  4. Band-local union-find in LDS over the 8 rows, flattened so every run
+//     ends one hop from its band root; band roots are flagged in the run
+//     word and only they get component statistics initialised.  A band with
+//     more runs than the LDS buffer leaves every run a root and lets k_merge
+//     join all of its rows.
 __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
     int ty;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
@@ -423,28 +423,41 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
     const int row_on = roi[0], rs = roi[1], re = roi[2];
 
     __shared__ uint8_t thr[1024];
-    __shared__ uint64_t masks[kTile][128];
     __shared__ uint32_t s_off[kTile + 1];
-    __shared__ uint32_t b_run[kBandRuns];
-    __shared__ uint32_t b_lab[kBandRuns];
+    __shared__ __align__(16) uint32_t b_run[kBandRuns];
+    __shared__ __align__(16) uint32_t b_lab[kBandRuns];
+    // Short-lived buffers share storage with the band arrays: the tile
+    // staging for the pooled thresholds lives in b_run until the runs are
+    // written, the pixel masks live in b_lab until the band union starts.
+    uint8_t* pool_min = reinterpret_cast<uint8_t*>(b_run);
+    uint8_t* pool_max = pool_min + 3 * 1026;
+    uint64_t (*masks)[128] = reinterpret_cast<uint64_t (*)[128]>(b_lab);
 
-    // Pooled 3x3 thresholds for this tile row (k_tile_threshold).
+    // Pooled 3x3 thresholds for this tile row (k_tile_threshold): the three
+    // tile rows are staged in LDS (padded with neutral values), then pooled.
     const int ref0 = static_cast<int>(d->scal[S_REF0]);
     const int ref1 = static_cast<int>(d->scal[S_REF1]);
     const int floor_contrast = max(kMinTileContrast, (ref0 * 3) / 2);
+    for (int i = t; i < 3 * (tw + 2); i += blockDim.x) {
+        const int dy = i / (tw + 2) - 1;
+        const int xx = i % (tw + 2) - 1;
+        const int yy = ty + dy;
+        uint8_t mn = 255, mx = 0;
+        if (yy >= 0 && yy < th && xx >= 0 && xx < tw) {
+            mn = d->tile_min[static_cast<size_t>(yy) * tw + xx];
+            mx = d->tile_max[static_cast<size_t>(yy) * tw + xx];
+        }
+        pool_min[i] = mn;
+        pool_max[i] = mx;
+    }
+    __syncthreads();
     for (int tx = t; tx < tw; tx += blockDim.x) {
         int mn = 255, mx = 0;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int yy = ty + dy;
-            if (yy < 0 || yy >= th) continue;
-            const uint8_t* rmin = d->tile_min + static_cast<size_t>(yy) * tw;
-            const uint8_t* rmax = d->tile_max + static_cast<size_t>(yy) * tw;
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int xx = tx + dx;
-                if (xx < 0 || xx >= tw) continue;
-                mn = min(mn, static_cast<int>(rmin[xx]));
-                mx = max(mx, static_cast<int>(rmax[xx]));
-            }
+        for (int dy = 0; dy < 3; ++dy) {
+            const uint8_t* rmin = pool_min + dy * (tw + 2) + tx;
+            const uint8_t* rmax = pool_max + dy * (tw + 2) + tx;
+            mn = min(mn, min(static_cast<int>(rmin[0]), min(static_cast<int>(rmin[1]), static_cast<int>(rmin[2]))));
+            mx = max(mx, max(static_cast<int>(rmax[0]), max(static_cast<int>(rmax[1]), static_cast<int>(rmax[2]))));
         }
         const int tv = (mx - mn >= floor_contrast) ? ((mn + mx + 1) >> 1) : ref1;
         thr[tx] = static_cast<uint8_t>(tv);
@@ -581,10 +594,10 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
                 } else {
                     runs16[2 * slot] = static_cast<uint16_t>(w * 64 + b);
                     d->label[slot] = slot;
+                    d->area[slot] = 0;
+                    uint4* bb = reinterpret_cast<uint4*>(d->bbox + 4 * static_cast<size_t>(slot));
+                    *bb = make_uint4(0xffffffffu, 0xffffffffu, 0u, 0u);
                 }
-                d->area[slot] = 0;
-                uint4* bb = reinterpret_cast<uint4*>(d->bbox + 4 * static_cast<size_t>(slot));
-                *bb = make_uint4(0xffffffffu, 0xffffffffu, 0u, 0u);
             }
             while (ends) {
                 const int b = __ffsll(static_cast<unsigned long long>(ends)) - 1;
@@ -604,35 +617,33 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
     if (!local) return;
     __syncthreads();
 
-    // Band-local union-find, rows in order (k_merge_runs :525-556 on the band).
+    // Band-local union-find (k_merge_runs :525-556 on the band's rows), all
+    // row pairs at once: LDS atomics keep it coherent and a band's trees are
+    // at most a few hops deep.
     for (uint32_t i = t; i < nb; i += blockDim.x) b_lab[i] = i;
     __syncthreads();
-    for (int r = 1; r < nrows; ++r) {
-        const uint32_t beg = s_off[r], end = s_off[r + 1], pbeg = s_off[r - 1];
-        if (beg < end && pbeg < beg) {
-            for (uint32_t i = beg + t; i < end; i += blockDim.x) {
-                const uint32_t run = b_run[i];
-                const int x0 = run_x0(run), x1 = run_x1(run);
-                int lo = static_cast<int>(pbeg), hi = static_cast<int>(beg) - 1, first = static_cast<int>(beg);
-                while (lo <= hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (run_x1(b_run[mid]) >= x0 - 1) {
-                        first = mid;
-                        hi = mid - 1;
-                    } else {
-                        lo = mid + 1;
-                    }
-                }
-                for (int k = first; k < static_cast<int>(beg); ++k) {
-                    if (run_x0(b_run[k]) > x1 + 1) break;
-                    lds_union(b_lab, i, static_cast<uint32_t>(k));
-                }
+    for (uint32_t i = s_off[1] + t; i < nb; i += blockDim.x) {
+        int r = 1;
+        while (i >= s_off[r + 1]) ++r;
+        const uint32_t beg = s_off[r], pbeg = s_off[r - 1];
+        const uint32_t run = b_run[i];
+        const int x0 = run_x0(run), x1 = run_x1(run);
+        int lo = static_cast<int>(pbeg), hi = static_cast<int>(beg) - 1, first = static_cast<int>(beg);
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1;
+            if (run_x1(b_run[mid]) >= x0 - 1) {
+                first = mid;
+                hi = mid - 1;
+            } else {
+                lo = mid + 1;
             }
-            __syncthreads();
-            for (uint32_t i = beg + t; i < end; i += blockDim.x) b_lab[i] = lds_find(b_lab, i);
         }
-        __syncthreads();
+        for (int k = first; k < static_cast<int>(beg); ++k) {
+            if (run_x0(b_run[k]) > x1 + 1) break;
+            lds_union(b_lab, i, static_cast<uint32_t>(k));
+        }
     }
+    __syncthreads();
     for (uint32_t i = t; i < nb; i += blockDim.x) b_lab[i] = lds_find(b_lab, i);
     __syncthreads();
     for (uint32_t i = t; i < nb; i += blockDim.x) {
@@ -645,6 +656,11 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
         const uint32_t rslot = static_cast<uint32_t>(y0 + rr) * static_cast<uint32_t>(row_cap) + (root - s_off[rr]);
         d->runs[slot] = b_run[i] | (root == i ? kRootFlag : 0u);
         d->label[slot] = rslot;
+        if (root == i) {
+            d->area[slot] = 0;
+            uint4* bb = reinterpret_cast<uint4*>(d->bbox + 4 * static_cast<size_t>(slot));
+            *bb = make_uint4(0xffffffffu, 0xffffffffu, 0u, 0u);
+        }
     }
 }
 
@@ -682,45 +698,89 @@ __device__ inline void uf_union(uint32_t* label, uint32_t a, uint32_t b) {
     }
 }
 
-// One wave per row y that starts a band (or lies in a band that was not
-// joined locally): every run unions with the runs of row y-1 that touch
-// [x0-1, x1+1] (8-connectivity; reference k_merge_runs :509-557).  Runs of
-// one row never need same-row joins: they are maximal, or in row 0 they are
-// fragments the reference leaves unjoined.
+constexpr int kPairSlots = 2048;
+
+// One workgroup per band.  Its first row is joined to the last row of the
+// band above (8-connectivity: runs touching [x0-1, x1+1]; reference
+// k_merge_runs :509-557); a band that was not joined in LDS also joins all
+// of its own rows here.  The (band root, band root) pairs of all touching
+// runs are first de-duplicated in an LDS hash set, so a large component
+// whose runs cross the boundary hundreds of times costs one union per
+// distinct pair instead of one per run.  Runs of one row never need
+// same-row joins: they are maximal, or in row 0 they are fragments the
+// reference leaves unjoined.
 __global__ __launch_bounds__(256) void k_merge(Plan plan) {
-    int blk;
-    const FrameDesc* d = plan.frames + frame_of(plan, SH_ROW4, blockIdx.x, &blk);
+    int ty;
+    const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
     if (!d->active) return;
-    const int y = blk * 4 + static_cast<int>(threadIdx.x) / kWave;
-    const int lane = threadIdx.x % kWave;
-    if (y < 1 || y >= d->height) return;
-    if ((y & (kTile - 1)) != 0 && d->row_roi[4 * (y >> 3) + 3]) return;  // joined in its band
-    const int n = static_cast<int>(d->row_count[y]);
-    const int pn = static_cast<int>(d->row_count[y - 1]);
-    if (n == 0 || pn == 0) return;
+    const int32_t* roi = d->row_roi + 4 * ty;
+    if (!roi[0]) return;
+    const int H = d->height;
+    const int y0 = ty * kTile;
+    const int nrows = min(kTile, H - y0);
+    const bool local = roi[3] != 0;
+    const int first_row = ty == 0 ? 1 : 0;
+    const int last_row = local ? 1 : nrows;  // exclusive, band-relative
+    if (first_row >= last_row) return;
+    const int t = threadIdx.x;
+    const int wave = t / kWave, lane = t % kWave;
+    __shared__ unsigned long long pairs[kPairSlots];
+    for (int e = t; e < kPairSlots; e += blockDim.x) pairs[e] = ~0ull;
+    __syncthreads();
     const uint32_t row_cap = static_cast<uint32_t>(d->row_cap);
-    const uint32_t* cur = d->runs + static_cast<size_t>(y) * row_cap;
-    const uint32_t* prev = d->runs + static_cast<size_t>(y - 1) * row_cap;
-    const uint32_t cur_base = static_cast<uint32_t>(y) * row_cap;
-    const uint32_t prev_base = static_cast<uint32_t>(y - 1) * row_cap;
-    for (int i = lane; i < n; i += kWave) {
-        const uint32_t run = cur[i];
-        const int x0 = run_x0(run);
-        const int x1 = run_x1(run);
-        int lo = 0, hi = pn - 1, first = pn;
-        while (lo <= hi) {
-            const int mid = (lo + hi) >> 1;
-            if (run_x1(prev[mid]) >= x0 - 1) {
-                first = mid;
-                hi = mid - 1;
-            } else {
-                lo = mid + 1;
+    uint32_t* label = d->label;
+    for (int r = first_row + wave; r < last_row; r += static_cast<int>(blockDim.x) / kWave) {
+        const int y = y0 + r;
+        const int n = static_cast<int>(d->row_count[y]);
+        const int pn = static_cast<int>(d->row_count[y - 1]);
+        if (n == 0 || pn == 0) continue;
+        const uint32_t* cur = d->runs + static_cast<size_t>(y) * row_cap;
+        const uint32_t* prev = d->runs + static_cast<size_t>(y - 1) * row_cap;
+        const uint32_t cur_base = static_cast<uint32_t>(y) * row_cap;
+        const uint32_t prev_base = static_cast<uint32_t>(y - 1) * row_cap;
+        for (int i = lane; i < n; i += kWave) {
+            const uint32_t run = cur[i];
+            const int x0 = run_x0(run);
+            const int x1 = run_x1(run);
+            int lo = 0, hi = pn - 1, first = pn;
+            while (lo <= hi) {
+                const int mid = (lo + hi) >> 1;
+                if (run_x1(prev[mid]) >= x0 - 1) {
+                    first = mid;
+                    hi = mid - 1;
+                } else {
+                    lo = mid + 1;
+                }
+            }
+            const uint32_t a = label[cur_base + i];
+            uint32_t last_b = 0xffffffffu;
+            for (int k = first; k < pn; ++k) {
+                if (run_x0(prev[k]) > x1 + 1) break;
+                const uint32_t b = label[prev_base + k];
+                if (b == a || b == last_b) continue;
+                last_b = b;
+                const unsigned long long key =
+                    (static_cast<unsigned long long>(min(a, b)) << 32) | max(a, b);
+                bool stored = false;
+                const uint32_t h = static_cast<uint32_t>((key * 0x9e3779b97f4a7c15ull) >> 53);
+                for (int probe = 0; probe < 16; ++probe) {
+                    const int e = static_cast<int>((h + static_cast<uint32_t>(probe)) & (kPairSlots - 1));
+                    unsigned long long old = pairs[e];
+                    if (old == ~0ull) old = atomicCAS(&pairs[e], ~0ull, key);
+                    if (old == ~0ull || old == key) {
+                        stored = true;
+                        break;
+                    }
+                }
+                if (!stored) uf_union(label, a, b);
             }
         }
-        for (int k = first; k < pn; ++k) {
-            if (run_x0(prev[k]) > x1 + 1) break;
-            uf_union(d->label, cur_base + i, prev_base + k);
-        }
+    }
+    __syncthreads();
+    for (int e = t; e < kPairSlots; e += blockDim.x) {
+        const unsigned long long key = pairs[e];
+        if (key == ~0ull) continue;
+        uf_union(label, static_cast<uint32_t>(key >> 32), static_cast<uint32_t>(key & 0xffffffffu));
     }
 }
 
@@ -985,6 +1045,31 @@ __device__ inline float directional_rise(const uint8_t* image, int width, int he
     const float behind = sample_bilinear(image, width, height, stride, x - dx * 0.7f, y - dy * 0.7f);
     if (ahead < 0.0f || behind < 0.0f) return 0.0f;
     return ahead - behind;
+}
+
+// sample_bilinear split into address and value halves so the loads of
+// several samples can be in flight at once; tap_value evaluates the same
+// expression as sample_bilinear.
+struct Tap {
+    const uint8_t* p;
+    float fx, fy;
+    bool ok;
+};
+
+__device__ inline Tap make_tap(const uint8_t* image, int width, int height, int stride, float x, float y) {
+    Tap t;
+    t.ok = x >= 0.0f && y >= 0.0f && x < static_cast<float>(width - 1) && y < static_cast<float>(height - 1);
+    const int ix = t.ok ? static_cast<int>(x) : 0;
+    const int iy = t.ok ? static_cast<int>(y) : 0;
+    t.fx = x - static_cast<float>(ix);
+    t.fy = y - static_cast<float>(iy);
+    t.p = image + static_cast<size_t>(iy) * stride + ix;
+    return t;
+}
+
+__device__ inline float tap_value(const Tap& t, float a, float b, float c, float e) {
+    if (!t.ok) return -1.0f;
+    return (a * (1.0f - t.fx) + b * t.fx) * (1.0f - t.fy) + (c * (1.0f - t.fx) + e * t.fx) * t.fy;
 }
 
 __device__ inline float hyp(float x, float y) { return sqrtf(x * x + y * y); }
@@ -1353,21 +1438,60 @@ __global__ __launch_bounds__(256) void k_fit(Plan plan) {
             const bool first_pass = iteration == 0;
             const EdgeGeom g = edge_geometry(fit_quad, wave, d->dim, first_pass);
             if (g.ok) {
+                // Every (sample, probe) rise of this edge, four per lane per
+                // round with all 32 byte loads issued before the arithmetic.
                 const int total = g.samples * g.probes;
-                for (int it = lane; it < total; it += kWave) {
-                    const int s = it / g.probes;
-                    const int i = it - s * g.probes;
-                    const float alpha = static_cast<float>(s + 1) / static_cast<float>(g.samples + 1);
-                    const float px = g.ax + alpha * g.ex;
-                    const float py = g.ay + alpha * g.ey;
-                    const float tt = g.step * static_cast<float>(i - g.half);
-                    mags[wave][s][i] = directional_rise(gray, W, H, stride, px + g.nx * tt, py + g.ny * tt, g.nx, g.ny);
+                for (int it0 = lane; it0 < total; it0 += 4 * kWave) {
+                    Tap ta[4], tb[4];
+                    int s_of[4], i_of[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int it = it0 + u * kWave;
+                        const bool live = it < total;
+                        const int s = live ? it / g.probes : 0;
+                        const int i = live ? it - s * g.probes : 0;
+                        s_of[u] = s;
+                        i_of[u] = i;
+                        const float alpha = static_cast<float>(s + 1) / static_cast<float>(g.samples + 1);
+                        const float px = g.ax + alpha * g.ex;
+                        const float py = g.ay + alpha * g.ey;
+                        const float tt = g.step * static_cast<float>(i - g.half);
+                        const float x = px + g.nx * tt;
+                        const float y = py + g.ny * tt;
+                        ta[u] = make_tap(gray, W, H, stride, x + g.nx * 0.7f, y + g.ny * 0.7f);
+                        tb[u] = make_tap(gray, W, H, stride, x - g.nx * 0.7f, y - g.ny * 0.7f);
+                        ta[u].ok = ta[u].ok && live;
+                        tb[u].ok = tb[u].ok && live;
+                    }
+                    float va[4][4], vb[4][4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const uint8_t* p = ta[u].p;
+                        const uint8_t* q = tb[u].p;
+                        va[u][0] = ta[u].ok ? static_cast<float>(p[0]) : 0.0f;
+                        va[u][1] = ta[u].ok ? static_cast<float>(p[1]) : 0.0f;
+                        va[u][2] = ta[u].ok ? static_cast<float>(p[stride]) : 0.0f;
+                        va[u][3] = ta[u].ok ? static_cast<float>(p[stride + 1]) : 0.0f;
+                        vb[u][0] = tb[u].ok ? static_cast<float>(q[0]) : 0.0f;
+                        vb[u][1] = tb[u].ok ? static_cast<float>(q[1]) : 0.0f;
+                        vb[u][2] = tb[u].ok ? static_cast<float>(q[stride]) : 0.0f;
+                        vb[u][3] = tb[u].ok ? static_cast<float>(q[stride + 1]) : 0.0f;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        if (it0 + u * kWave >= total) continue;
+                        const float ahead = tap_value(ta[u], va[u][0], va[u][1], va[u][2], va[u][3]);
+                        const float behind = tap_value(tb[u], vb[u][0], vb[u][1], vb[u][2], vb[u][3]);
+                        mags[wave][s_of[u]][i_of[u]] = (ahead < 0.0f || behind < 0.0f) ? 0.0f : ahead - behind;
+                    }
                 }
             }
             __syncthreads();
             if (lane < 32) {
                 float mx = 0.0f, my = 0.0f, mxx = 0.0f, mxy = 0.0f, myy = 0.0f;
                 int used = 0;
+                // Moments about the edge midpoint (see the oracle's note).
+                const float rx = g.ax + 0.5f * g.ex, ry = g.ay + 0.5f * g.ey;
                 if (g.ok && lane < g.samples) {
                     const float* mg = mags[wave][lane];
                     const float alpha = static_cast<float>(lane + 1) / static_cast<float>(g.samples + 1);
@@ -1406,8 +1530,8 @@ __global__ __launch_bounds__(256) void k_fit(Plan plan) {
                                 tt += fminf(fmaxf(0.5f * (m0 - m2) / den, -0.5f), 0.5f) * g.step;
                             }
                         }
-                        const float x = px + g.nx * tt;
-                        const float y = py + g.ny * tt;
+                        const float x = px + g.nx * tt - rx;
+                        const float y = py + g.ny * tt - ry;
                         mx = x;
                         my = y;
                         mxx = x * x;
@@ -1445,8 +1569,9 @@ __global__ __launch_bounds__(256) void k_fit(Plan plan) {
                         const float cyy = red[wave][0][4] * inv - cy * cy;
                         float lnx, lny;
                         axis_normal(cxx, cxy, cyy, &lnx, &lny);
-                        edge_lines[wave][0] = cx;
-                        edge_lines[wave][1] = cy;
+                        const float rx = g.ax + 0.5f * g.ex, ry = g.ay + 0.5f * g.ey;
+                        edge_lines[wave][0] = rx + cx;
+                        edge_lines[wave][1] = ry + cy;
                         edge_lines[wave][2] = lnx;
                         edge_lines[wave][3] = lny;
                         have = 1;
@@ -1858,7 +1983,7 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
         case 0: hipLaunchKernelGGL(k_tile_stats, dim3(plan.blocks[SH_TSTAT]), dim3(256), 0, stream, plan); break;
         case 1: hipLaunchKernelGGL(k_frame_ref, dim3(nf), dim3(256), 0, stream, plan); break;
         case 2: hipLaunchKernelGGL(k_binarize_runs, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
-        case 3: hipLaunchKernelGGL(k_merge, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
+        case 3: hipLaunchKernelGGL(k_merge, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
         case 4: hipLaunchKernelGGL(k_flatten, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
         case 5: hipLaunchKernelGGL(k_stats, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
         case 6: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;

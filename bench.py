@@ -9,6 +9,16 @@ bd_batch_detect_device call, which replays the captured hipGraph and
 returns every frame's detections to the host.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+                  [--single-frames N] [--cpu-seconds S] [--serve-seconds S]
+                  [--uneven-steps N]
+
+Besides the headline line it reports an uneven batch (16 frames each of
+640x480, 1280x800, 1920x1200 and 4096x3072 in one launch) and a serving run
+(six Poisson camera streams, continuous batching, p50/p99/p99.9 latency).
+
+--single-frames 0 and --cpu-seconds 0 leave only batched launches, which is
+the command profiled under profiles/ so that per-kernel averages there match
+the stages_ms figures.
 
 Prints one JSON line.  Only one GPU is used; --gpus above 1 exits with a
 message.
@@ -61,6 +71,60 @@ def cpu_baseline(frames, seconds: float):
     }
 
 
+def uneven_batch(bdcudatag, devmem, scenes, steps):
+    """64 frames of four sizes (16 each of 640x480, 1280x800, 1920x1200,
+    4096x3072), 0-32 tags per frame, one graph launch per step."""
+    sizes = [(640, 480), (1280, 800), (1920, 1200), (4096, 3072)]
+    per = 16
+    threads = min(16, os.cpu_count() or 1)
+    bufs, ptrs, all_sizes = [], [], []
+    for k, (w, h) in enumerate(sizes):
+        imgs = scenes.scene_batch(2, w, h, 1000 * k, per, threads=threads)
+        buf = devmem.DeviceBuffer(imgs)
+        bufs.append(buf)
+        ptrs += [(buf.ptr + i * w * h, w) for i in range(per)]
+        all_sizes += [(w, h)] * per
+    px = sum(w * h for w, h in all_sizes)
+    batch = bdcudatag.Batch(0, all_sizes)
+    batch.prepare_device(ptrs)
+    for _ in range(2):
+        batch.detect_raw()
+    devmem.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.detect_raw()
+    devmem.sync()
+    t = time.perf_counter() - t0
+    dev_ms = batch.replay(steps, device=True) / steps
+    return {"frames": len(all_sizes), "mpix_per_step": round(px / 1e6, 2),
+            "mpix_s": round(px * steps / t / 1e6, 1), "ms_per_step": round(t / steps * 1e3, 3),
+            "device_ms_per_step": round(dev_ms, 3)}
+
+
+def serving(bdcudatag, devmem, scenes, seconds):
+    """Six simulated cameras with Poisson arrivals (4 x 1280x800 at 120 fps,
+    2 x 1600x1200 at 60 fps), frames already in HBM, continuous batching."""
+    classes = [{"width": 1280, "height": 800, "lanes": 4, "lane_frames": 4, "queue_depth": 64},
+               {"width": 1600, "height": 1200, "lanes": 2, "lane_frames": 2, "queue_depth": 32}]
+    threads = min(16, os.cpu_count() or 1)
+    bufs, streams = [], []
+    for ci, (c, n, fps) in enumerate(((classes[0], 4, 120.0), (classes[1], 2, 60.0))):
+        for k in range(n):
+            imgs = scenes.scene_batch(4, c["width"], c["height"], 10000 * ci + 100 * k, 16, threads=threads)
+            buf = devmem.DeviceBuffer(imgs)
+            bufs.append(buf)
+            px = c["width"] * c["height"]
+            streams.append({"class_index": ci, "fps": fps, "stride": c["width"],
+                            "frames": [buf.ptr + i * px for i in range(16)]})
+    srv = bdcudatag.Server(0, classes)
+    srv.simulate(streams, seconds=min(1.0, seconds), seed=99)  # warm-up
+    rep = srv.simulate(streams, seconds=seconds, seed=1)
+    srv.close()
+    keys = ("offered", "completed", "dropped", "launches", "seconds", "fps", "mpix_s", "p50_ms",
+            "p99_ms", "p999_ms", "max_ms", "mean_ms", "mean_batch")
+    return {k: (round(rep[k], 4) if isinstance(rep[k], float) else rep[k]) for k in keys}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +135,8 @@ def main():
     ap.add_argument("--height", type=int, default=800)
     ap.add_argument("--single-frames", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--uneven-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames shard "
@@ -92,34 +158,47 @@ def main():
     batch = bdcudatag.Batch(0, [(W, H)] * B)
     batch.detect(list(frames))  # stage frames for the per-kernel timing below
     stages = batch.time_stages(reps=5)
+    res = batch.detect_device(ptrs)
+    found = sum(len(r) for r in res)
+    batch.prepare_device(ptrs)
     for _ in range(WU):
-        batch.detect_device(ptrs)
+        batch.detect_raw()
     devmem.sync()
     t0 = time.perf_counter()
     for _ in range(K):
-        res = batch.detect_device(ptrs)
+        batch.detect_raw()
     devmem.sync()
     t_batch = time.perf_counter() - t0
     batched_mpix = B * px * K / t_batch / 1e6
     replay_ms = batch.replay(K, device=True)
-    found = sum(len(r) for r in res)
 
     # --- single-frame path -------------------------------------------------
-    det = bdcudatag.Detector(0, W, H)
-    lat_dev, lat_host = [], []
-    for i in range(WU):
-        det.detect_device(*ptrs[i % B])
-        det.detect(frames[i % B])
-    for i in range(args.single_frames):
-        t = time.perf_counter()
-        det.detect_device(*ptrs[i % B])
-        lat_dev.append((time.perf_counter() - t) * 1e3)
-    for i in range(args.single_frames):
-        t = time.perf_counter()
-        det.detect(frames[i % B])
-        lat_host.append((time.perf_counter() - t) * 1e3)
-    single_mpix = px / (percentile(lat_dev, 50) * 1e-3) / 1e6
-    single_stages = det.batch.time_stages(reps=10)
+    single = None
+    if args.single_frames > 0:
+        det = bdcudatag.Detector(0, W, H)
+        lat_dev, lat_host = [], []
+        for i in range(WU):
+            det.detect_device(*ptrs[i % B])
+            det.detect(frames[i % B])
+        for i in range(args.single_frames):
+            t = time.perf_counter()
+            det.detect_device(*ptrs[i % B])
+            lat_dev.append((time.perf_counter() - t) * 1e3)
+        for i in range(args.single_frames):
+            t = time.perf_counter()
+            det.detect(frames[i % B])
+            lat_host.append((time.perf_counter() - t) * 1e3)
+        single_mpix = px / (percentile(lat_dev, 50) * 1e-3) / 1e6
+        single_stages = det.batch.time_stages(reps=10)
+        single = {
+            "mpix_s": round(single_mpix, 1),
+            "p50_ms": round(percentile(lat_dev, 50), 4),
+            "p99_ms": round(percentile(lat_dev, 99), 4),
+            "host_input_p50_ms": round(percentile(lat_host, 50), 4),
+            "host_input_p99_ms": round(percentile(lat_host, 99), 4),
+            "device_ms": round(single_stages["pipeline"], 4),
+            "stages_ms": {k: round(v, 4) for k, v in single_stages.items()},
+        }
 
     # --- roofline of the dominant kernel ------------------------------------
     kern = {k: v for k, v in stages.items() if k != "pipeline"}
@@ -146,15 +225,8 @@ def main():
                         "inputs resident in HBM, one graph replay per step",
             "batch": B, "width": W, "height": H, "family": "tag36h11",
         },
-        "single_frame": {
-            "mpix_s": round(single_mpix, 1),
-            "p50_ms": round(percentile(lat_dev, 50), 4),
-            "p99_ms": round(percentile(lat_dev, 99), 4),
-            "host_input_p50_ms": round(percentile(lat_host, 50), 4),
-            "host_input_p99_ms": round(percentile(lat_host, 99), 4),
-            "device_ms": round(single_stages["pipeline"], 4),
-        },
-        "batched_vs_single": round(batched_mpix / single_mpix, 2),
+        "single_frame": single,
+        "batched_vs_single": round(batched_mpix / single["mpix_s"], 2) if single else None,
         "batch_device_ms_per_step": round(replay_ms / K, 4),
         "detections_per_step": found,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
@@ -170,7 +242,11 @@ def main():
                               "frac": round(stream_achieved / HBM_PEAK_GBS, 4)},
         },
     }
-    out["cpu_baseline"] = cpu_baseline(frames, args.cpu_seconds)
+    if args.uneven_steps > 0:
+        out["uneven_batch"] = uneven_batch(bdcudatag, devmem, scenes, args.uneven_steps)
+    if args.serve_seconds > 0:
+        out["serving"] = serving(bdcudatag, devmem, scenes, args.serve_seconds)
+    out["cpu_baseline"] = cpu_baseline(frames, args.cpu_seconds) if args.cpu_seconds > 0 else None
     print(json.dumps(out), flush=True)
 
 

@@ -846,6 +846,11 @@ static int fit_edge_line(const real quad[4][2], int edge, const uint8_t* image, 
     const int probes = 2 * half + 1;
     const real step = range / (real)half;
     const int samples = imin(imax((int)(len / RL(6.0)), 8), 16);
+    /* Moments are taken about the edge midpoint instead of the image origin
+     * (the reference sums raw x, x^2 at gpu_detector.cu:983-987): the same
+     * line, without the cancellation that raw second moments of coordinates
+     * near 1000 suffer in FP32. */
+    const real rx = ax + RL(0.5) * ex, ry = ay + RL(0.5) * ey;
 
     real lm[32][5];
     int lused[32];
@@ -898,8 +903,8 @@ static int fit_edge_line(const real quad[4][2], int edge, const uint8_t* image, 
                         t += R_FMIN(R_FMAX(RL(0.5) * (m0 - m2) / den, RL(-0.5)), RL(0.5)) * step;
                     }
                 }
-                const real x = px + nx * t;
-                const real y = py + ny * t;
+                const real x = px + nx * t - rx;
+                const real y = py + ny * t - ry;
                 mx = x;
                 my = y;
                 mxx = x * x;
@@ -934,8 +939,8 @@ static int fit_edge_line(const real quad[4][2], int edge, const uint8_t* image, 
     const real cyy = lm[0][4] * inv - cy * cy;
     real lnx, lny;
     axis_normal(cxx, cxy, cyy, &lnx, &lny);
-    line[0] = cx;
-    line[1] = cy;
+    line[0] = rx + cx;
+    line[1] = ry + cy;
     line[2] = lnx;
     line[3] = lny;
     return 1;
