@@ -167,6 +167,10 @@ static int32_t detect_one(bd_detector* detector, const uint8_t* gray, int32_t st
         set_error("stride is smaller than the image width");
         return BD_ERROR_INVALID_ARGUMENT;
     }
+    if (device && !bd::frame_span_fits(detector->width, detector->height, stride)) {
+        set_error("device frame spans 4 GiB or more");
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
     if (!check_output(out, out_capacity)) return BD_ERROR_INVALID_ARGUMENT;
     bd_frame frame{gray, detector->width, detector->height, stride, 0};
     int32_t found = 0;
@@ -275,6 +279,10 @@ static int32_t batch_detect(bd_batch* batch, const bd_frame* frames, int32_t cou
         }
         if (frames[i].stride < frames[i].width) {
             set_error("stride is smaller than the image width");
+            return BD_ERROR_INVALID_ARGUMENT;
+        }
+        if (device && !bd::frame_span_fits(frames[i].width, frames[i].height, frames[i].stride)) {
+            set_error("device frame spans 4 GiB or more");
             return BD_ERROR_INVALID_ARGUMENT;
         }
     }

@@ -138,6 +138,9 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         input_off_[f] = input_bytes_;
         input_bytes_ = align_up(input_bytes_ + static_cast<size_t>(W) * H, 256);
     }
+    size_t work_cap = 0;
+    for (int f = 0; f < nf; ++f) work_cap += static_cast<size_t>(maxc[f]);
+    const size_t o_work = take((kWorkItems + work_cap) * 4);
     const size_t o_scal = take(static_cast<size_t>(nf) * S_COUNT * 4);
     const size_t o_out = take(static_cast<size_t>(nf) * max_det_ * sizeof(bd_detection));
     const size_t o_desc_h = take(static_cast<size_t>(nf) * sizeof(FrameDesc));
@@ -155,6 +158,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     }
     uint8_t* base = static_cast<uint8_t*>(arena_);
     d_scal_ = reinterpret_cast<uint32_t*>(base + o_scal);
+    d_work_ = reinterpret_cast<uint32_t*>(base + o_work);
     d_out_ = reinterpret_cast<bd_detection*>(base + o_out);
     d_desc_host_ = reinterpret_cast<FrameDesc*>(base + o_desc_h);
     d_desc_dev_ = reinterpret_cast<FrameDesc*>(base + o_desc_d);
@@ -220,7 +224,6 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         d.out = d_out_ + static_cast<size_t>(f) * max_det_;
         prefix[SH_TILE_ROW * (nf + 1) + f + 1] = prefix[SH_TILE_ROW * (nf + 1) + f] + d.tile_h;
         prefix[SH_ROW4 * (nf + 1) + f + 1] = prefix[SH_ROW4 * (nf + 1) + f] + (H + 3) / 4;
-        prefix[SH_CAND * (nf + 1) + f + 1] = prefix[SH_CAND * (nf + 1) + f] + maxc[f];
         prefix[SH_FRAME * (nf + 1) + f + 1] = f + 1;
         const int pairs = (d.tile_w + 1) / 2;
         const int rows_per_wg = pairs >= 256 ? 1 : 256 / pairs;
@@ -244,6 +247,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         st = hipMemcpy(d_scal_, scal.data(), scal.size() * 4, hipMemcpyHostToDevice);
     }
     if (st == hipSuccess) st = hipMemset(d_input_, 0, std::max<size_t>(input_bytes_, 1));
+    if (st == hipSuccess) st = hipMemset(d_work_, 0, (kWorkItems + work_cap) * 4);
     if (st != hipSuccess) {
         if (error) *error = hip_error("detector setup copy", st);
         return false;
@@ -252,6 +256,8 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     plan_host_.frames = d_desc_host_;
     plan_host_.prefix = d_prefix_;
     plan_host_.nframes = nf;
+    plan_host_.work = d_work_;
+    plan_host_.work_cap = static_cast<int32_t>(work_cap);
     for (int k = 0; k < SH_COUNT; ++k) plan_host_.blocks[k] = prefix[k * (nf + 1) + nf];
     plan_dev_ = plan_host_;
     plan_dev_.frames = d_desc_dev_;

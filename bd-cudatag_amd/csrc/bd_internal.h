@@ -25,6 +25,13 @@ constexpr int kFitProbeHalf = 24;
 constexpr int kMaxProbes = 2 * kFitProbeHalf + 1;
 constexpr int kMaxSamples = 16;
 
+// Device frames are addressed by 32-bit byte offsets from their base in the
+// sampling kernels; a frame must span less than 4 GiB.
+inline bool frame_span_fits(int width, int height, int stride) {
+    return static_cast<uint64_t>(stride) * static_cast<uint64_t>(height - 1) + static_cast<uint64_t>(width) <=
+           0xffffffffull;
+}
+
 // Candidate keys sorted inside one workgroup's LDS before falling back to a
 // radix select over global memory.
 constexpr int kSortCap = 8192;
@@ -82,12 +89,21 @@ struct FrameDesc {
 
 // Prefix tables over frames: prefix[k][f] = first block of frame f for
 // kernel shape k; prefix[k][nframes] = total blocks.
-enum Shape : int { SH_TILE_ROW = 0, SH_ROW4 = 1, SH_CAND = 2, SH_FRAME = 3, SH_TSTAT = 4, SH_COUNT = 5 };
+enum Shape : int { SH_TILE_ROW = 0, SH_ROW4 = 1, SH_FRAME = 2, SH_TSTAT = 3, SH_COUNT = 4 };
+
+// Batch candidate list: a small header, then one entry per selected
+// candidate of any frame (frame << 10 | candidate index), appended by
+// k_select and consumed by the persistent k_fit and k_decode grids.
+enum WorkSlot : int { kWorkCount = 0, kWorkFitCursor = 1, kWorkDecodeCursor = 2, kWorkItems = 4 };
+constexpr int kFitGrid = 1024;     // 4 resident 256-thread workgroups per CU
+constexpr int kDecodeGrid = 1024;  // 4 resident 512-thread workgroups per CU
 
 struct Plan {
     const FrameDesc* frames;
     const int32_t* prefix;  // SH_COUNT * (nframes + 1)
+    uint32_t* work;         // kWorkItems + work_cap entries
     int32_t nframes;
+    int32_t work_cap;       // sum of max_candidates over frames
     int32_t blocks[SH_COUNT];
 };
 

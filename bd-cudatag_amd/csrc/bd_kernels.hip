@@ -81,6 +81,34 @@ __device__ inline uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
+// Cross-lane moves by DPP, no LDS traffic.  dpp_from_lower gives lane i the
+// value of lane i - 1 (lane 0 gets `edge`), dpp_from_upper the value of lane
+// i + 1 (lane 63 gets `edge`).  A source lane that is inactive in EXEC also
+// yields `edge`, so call these with the whole wave active.
+__device__ inline uint32_t dpp_from_lower(uint32_t edge, uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(edge), static_cast<int>(v),
+                                                             0x138 /* wave_shr:1 */, 0xf, 0xf, false));
+}
+__device__ inline uint32_t dpp_from_upper(uint32_t edge, uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(edge), static_cast<int>(v),
+                                                             0x130 /* wave_shl:1 */, 0xf, 0xf, false));
+}
+__device__ inline uint32_t wave_lane(uint32_t v, int lane) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), lane));
+}
+// Minimum of v over lanes 0-31, returned to every lane: row_shr 1/2/4/8
+// leaves each 16-lane row's minimum in its lane 15, row_bcast:15 folds row
+// 0 into row 1, and lane 31 is read back.
+__device__ inline uint32_t min_lanes_0_31(uint32_t v) {
+    const int all = static_cast<int>(0xffffffffu);
+    v = min(v, static_cast<uint32_t>(__builtin_amdgcn_update_dpp(all, static_cast<int>(v), 0x111, 0xf, 0xf, false)));
+    v = min(v, static_cast<uint32_t>(__builtin_amdgcn_update_dpp(all, static_cast<int>(v), 0x112, 0xf, 0xf, false)));
+    v = min(v, static_cast<uint32_t>(__builtin_amdgcn_update_dpp(all, static_cast<int>(v), 0x114, 0xf, 0xf, false)));
+    v = min(v, static_cast<uint32_t>(__builtin_amdgcn_update_dpp(all, static_cast<int>(v), 0x118, 0xf, 0xf, false)));
+    v = min(v, static_cast<uint32_t>(__builtin_amdgcn_update_dpp(all, static_cast<int>(v), 0x142, 0xa, 0xf, false)));
+    return wave_lane(v, 31);
+}
+
 // ---------------------------------------------------------------------------
 // (a) tile statistics
 // ---------------------------------------------------------------------------
@@ -211,6 +239,9 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
 // (reference gpu_detector.cu:174-230), the ROI rows (:282-357), and the
 // reset of the accumulators the next frame will use.
 __global__ __launch_bounds__(256) void k_frame_ref(Plan plan) {
+    // The batch's candidate list starts empty (block 0 exists in every
+    // launch, whichever frames are active).
+    if (blockIdx.x == 0 && threadIdx.x < kWorkItems) plan.work[threadIdx.x] = 0;
     const FrameDesc* d = plan.frames + blockIdx.x;
     if (!d->active) return;
     const int t = threadIdx.x;
@@ -1012,7 +1043,20 @@ __global__ __launch_bounds__(1024) void k_select(Plan plan) {
     for (int i = static_cast<int>(m) + t; i < p2; i += blockDim.x) keys[i] = ~0ull;
     __syncthreads();
     if (m > 1) bitonic_sort_lds(keys, p2);
-    for (uint32_t i = t; i < nsel; i += blockDim.x) d->candidates[i] = static_cast<uint32_t>(keys[i]);
+    __shared__ uint32_t s_work;
+    if (t == 0) s_work = atomicAdd(&plan.work[kWorkCount], nsel);
+    __syncthreads();
+    for (uint32_t i = t; i < nsel; i += blockDim.x) {
+        d->candidates[i] = static_cast<uint32_t>(keys[i]);
+        plan.work[kWorkItems + s_work + i] = (blockIdx.x << 10) | i;
+    }
+    // Slots past the selection read as "no quad" and "not decoded".
+    const uint32_t maxc = static_cast<uint32_t>(d->max_candidates);
+    for (uint32_t i = nsel * 8 + t; i < maxc * 8; i += blockDim.x) {
+        d->quads[i] = __int_as_float(0x7fc00000);
+        d->hull_quads[i] = __int_as_float(0x7fc00000);
+    }
+    for (uint32_t i = nsel + t; i < maxc; i += blockDim.x) d->dec[i].id = -1;
     if (t == 0) {
         d->scal[S_COUNTS + 0] = n;
         d->scal[S_NCAND] = nsel;
@@ -1024,7 +1068,14 @@ __global__ __launch_bounds__(1024) void k_select(Plan plan) {
 // (d) quad fit
 // ---------------------------------------------------------------------------
 
-__device__ inline float sample_bilinear(const uint8_t* image, int width, int height, int stride,
+// Frame bytes in the global address space, addressed by 32-bit offsets (the
+// host rejects device frames spanning 4 GiB or more), so that loads take
+// the frame base from scalar registers.
+typedef const uint8_t __attribute__((address_space(1))) gbyte;
+
+__device__ inline const gbyte* frame_bytes(const uint8_t* gray) { return (const gbyte*)gray; }
+
+__device__ inline float sample_bilinear(const gbyte* image, int width, int height, int stride,
                                         float x, float y) {
     if (!(x >= 0.0f && y >= 0.0f && x < static_cast<float>(width - 1) &&
           y < static_cast<float>(height - 1))) {
@@ -1034,12 +1085,12 @@ __device__ inline float sample_bilinear(const uint8_t* image, int width, int hei
     const int iy = static_cast<int>(y);
     const float fx = x - static_cast<float>(ix);
     const float fy = y - static_cast<float>(iy);
-    const uint8_t* p = image + static_cast<size_t>(iy) * stride + ix;
-    const float a = p[0], b = p[1], c = p[stride], e = p[stride + 1];
+    const uint32_t o = static_cast<uint32_t>(iy) * static_cast<uint32_t>(stride) + static_cast<uint32_t>(ix);
+    const float a = image[o], b = image[o + 1], c = image[o + stride], e = image[o + stride + 1];
     return (a * (1.0f - fx) + b * fx) * (1.0f - fy) + (c * (1.0f - fx) + e * fx) * fy;
 }
 
-__device__ inline float directional_rise(const uint8_t* image, int width, int height, int stride,
+__device__ inline float directional_rise(const gbyte* image, int width, int height, int stride,
                                          float x, float y, float dx, float dy) {
     const float ahead = sample_bilinear(image, width, height, stride, x + dx * 0.7f, y + dy * 0.7f);
     const float behind = sample_bilinear(image, width, height, stride, x - dx * 0.7f, y - dy * 0.7f);
@@ -1051,19 +1102,19 @@ __device__ inline float directional_rise(const uint8_t* image, int width, int he
 // several samples can be in flight at once; tap_value evaluates the same
 // expression as sample_bilinear.
 struct Tap {
-    const uint8_t* p;
+    uint32_t off;  // byte offset of the top-left neighbour
     float fx, fy;
     bool ok;
 };
 
-__device__ inline Tap make_tap(const uint8_t* image, int width, int height, int stride, float x, float y) {
+__device__ inline Tap make_tap(int width, int height, int stride, float x, float y) {
     Tap t;
     t.ok = x >= 0.0f && y >= 0.0f && x < static_cast<float>(width - 1) && y < static_cast<float>(height - 1);
     const int ix = t.ok ? static_cast<int>(x) : 0;
     const int iy = t.ok ? static_cast<int>(y) : 0;
     t.fx = x - static_cast<float>(ix);
     t.fy = y - static_cast<float>(iy);
-    t.p = image + static_cast<size_t>(iy) * stride + ix;
+    t.off = static_cast<uint32_t>(iy) * static_cast<uint32_t>(stride) + static_cast<uint32_t>(ix);
     return t;
 }
 
@@ -1181,6 +1232,12 @@ __device__ inline EdgeGeom edge_geometry(const float (*quad)[2], int edge, int d
     return g;
 }
 
+// Sampled bbox row s of `kept` (all rows when rows <= kMaxBoundaryRows;
+// collect_row_extremes :676-715).
+__device__ inline int fit_row(int s, int kept, int rows, int by0) {
+    return by0 + (kept > 1 ? (s * (rows - 1)) / (kept - 1) : 0);
+}
+
 // One workgroup (4 waves) per candidate.
 //  1. Row extremes of the component over (at most 1024 sampled) bbox rows,
 //     by binary search in each row's runs (collect_row_extremes :676-715).
@@ -1193,11 +1250,7 @@ __device__ inline EdgeGeom edge_geometry(const float (*quad)[2], int edge, int d
 //     evaluated in parallel into LDS, then one lane per (edge, sample) runs
 //     the peak rules and the parabola, and the moments are summed in the
 //     reference's 32-lane shuffle-down order (fit_edge_line_warp :881-1016).
-__global__ __launch_bounds__(256) void k_fit(Plan plan) {
-    int ci;
-    const FrameDesc* d = plan.frames + frame_of(plan, SH_CAND, blockIdx.x, &ci);
-    if (!d->active) return;
-    const int t = threadIdx.x;
+__device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
     const int wave = t / kWave, lane = t % kWave;
     float* out = d->quads + 8 * static_cast<size_t>(ci);
     float* hull_out = d->hull_quads + 8 * static_cast<size_t>(ci);
@@ -1205,15 +1258,18 @@ __global__ __launch_bounds__(256) void k_fit(Plan plan) {
         out[t] = __int_as_float(0x7fc00000);
         hull_out[t] = __int_as_float(0x7fc00000);
     }
-    if (ci >= static_cast<int>(d->scal[S_NCAND])) return;
 
     __shared__ uint32_t boundary[2 * kMaxBoundaryRows];
     __shared__ float pdot[8][kFitDirections];
     __shared__ int pidx[8][kFitDirections];
     __shared__ uint32_t extreme[kFitDirections];
-    __shared__ float mags[4][kMaxSamples][kMaxProbes];
-    __shared__ float red[4][32][5];
-    __shared__ int red_used[4][32];
+    __shared__ uint32_t hull_pts[kFitDirections];
+    // Probe magnitudes of the refinement; the row-extreme pass uses the same
+    // LDS before the refinement starts.
+    __shared__ uint32_t work_lds[4 * kMaxSamples * kMaxProbes];
+    static_assert(4 * kMaxSamples * kMaxProbes >= 2 * kMaxBoundaryRows + 1, "row-extreme work arrays");
+    float (*mags)[kMaxSamples][kMaxProbes] = reinterpret_cast<float (*)[kMaxSamples][kMaxProbes]>(work_lds);
+    __shared__ uint32_t wave_total[4];
     __shared__ float fit_quad[4][2];
     __shared__ float edge_lines[4][4];
     __shared__ int edge_have[4];
@@ -1229,33 +1285,108 @@ __global__ __launch_bounds__(256) void k_fit(Plan plan) {
     const int kept = min(rows, kMaxBoundaryRows);
     const int points = 2 * kept;
 
+    // Row extremes in three parallel steps: (1) per sampled row, the index
+    // range of runs overlapping [bx0, bx1] by two interleaved binary
+    // searches; (2) an exclusive scan of the range lengths; (3) every
+    // (row, run) pair of the flattened ranges checked by its own thread, the
+    // component's runs folded into the row's left/right by LDS atomics.
+    // Runs are x-ordered and the component lies inside its bbox, so this
+    // finds the same extremes as a scan of the whole row.
+    uint32_t* row_slot = work_lds;                      // first run slot of the range
+    uint32_t* row_off = work_lds + kMaxBoundaryRows;    // range offsets, [kept] = total
     for (int s = t; s < kept; s += blockDim.x) {
-        const int y = by0 + (kept > 1 ? (s * (rows - 1)) / (kept - 1) : 0);
+        const int y = fit_row(s, kept, rows, by0);
         const int n = static_cast<int>(d->row_count[y]);
         const uint32_t* rr = d->runs + static_cast<size_t>(y) * row_cap;
-        const uint32_t* lab = d->label + static_cast<size_t>(y) * row_cap;
-        int lo = 0, hi = n - 1, first = n;
-        while (lo <= hi) {
-            const int mid = (lo + hi) >> 1;
-            if (run_x1(rr[mid]) >= bx0) {
-                first = mid;
-                hi = mid - 1;
-            } else {
-                lo = mid + 1;
+        int a_lo = 0, a_hi = n, b_lo = 0, b_hi = n;
+        while (a_lo < a_hi || b_lo < b_hi) {
+            const bool sa = a_lo < a_hi, sb = b_lo < b_hi;
+            const int am = (a_lo + a_hi) >> 1, bm = (b_lo + b_hi) >> 1;
+            const uint32_t ra = sa ? rr[am] : 0u;
+            const uint32_t rb = sb ? rr[bm] : 0u;
+            if (sa) {
+                if (run_x1(ra) >= bx0) a_hi = am; else a_lo = am + 1;
+            }
+            if (sb) {
+                if (run_x0(rb) > bx1) b_hi = bm; else b_lo = bm + 1;
             }
         }
-        int left = -1, right = -1;
-        for (int k = first; k < n; ++k) {
-            const uint32_t run = rr[k];
-            const int x0 = run_x0(run);
-            const int x1 = run_x1(run);
-            if (x0 > bx1) break;
-            if (lab[k] != root) continue;
-            left = left < 0 ? x0 : min(left, x0);
-            right = right < 0 ? x1 : max(right, x1);
+        row_slot[s] = static_cast<uint32_t>(y) * row_cap + static_cast<uint32_t>(a_lo);
+        row_off[s] = static_cast<uint32_t>(max(b_lo - a_lo, 0));
+        boundary[2 * s] = 0xffffffffu;  // min x0
+        boundary[2 * s + 1] = 0u;       // max x1 + 1
+    }
+    __syncthreads();
+    {
+        uint32_t c[4], sum = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int s = 4 * t + j;
+            c[j] = s < kept ? row_off[s] : 0u;
+            sum += c[j];
         }
-        boundary[2 * s] = left < 0 ? 0xffffffffu : ((static_cast<uint32_t>(left) & 0xffffu) | (static_cast<uint32_t>(y) << 16));
-        boundary[2 * s + 1] = right < 0 ? 0xffffffffu : ((static_cast<uint32_t>(right) & 0xffffu) | (static_cast<uint32_t>(y) << 16));
+        uint32_t wtot;
+        uint32_t ex = wave_excl_scan(sum, lane, &wtot);
+        if (lane == 0) wave_total[wave] = wtot;
+        __syncthreads();
+        for (int w = 0; w < wave; ++w) ex += wave_total[w];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int s = 4 * t + j;
+            if (s < kept) row_off[s] = ex;
+            ex += c[j];
+        }
+        if (t == 0) row_off[kept] = wave_total[0] + wave_total[1] + wave_total[2] + wave_total[3];
+    }
+    __syncthreads();
+    {
+        const uint32_t total = row_off[kept];
+        const uint32_t per = (total + blockDim.x - 1) / blockDim.x;
+        uint32_t j = static_cast<uint32_t>(t) * per;
+        const uint32_t jend = min(j + per, total);
+        if (j < jend) {
+            int lo = 0, hi = kept - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (row_off[mid] <= j) lo = mid; else hi = mid - 1;
+            }
+            int srow = lo;
+            uint32_t next_off = row_off[srow + 1];
+            while (j < jend) {
+                uint32_t slot[4];
+                int sr[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    sr[u] = -1;
+                    slot[u] = 0;
+                    if (j < jend) {
+                        while (j >= next_off) next_off = row_off[++srow + 1];
+                        sr[u] = srow;
+                        slot[u] = row_slot[srow] + (j - row_off[srow]);
+                        ++j;
+                    }
+                }
+                uint32_t run[4], lab[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    run[u] = sr[u] >= 0 ? d->runs[slot[u]] : 0u;
+                    lab[u] = sr[u] >= 0 ? d->label[slot[u]] : ~root;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (lab[u] != root) continue;
+                    atomicMin(&boundary[2 * sr[u]], static_cast<uint32_t>(run_x0(run[u])));
+                    atomicMax(&boundary[2 * sr[u] + 1], static_cast<uint32_t>(run_x1(run[u]) + 1));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int s = t; s < kept; s += blockDim.x) {
+        const uint32_t y = static_cast<uint32_t>(fit_row(s, kept, rows, by0)) << 16;
+        const uint32_t l = boundary[2 * s], r = boundary[2 * s + 1];
+        boundary[2 * s] = l == 0xffffffffu ? 0xffffffffu : (l | y);
+        boundary[2 * s + 1] = r == 0u ? 0xffffffffu : ((r - 1u) | y);
     }
     __syncthreads();
 
@@ -1293,68 +1424,64 @@ __global__ __launch_bounds__(256) void k_fit(Plan plan) {
     }
     __syncthreads();
 
-    // Hull reduction on wave 0.
+    // Hull reduction on wave 0, one hull vertex (packed x | y << 16) per lane.
     if (wave == 0) {
-        // Ordered hull without repeats (:1172-1180), built by lane 0 and
-        // broadcast one vertex per lane.
-        uint32_t hv = 0xffffffffu;
-        int hc = 0;
-        if (lane == 0) {
-            uint32_t hull[kFitDirections];
-            for (int i = 0; i < kFitDirections; ++i) {
-                const uint32_t p = extreme[i];
-                if (p == 0xffffffffu) continue;
-                if (hc > 0 && hull[hc - 1] == p) continue;
-                hull[hc++] = p;
-            }
-            while (hc > 1 && hull[0] == hull[hc - 1]) --hc;
-            for (int i = 0; i < hc; ++i) extreme[i] = hull[i];
-        }
-        hc = __shfl(hc, 0, kWave);
+        // Ordered hull without repeats (:1172-1180): keep a valid extreme
+        // unless it equals the previous valid one, compact by ballot, then
+        // drop a last vertex equal to the first (consecutive kept vertices
+        // differ, so at most one).
+        const uint32_t p = lane < kFitDirections ? extreme[lane] : 0xffffffffu;
+        const bool valid = p != 0xffffffffu;
+        const uint64_t below = __ballot(valid) & ((1ull << lane) - 1ull);
+        const uint32_t pp = __shfl(p, below ? 63 - __clzll(static_cast<long long>(below)) : 0, kWave);
+        const bool keep = valid && (below == 0 || pp != p);
+        const uint64_t kmask = __ballot(keep);
+        int hc = __popcll(kmask);
+        if (keep) hull_pts[__popcll(kmask & ((1ull << lane) - 1ull))] = p;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-        if (lane < hc) hv = extreme[lane];
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        uint32_t hv = lane < hc ? hull_pts[lane] : 0xffffffffu;
+        if (hc > 1 && wave_lane(hv, 0) == wave_lane(hv, hc - 1)) {
+            --hc;
+            if (lane == hc) hv = 0xffffffffu;
+        }
+        // Smallest-triangle removal (hull_to_quad :736-831): drop the vertex
+        // whose triangle with its neighbours has the least area, the lowest
+        // index on ties, until four remain.  hc <= 32, so lanes 0-31 hold
+        // the polygon.
         int ok = hc >= 4 && hc <= kMaxHullVertices;
-        float vx = static_cast<float>(hv & 0xffffu);
-        float vy = static_cast<float>(hv >> 16);
         int count = hc;
         while (ok && count > 4) {
-            const int prev = (lane + count - 1) % count;
-            const int next = (lane + 1) % count;
-            const float pvx = __shfl(vx, prev, kWave), pvy = __shfl(vy, prev, kWave);
-            const float nvx = __shfl(vx, next, kWave), nvy = __shfl(vy, next, kWave);
-            const float ax = vx - pvx, ay = vy - pvy;
-            const float bx = nvx - vx, by = nvy - vy;
-            float area = fabsf(ax * by - ay * bx) * 0.5f;
-            int idx = lane;
-            if (lane >= count || !(area < FLT_MAX)) {
-                area = FLT_MAX;
-                idx = kWave;
-            }
-            for (int off = kWave / 2; off > 0; off >>= 1) {
-                const float oa = __shfl_xor(area, off, kWave);
-                const int oi = __shfl_xor(idx, off, kWave);
-                if (oa < area || (oa == area && oi < idx)) {
-                    area = oa;
-                    idx = oi;
-                }
-            }
-            if (idx >= kWave) {
+            const uint32_t first = wave_lane(hv, 0);
+            const uint32_t prev = dpp_from_lower(wave_lane(hv, count - 1), hv);
+            // The DPP move runs on every lane: a lane masked off in EXEC
+            // would read as out of range to its lower neighbour.
+            const uint32_t upper = dpp_from_upper(first, hv);
+            const uint32_t next = lane == count - 1 ? first : upper;
+            const float vx = static_cast<float>(hv & 0xffffu), vy = static_cast<float>(hv >> 16);
+            const float ax = vx - static_cast<float>(prev & 0xffffu), ay = vy - static_cast<float>(prev >> 16);
+            const float bx = static_cast<float>(next & 0xffffu) - vx, by = static_cast<float>(next >> 16) - vy;
+            const float area = fabsf(ax * by - ay * bx) * 0.5f;
+            const bool live = lane < count && area < FLT_MAX;
+            // Non-negative floats order as their bit patterns.
+            const uint32_t key = live ? __float_as_uint(area) : 0xffffffffu;
+            const uint32_t least = min_lanes_0_31(key);
+            const uint64_t tie = __ballot(live && key == least);
+            if (tie == 0) {
                 ok = 0;
                 break;
             }
-            const float sx = __shfl(vx, min(lane + 1, kWave - 1), kWave);
-            const float sy = __shfl(vy, min(lane + 1, kWave - 1), kWave);
-            if (lane >= idx) {
-                vx = sx;
-                vy = sy;
-            }
+            const int idx = __ffsll(static_cast<long long>(tie)) - 1;
+            const uint32_t shifted = dpp_from_upper(hv, hv);
+            if (lane >= idx) hv = shifted;
             --count;
         }
         float qx[4], qy[4];
         for (int i = 0; i < 4; ++i) {
-            qx[i] = __shfl(vx, i, kWave);
-            qy[i] = __shfl(vy, i, kWave);
+            const uint32_t q = wave_lane(hv, i);
+            qx[i] = static_cast<float>(q & 0xffffu);
+            qy[i] = static_cast<float>(q >> 16);
         }
         if (lane == 0) {
             if (ok) {
@@ -1432,62 +1559,64 @@ __global__ __launch_bounds__(256) void k_fit(Plan plan) {
     if (!quad_ok) return;
 
     if (d->refine) {
-        const uint8_t* gray = d->gray;
+        const gbyte* gray = frame_bytes(d->gray);
         const int W = d->width, H = d->height, stride = d->stride;
         for (int iteration = 0; iteration < kFitIterations; ++iteration) {
             const bool first_pass = iteration == 0;
             const EdgeGeom g = edge_geometry(fit_quad, wave, d->dim, first_pass);
             if (g.ok) {
-                // Every (sample, probe) rise of this edge, four per lane per
-                // round with all 32 byte loads issued before the arithmetic.
+                // Every (sample, probe) rise of this edge, three per lane per
+                // round with all 24 byte loads issued before the arithmetic
+                // (three keeps the kernel at 128 VGPRs, four waves per SIMD).
                 const int total = g.samples * g.probes;
-                for (int it0 = lane; it0 < total; it0 += 4 * kWave) {
-                    Tap ta[4], tb[4];
-                    int s_of[4], i_of[4];
+                for (int it0 = lane; it0 < total; it0 += 3 * kWave) {
+                    Tap ta[3], tb[3];
+                    int slot[3];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
+                    for (int u = 0; u < 3; ++u) {
                         const int it = it0 + u * kWave;
                         const bool live = it < total;
                         const int s = live ? it / g.probes : 0;
                         const int i = live ? it - s * g.probes : 0;
-                        s_of[u] = s;
-                        i_of[u] = i;
+                        slot[u] = s * kMaxProbes + i;
                         const float alpha = static_cast<float>(s + 1) / static_cast<float>(g.samples + 1);
                         const float px = g.ax + alpha * g.ex;
                         const float py = g.ay + alpha * g.ey;
                         const float tt = g.step * static_cast<float>(i - g.half);
                         const float x = px + g.nx * tt;
                         const float y = py + g.ny * tt;
-                        ta[u] = make_tap(gray, W, H, stride, x + g.nx * 0.7f, y + g.ny * 0.7f);
-                        tb[u] = make_tap(gray, W, H, stride, x - g.nx * 0.7f, y - g.ny * 0.7f);
+                        ta[u] = make_tap(W, H, stride, x + g.nx * 0.7f, y + g.ny * 0.7f);
+                        tb[u] = make_tap(W, H, stride, x - g.nx * 0.7f, y - g.ny * 0.7f);
                         ta[u].ok = ta[u].ok && live;
                         tb[u].ok = tb[u].ok && live;
                     }
-                    float va[4][4], vb[4][4];
+                    float va[3][4], vb[3][4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const uint8_t* p = ta[u].p;
-                        const uint8_t* q = tb[u].p;
-                        va[u][0] = ta[u].ok ? static_cast<float>(p[0]) : 0.0f;
-                        va[u][1] = ta[u].ok ? static_cast<float>(p[1]) : 0.0f;
-                        va[u][2] = ta[u].ok ? static_cast<float>(p[stride]) : 0.0f;
-                        va[u][3] = ta[u].ok ? static_cast<float>(p[stride + 1]) : 0.0f;
-                        vb[u][0] = tb[u].ok ? static_cast<float>(q[0]) : 0.0f;
-                        vb[u][1] = tb[u].ok ? static_cast<float>(q[1]) : 0.0f;
-                        vb[u][2] = tb[u].ok ? static_cast<float>(q[stride]) : 0.0f;
-                        vb[u][3] = tb[u].ok ? static_cast<float>(q[stride + 1]) : 0.0f;
+                    for (int u = 0; u < 3; ++u) {
+                        const uint32_t p = ta[u].off, q = tb[u].off;
+                        va[u][0] = ta[u].ok ? static_cast<float>(gray[p]) : 0.0f;
+                        va[u][1] = ta[u].ok ? static_cast<float>(gray[p + 1]) : 0.0f;
+                        va[u][2] = ta[u].ok ? static_cast<float>(gray[p + stride]) : 0.0f;
+                        va[u][3] = ta[u].ok ? static_cast<float>(gray[p + stride + 1]) : 0.0f;
+                        vb[u][0] = tb[u].ok ? static_cast<float>(gray[q]) : 0.0f;
+                        vb[u][1] = tb[u].ok ? static_cast<float>(gray[q + 1]) : 0.0f;
+                        vb[u][2] = tb[u].ok ? static_cast<float>(gray[q + stride]) : 0.0f;
+                        vb[u][3] = tb[u].ok ? static_cast<float>(gray[q + stride + 1]) : 0.0f;
                     }
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
+                    for (int u = 0; u < 3; ++u) {
                         if (it0 + u * kWave >= total) continue;
                         const float ahead = tap_value(ta[u], va[u][0], va[u][1], va[u][2], va[u][3]);
                         const float behind = tap_value(tb[u], vb[u][0], vb[u][1], vb[u][2], vb[u][3]);
-                        mags[wave][s_of[u]][i_of[u]] = (ahead < 0.0f || behind < 0.0f) ? 0.0f : ahead - behind;
+                        (&mags[wave][0][0])[slot[u]] = (ahead < 0.0f || behind < 0.0f) ? 0.0f : ahead - behind;
                     }
                 }
             }
-            __syncthreads();
-            if (lane < 32) {
+            // mags[wave] is written and read by this wave only.
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            {
                 float mx = 0.0f, my = 0.0f, mxx = 0.0f, mxy = 0.0f, myy = 0.0f;
                 int used = 0;
                 // Moments about the edge midpoint (see the oracle's note).
@@ -1540,44 +1669,36 @@ __global__ __launch_bounds__(256) void k_fit(Plan plan) {
                         used = 1;
                     }
                 }
-                red[wave][lane][0] = mx;
-                red[wave][lane][1] = my;
-                red[wave][lane][2] = mxx;
-                red[wave][lane][3] = mxy;
-                red[wave][lane][4] = myy;
-                red_used[wave][lane] = used;
-            }
-            __syncthreads();
-            if (lane == 0) {
-                int have = 0;
-                if (g.ok) {
-                    // The reference's shuffle-down tree over 32 lanes: after
-                    // offset `off`, lane i < off holds lane i + lane i+off.
-                    for (int off = 16; off > 0; off >>= 1) {
-                        for (int i = 0; i < off; ++i) {
-                            for (int k = 0; k < 5; ++k) red[wave][i][k] = red[wave][i][k] + red[wave][i + off][k];
-                            red_used[wave][i] += red_used[wave][i + off];
-                        }
-                    }
-                    const int used = red_used[wave][0];
-                    if (used >= 4) {
+                // The reference's shuffle-down tree over 32 lanes (lane i
+                // adds lane i + off, off = 16 .. 1); lanes 32-63 hold zeros
+                // and reduce in their own 32-lane segment.
+                for (int off = 16; off > 0; off >>= 1) {
+                    mx += __shfl_down(mx, off, 32);
+                    my += __shfl_down(my, off, 32);
+                    mxx += __shfl_down(mxx, off, 32);
+                    mxy += __shfl_down(mxy, off, 32);
+                    myy += __shfl_down(myy, off, 32);
+                    used += __shfl_down(used, off, 32);
+                }
+                if (lane == 0) {
+                    int have = 0;
+                    if (g.ok && used >= 4) {
                         const float inv = 1.0f / static_cast<float>(used);
-                        const float cx = red[wave][0][0] * inv;
-                        const float cy = red[wave][0][1] * inv;
-                        const float cxx = red[wave][0][2] * inv - cx * cx;
-                        const float cxy = red[wave][0][3] * inv - cx * cy;
-                        const float cyy = red[wave][0][4] * inv - cy * cy;
+                        const float cx = mx * inv;
+                        const float cy = my * inv;
+                        const float cxx = mxx * inv - cx * cx;
+                        const float cxy = mxy * inv - cx * cy;
+                        const float cyy = myy * inv - cy * cy;
                         float lnx, lny;
                         axis_normal(cxx, cxy, cyy, &lnx, &lny);
-                        const float rx = g.ax + 0.5f * g.ex, ry = g.ay + 0.5f * g.ey;
                         edge_lines[wave][0] = rx + cx;
                         edge_lines[wave][1] = ry + cy;
                         edge_lines[wave][2] = lnx;
                         edge_lines[wave][3] = lny;
                         have = 1;
                     }
+                    edge_have[wave] = have;
                 }
-                edge_have[wave] = have;
             }
             __syncthreads();
             if (t == 0) {
@@ -1606,6 +1727,32 @@ __global__ __launch_bounds__(256) void k_fit(Plan plan) {
             out[2 * i + 1] = fit_quad[i][1];
         }
         atomicAdd(&d->scal[S_COUNTS + 3], 1u);
+    }
+}
+
+// Next work-list item of a persistent workgroup: the first is its block
+// index, later ones come from the kernel's shared cursor.
+__device__ inline uint32_t next_item(uint32_t* cursor) {
+    __shared__ uint32_t s_next;
+    __syncthreads();
+    if (threadIdx.x == 0) s_next = gridDim.x + atomicAdd(cursor, 1u);
+    __syncthreads();
+    return s_next;
+}
+
+// Persistent workgroups over the batch's candidate list (k_select appends
+// frame << 10 | candidate for every selected candidate of every frame).
+__global__ __launch_bounds__(256) void k_fit(Plan plan) {
+    const uint32_t total = plan.work[kWorkCount];
+    for (uint32_t item = blockIdx.x; item < total; item = next_item(&plan.work[kWorkFitCursor])) {
+        // Uniform across the workgroup: scalar registers for the descriptor.
+        const uint32_t code = __builtin_amdgcn_readfirstlane(plan.work[kWorkItems + item]);
+        // An opaque copy of the thread index keeps the compiler from hoisting
+        // every per-thread address out of the item loop (which would hold
+        // them in registers across it and halve occupancy).
+        int t = threadIdx.x;
+        asm volatile("" : "+v"(t));
+        fit_candidate(plan.frames + (code >> 10), static_cast<int>(code & 1023u), t);
     }
 }
 
@@ -1643,18 +1790,9 @@ __device__ inline bool homography_from_unit_quad(const float (*q)[2], float h[9]
 // of wave 0 applies the reference's preference (refined first, highest
 // score, first on ties).  reference decode_quad :1291-1430 and
 // k_decode_quads :1438-1537.
-__global__ __launch_bounds__(512) void k_decode(Plan plan) {
-    int ci;
-    const FrameDesc* d = plan.frames + frame_of(plan, SH_CAND, blockIdx.x, &ci);
-    if (!d->active) return;
-    const int t = threadIdx.x;
+__device__ void decode_candidate(const FrameDesc* d, int ci, int t) {
     const int wave = t / kWave, lane = t % kWave;
     bd_detection* res = d->dec + ci;
-    if (ci >= d->max_candidates) return;
-    if (ci >= static_cast<int>(d->scal[S_NCAND])) {
-        if (t == 0) res->id = -1;
-        return;
-    }
     const float* qsrc = d->quads + 8 * static_cast<size_t>(ci);
     if (qsrc[0] != qsrc[0]) {
         if (t == 0) res->id = -1;
@@ -1690,7 +1828,7 @@ __global__ __launch_bounds__(512) void k_decode(Plan plan) {
         } else {
             const float x = (h[0] * u + h[1] * v + h[2]) / den;
             const float y = (h[3] * u + h[4] * v + h[5]) / den;
-            raw = sample_bilinear(d->gray, d->width, d->height, d->stride, x - 0.0f, y - 0.0f);
+            raw = sample_bilinear(frame_bytes(d->gray), d->width, d->height, d->stride, x - 0.0f, y - 0.0f);
             if (raw < 0.0f) cell_ok = false;
         }
     }
@@ -1811,6 +1949,17 @@ __global__ __launch_bounds__(512) void k_decode(Plan plan) {
             res->reserved = 0;
             atomicAdd(&d->scal[S_DECODED], 1u);
         }
+    }
+}
+
+__global__ __launch_bounds__(512) void k_decode(Plan plan) {
+    const uint32_t total = plan.work[kWorkCount];
+    for (uint32_t item = blockIdx.x; item < total; item = next_item(&plan.work[kWorkDecodeCursor])) {
+        // Uniform across the workgroup: scalar registers for the descriptor.
+        const uint32_t code = __builtin_amdgcn_readfirstlane(plan.work[kWorkItems + item]);
+        int t = threadIdx.x;
+        asm volatile("" : "+v"(t));
+        decode_candidate(plan.frames + (code >> 10), static_cast<int>(code & 1023u), t);
     }
 }
 
@@ -1988,8 +2137,8 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
         case 5: hipLaunchKernelGGL(k_stats, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
         case 6: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
         case 7: hipLaunchKernelGGL(k_select, dim3(nf), dim3(1024), 0, stream, plan); break;
-        case 8: hipLaunchKernelGGL(k_fit, dim3(plan.blocks[SH_CAND]), dim3(256), 0, stream, plan); break;
-        case 9: hipLaunchKernelGGL(k_decode, dim3(plan.blocks[SH_CAND]), dim3(512), 0, stream, plan); break;
+        case 8: hipLaunchKernelGGL(k_fit, dim3(min(plan.work_cap, kFitGrid)), dim3(256), 0, stream, plan); break;
+        case 9: hipLaunchKernelGGL(k_decode, dim3(min(plan.work_cap, kDecodeGrid)), dim3(512), 0, stream, plan); break;
         case 10: hipLaunchKernelGGL(k_finalize, dim3(nf), dim3(256), 0, stream, plan); break;
         default: return hipErrorInvalidValue;
     }

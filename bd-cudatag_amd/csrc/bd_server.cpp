@@ -323,7 +323,8 @@ int64_t bd_server_submit(bd_server* server, int32_t class_index, const uint8_t* 
                          int32_t stride) {
     if (!server || !gray_device) return BD_ERROR_INVALID_ARGUMENT;
     const auto& classes = server->server.classes();
-    if (class_index < 0 || class_index >= static_cast<int>(classes.size()) || stride < classes[class_index].width)
+    if (class_index < 0 || class_index >= static_cast<int>(classes.size()) || stride < classes[class_index].width ||
+        !bd::frame_span_fits(classes[class_index].width, classes[class_index].height, stride))
         return BD_ERROR_INVALID_ARGUMENT;
     return server->server.submit(class_index, gray_device, stride, bd::Clock::now());
 }
@@ -355,7 +356,8 @@ int32_t bd_server_simulate(bd_server* server, const bd_stream_spec* streams, int
     for (int s = 0; s < nstreams; ++s) {
         const bd_stream_spec& st = streams[s];
         if (st.class_index < 0 || st.class_index >= static_cast<int>(classes.size()) || st.fps <= 0.0 ||
-            st.frame_count < 1 || !st.frames)
+            st.frame_count < 1 || !st.frames || st.stride < classes[st.class_index].width ||
+            !bd::frame_span_fits(classes[st.class_index].width, classes[st.class_index].height, st.stride))
             return BD_ERROR_INVALID_ARGUMENT;
         uint64_t rng = seed ^ (0x632be59bd9b4e019ull * static_cast<uint64_t>(s + 1));
         double t = 0.0;

@@ -130,7 +130,11 @@ BD_API int32_t bd_detector_detect_host(bd_detector* detector, const uint8_t* gra
                                        int32_t stride, bd_detection* out,
                                        int32_t out_capacity);
 
-/* reference bdcudatag.h:133-139: same, reading a HIP device pointer in place. */
+/*
+ * reference bdcudatag.h:133-139: same, reading a HIP device pointer in place.
+ * A device frame must span less than 4 GiB (stride * (height - 1) + width);
+ * larger spans return BD_ERROR_INVALID_ARGUMENT.
+ */
 BD_API int32_t bd_detector_detect_device(bd_detector* detector, const uint8_t* gray_device,
                                          int32_t stride, bd_detection* out,
                                          int32_t out_capacity);

@@ -159,6 +159,9 @@ def test_device_pointer_path():
     pbuf = devmem.DeviceBuffer(padded)
     c = det.detect_device(pbuf.ptr, 704)
     assert [d["id"] for d in c] == [11] and np.array_equal(c[0]["corners"], b[0]["corners"])
+    # a device frame spanning 4 GiB or more is rejected before any launch
+    with pytest.raises(RuntimeError, match="4 GiB"):
+        det.detect_device(buf.ptr, 9_000_000)
 
 
 def test_stats_and_thread_safety():
