@@ -140,7 +140,8 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     }
     size_t work_cap = 0;
     for (int f = 0; f < nf; ++f) work_cap += static_cast<size_t>(maxc[f]);
-    const size_t o_work = take((kWorkItems + work_cap) * 4);
+    const size_t work_words = kWorkItems + static_cast<size_t>(kHeadRanks) * nf + work_cap;
+    const size_t o_work = take(work_words * 4);
     const size_t o_scal = take(static_cast<size_t>(nf) * S_COUNT * 4);
     const size_t o_out = take(static_cast<size_t>(nf) * max_det_ * sizeof(bd_detection));
     const size_t o_desc_h = take(static_cast<size_t>(nf) * sizeof(FrameDesc));
@@ -247,7 +248,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         st = hipMemcpy(d_scal_, scal.data(), scal.size() * 4, hipMemcpyHostToDevice);
     }
     if (st == hipSuccess) st = hipMemset(d_input_, 0, std::max<size_t>(input_bytes_, 1));
-    if (st == hipSuccess) st = hipMemset(d_work_, 0, (kWorkItems + work_cap) * 4);
+    if (st == hipSuccess) st = hipMemset(d_work_, 0, work_words * 4);
     if (st != hipSuccess) {
         if (error) *error = hip_error("detector setup copy", st);
         return false;

@@ -36,8 +36,11 @@ inline bool frame_span_fits(int width, int height, int stride) {
 // radix select over global memory.
 constexpr int kSortCap = 8192;
 // Runs of one 8-row band joined in LDS by k_binarize_runs; a band with more
-// runs is joined row by row in global memory by k_merge instead.
-constexpr int kBandRuns = 8192;
+// runs is joined row by row in global memory by k_merge instead.  4096 runs
+// (32 KB of band arrays) keeps four workgroups resident per CU and covers
+// noisy 1280-2048 px wide frames (about 1100 runs per band on average, 2100
+// at most, on the eight-tag 1280x800 workload).
+constexpr int kBandRuns = 4096;
 
 // Per-frame scalar block (u32 each).  Accumulators are cleared by the kernel
 // that consumes them, so a replayed graph needs no memset nodes.
@@ -93,15 +96,19 @@ enum Shape : int { SH_TILE_ROW = 0, SH_ROW4 = 1, SH_FRAME = 2, SH_TSTAT = 3, SH_
 
 // Batch candidate list: a small header, then one entry per selected
 // candidate of any frame (frame << 10 | candidate index), appended by
-// k_select and consumed by the persistent k_fit and k_decode grids.
-enum WorkSlot : int { kWorkCount = 0, kWorkFitCursor = 1, kWorkDecodeCursor = 2, kWorkItems = 4 };
+// k_select and consumed by the persistent k_fit and k_decode grids.  The
+// first kHeadRanks candidates of every frame (its largest) sit in a head
+// region of kHeadRanks * nframes entries that is handed out first; the rest
+// follow in a tail region.
+enum WorkSlot : int { kWorkCount = 0, kWorkFitCursor = 1, kWorkDecodeCursor = 2, kWorkHeadCount = 3, kWorkItems = 4 };
+constexpr int kHeadRanks = 2;
 constexpr int kFitGrid = 1024;     // 4 resident 256-thread workgroups per CU
 constexpr int kDecodeGrid = 1024;  // 4 resident 512-thread workgroups per CU
 
 struct Plan {
     const FrameDesc* frames;
     const int32_t* prefix;  // SH_COUNT * (nframes + 1)
-    uint32_t* work;         // kWorkItems + work_cap entries
+    uint32_t* work;         // kWorkItems + kHeadRanks * nframes + work_cap entries
     int32_t nframes;
     int32_t work_cap;       // sum of max_candidates over frames
     int32_t blocks[SH_COUNT];

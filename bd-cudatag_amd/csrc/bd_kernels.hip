@@ -66,19 +66,34 @@ __device__ inline int frame_of(const Plan& plan, int shape, int block, int* loca
     return lo;
 }
 
+// Inclusive prefix sum over the wave by DPP (the whole wave must be
+// active): row_shr 1/2/4/8 scan each 16-lane row, row_bcast:15 carries row
+// 0 into row 1 and row 2 into row 3, row_bcast:31 carries lanes 0-31 into
+// rows 2 and 3.
+template <int kCtrl, int kRowMask>
+__device__ inline uint32_t dpp_add(uint32_t x) {
+    return x + static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), kCtrl, kRowMask, 0xf, false));
+}
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v) {
+    v = dpp_add<0x111, 0xf>(v);
+    v = dpp_add<0x112, 0xf>(v);
+    v = dpp_add<0x114, 0xf>(v);
+    v = dpp_add<0x118, 0xf>(v);
+    v = dpp_add<0x142, 0xa>(v);
+    v = dpp_add<0x143, 0xc>(v);
+    return v;
+}
+
 __device__ inline uint32_t wave_excl_scan(uint32_t v, int lane, uint32_t* total) {
-    uint32_t x = v;
-    for (int off = 1; off < kWave; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off, kWave);
-        if (lane >= off) x += y;
-    }
-    *total = __shfl(x, kWave - 1, kWave);
+    (void)lane;
+    const uint32_t x = wave_incl_scan(v);
+    *total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), kWave - 1));
     return x - v;
 }
 
 __device__ inline uint32_t wave_sum(uint32_t v) {
-    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    return v;
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(wave_incl_scan(v)), kWave - 1));
 }
 
 // Cross-lane moves by DPP, no LDS traffic.  dpp_from_lower gives lane i the
@@ -608,36 +623,48 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
         const uint32_t base = static_cast<uint32_t>(y) * static_cast<uint32_t>(row_cap);
         const uint32_t lbase = s_off[r];
         uint32_t carry_s = 0, carry_e = 0;
-        for (int w0 = 0; w0 < w64; w0 += kWave) {
-            const int w = w0 + lane;
-            uint64_t starts = 0, ends = 0;
-            if (w < w64) edges(r, w, &starts, &ends);
-            uint32_t tot_s, tot_e;
-            uint32_t ks = carry_s + wave_excl_scan(__popcll(starts), lane, &tot_s);
-            uint32_t ke = carry_e + wave_excl_scan(__popcll(ends), lane, &tot_e);
-            while (starts) {
-                const int b = __ffsll(static_cast<unsigned long long>(starts)) - 1;
-                starts &= starts - 1;
-                const uint32_t k = ks++;
-                const uint32_t slot = base + k;
-                if (local) {
-                    lds16[2 * (lbase + k)] = static_cast<uint16_t>(w * 64 + b);
-                } else {
-                    runs16[2 * slot] = static_cast<uint16_t>(w * 64 + b);
-                    d->label[slot] = slot;
-                    d->area[slot] = 0;
-                    uint4* bb = reinterpret_cast<uint4*>(d->bbox + 4 * static_cast<size_t>(slot));
-                    *bb = make_uint4(0xffffffffu, 0xffffffffu, 0u, 0u);
-                }
+        // One 32-pixel half word per lane, starts and ends extracted in
+        // one loop.
+        const int nh = 2 * w64;
+        for (int h0 = 0; h0 < nh; h0 += kWave) {
+            const int hw = h0 + lane;
+            uint32_t starts = 0, ends = 0;
+            if (hw < nh) {
+                uint64_t st, en;
+                edges(r, hw >> 1, &st, &en);
+                const int sh = 32 * (hw & 1);
+                starts = static_cast<uint32_t>(st >> sh);
+                ends = static_cast<uint32_t>(en >> sh);
             }
-            while (ends) {
-                const int b = __ffsll(static_cast<unsigned long long>(ends)) - 1;
-                ends &= ends - 1;
-                const uint32_t k = ke++;
-                if (local) {
-                    lds16[2 * (lbase + k) + 1] = static_cast<uint16_t>(w * 64 + b);
-                } else {
-                    runs16[2 * (base + k) + 1] = static_cast<uint16_t>((w * 64 + b) | 0x8000);
+            uint32_t tot_s, tot_e;
+            uint32_t ks = carry_s + wave_excl_scan(__popc(starts), lane, &tot_s);
+            uint32_t ke = carry_e + wave_excl_scan(__popc(ends), lane, &tot_e);
+            const int xb = hw * 32;
+            while (starts | ends) {
+                if (starts) {
+                    const int b = __ffs(starts) - 1;
+                    starts &= starts - 1;
+                    const uint32_t k = ks++;
+                    const uint32_t slot = base + k;
+                    if (local) {
+                        lds16[2 * (lbase + k)] = static_cast<uint16_t>(xb + b);
+                    } else {
+                        runs16[2 * slot] = static_cast<uint16_t>(xb + b);
+                        d->label[slot] = slot;
+                        d->area[slot] = 0;
+                        uint4* bb = reinterpret_cast<uint4*>(d->bbox + 4 * static_cast<size_t>(slot));
+                        *bb = make_uint4(0xffffffffu, 0xffffffffu, 0u, 0u);
+                    }
+                }
+                if (ends) {
+                    const int b = __ffs(ends) - 1;
+                    ends &= ends - 1;
+                    const uint32_t k = ke++;
+                    if (local) {
+                        lds16[2 * (lbase + k) + 1] = static_cast<uint16_t>(xb + b);
+                    } else {
+                        runs16[2 * (base + k) + 1] = static_cast<uint16_t>((xb + b) | 0x8000);
+                    }
                 }
             }
             carry_s += tot_s;
@@ -651,11 +678,21 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
     // Band-local union-find (k_merge_runs :525-556 on the band's rows), all
     // row pairs at once: LDS atomics keep it coherent and a band's trees are
     // at most a few hops deep.
+    // Row offsets in registers; the row of band run i is the number of row
+    // starts at or below i, minus one.
+    uint32_t so[kTile + 1];
+#pragma unroll
+    for (int q = 0; q <= kTile; ++q) so[q] = q <= nrows ? s_off[q] : nb;
+    const auto row_of = [&so](uint32_t i) {
+        int r = 0;
+#pragma unroll
+        for (int q = 1; q < kTile; ++q) r += i >= so[q] ? 1 : 0;
+        return r;
+    };
     for (uint32_t i = t; i < nb; i += blockDim.x) b_lab[i] = i;
     __syncthreads();
-    for (uint32_t i = s_off[1] + t; i < nb; i += blockDim.x) {
-        int r = 1;
-        while (i >= s_off[r + 1]) ++r;
+    for (uint32_t i = so[1] + t; i < nb; i += blockDim.x) {
+        const int r = row_of(i);
         const uint32_t beg = s_off[r], pbeg = s_off[r - 1];
         const uint32_t run = b_run[i];
         const int x0 = run_x0(run), x1 = run_x1(run);
@@ -678,12 +715,10 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
     for (uint32_t i = t; i < nb; i += blockDim.x) b_lab[i] = lds_find(b_lab, i);
     __syncthreads();
     for (uint32_t i = t; i < nb; i += blockDim.x) {
-        int r = 0;
-        while (i >= s_off[r + 1]) ++r;
+        const int r = row_of(i);
         const uint32_t slot = static_cast<uint32_t>(y0 + r) * static_cast<uint32_t>(row_cap) + (i - s_off[r]);
         const uint32_t root = b_lab[i];
-        int rr = 0;
-        while (root >= s_off[rr + 1]) ++rr;
+        const int rr = row_of(root);
         const uint32_t rslot = static_cast<uint32_t>(y0 + rr) * static_cast<uint32_t>(row_cap) + (root - s_off[rr]);
         d->runs[slot] = b_run[i] | (root == i ? kRootFlag : 0u);
         d->label[slot] = rslot;
@@ -1043,12 +1078,21 @@ __global__ __launch_bounds__(1024) void k_select(Plan plan) {
     for (int i = static_cast<int>(m) + t; i < p2; i += blockDim.x) keys[i] = ~0ull;
     __syncthreads();
     if (m > 1) bitonic_sort_lds(keys, p2);
-    __shared__ uint32_t s_work;
-    if (t == 0) s_work = atomicAdd(&plan.work[kWorkCount], nsel);
+    // The frame's largest candidates go to the head of the batch list, so
+    // that the longest fits of the batch start first.
+    __shared__ uint32_t s_head, s_tail;
+    const uint32_t nhead = min(nsel, static_cast<uint32_t>(kHeadRanks));
+    if (t == 0) {
+        s_head = atomicAdd(&plan.work[kWorkHeadCount], nhead);
+        s_tail = atomicAdd(&plan.work[kWorkCount], nsel - nhead);
+    }
     __syncthreads();
+    uint32_t* head = plan.work + kWorkItems;
+    uint32_t* tail = head + kHeadRanks * plan.nframes;
     for (uint32_t i = t; i < nsel; i += blockDim.x) {
         d->candidates[i] = static_cast<uint32_t>(keys[i]);
-        plan.work[kWorkItems + s_work + i] = (blockIdx.x << 10) | i;
+        const uint32_t code = (blockIdx.x << 10) | i;
+        if (i < nhead) head[s_head + i] = code; else tail[s_tail + i - nhead] = code;
     }
     // Slots past the selection read as "no quad" and "not decoded".
     const uint32_t maxc = static_cast<uint32_t>(d->max_candidates);
@@ -1259,16 +1303,18 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
         hull_out[t] = __int_as_float(0x7fc00000);
     }
 
-    __shared__ uint32_t boundary[2 * kMaxBoundaryRows];
+    __shared__ __align__(16) uint32_t boundary[2 * kMaxBoundaryRows];
     __shared__ float pdot[8][kFitDirections];
-    __shared__ int pidx[8][kFitDirections];
+    __shared__ uint32_t ppoint[8][kFitDirections];
     __shared__ uint32_t extreme[kFitDirections];
     __shared__ uint32_t hull_pts[kFitDirections];
     // Probe magnitudes of the refinement; the row-extreme pass uses the same
     // LDS before the refinement starts.
-    __shared__ uint32_t work_lds[4 * kMaxSamples * kMaxProbes];
-    static_assert(4 * kMaxSamples * kMaxProbes >= 2 * kMaxBoundaryRows + 1, "row-extreme work arrays");
-    float (*mags)[kMaxSamples][kMaxProbes] = reinterpret_cast<float (*)[kMaxSamples][kMaxProbes]>(work_lds);
+    // Rows padded to a multiple of four floats for 16-byte LDS reads.
+    constexpr int kProbeStride = (kMaxProbes + 3) & ~3;
+    __shared__ __align__(16) uint32_t work_lds[4 * kMaxSamples * kProbeStride];
+    static_assert(4 * kMaxSamples * kProbeStride >= 2 * kMaxBoundaryRows + 1, "row-extreme work arrays");
+    float (*mags)[kMaxSamples][kProbeStride] = reinterpret_cast<float (*)[kMaxSamples][kProbeStride]>(work_lds);
     __shared__ uint32_t wave_total[4];
     __shared__ float fit_quad[4][2];
     __shared__ float edge_lines[4][4];
@@ -1340,11 +1386,12 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
     }
     __syncthreads();
     {
+        // Item j goes to thread j % 256 (coalesced run and label loads); a
+        // thread's row only moves forward, from a binary search for its
+        // first item.
         const uint32_t total = row_off[kept];
-        const uint32_t per = (total + blockDim.x - 1) / blockDim.x;
-        uint32_t j = static_cast<uint32_t>(t) * per;
-        const uint32_t jend = min(j + per, total);
-        if (j < jend) {
+        uint32_t j = static_cast<uint32_t>(t);
+        if (j < total) {
             int lo = 0, hi = kept - 1;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
@@ -1352,31 +1399,50 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
             }
             int srow = lo;
             uint32_t next_off = row_off[srow + 1];
-            while (j < jend) {
-                uint32_t slot[4];
-                int sr[4];
+            const uint32_t step = blockDim.x;
+            while (j < total) {
+                constexpr int kItems = 8;
+                uint32_t slot[kItems];
+                int sr[kItems];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kItems; ++u) {
                     sr[u] = -1;
                     slot[u] = 0;
-                    if (j < jend) {
+                    if (j < total) {
                         while (j >= next_off) next_off = row_off[++srow + 1];
                         sr[u] = srow;
                         slot[u] = row_slot[srow] + (j - row_off[srow]);
-                        ++j;
+                        j += step;
                     }
                 }
-                uint32_t run[4], lab[4];
+                uint32_t lab[kItems], run[kItems];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    run[u] = sr[u] >= 0 ? d->runs[slot[u]] : 0u;
+                for (int u = 0; u < kItems; ++u) {
                     lab[u] = sr[u] >= 0 ? d->label[slot[u]] : ~root;
+                    run[u] = sr[u] >= 0 ? d->runs[slot[u]] : 0u;
                 }
+                const uint64_t below = (1ull << lane) - 1ull;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (lab[u] != root) continue;
-                    atomicMin(&boundary[2 * sr[u]], static_cast<uint32_t>(run_x0(run[u])));
-                    atomicMax(&boundary[2 * sr[u] + 1], static_cast<uint32_t>(run_x1(run[u]) + 1));
+                for (int u = 0; u < kItems; ++u) {
+                    // Consecutive lanes hold consecutive items, so a row's
+                    // items in this wave are a contiguous lane segment: only
+                    // the segment's first match (left) and last match (right)
+                    // update LDS.  Segment starts: lanes whose row differs
+                    // from the lower neighbour's.
+                    const bool hit = lab[u] == root;
+                    const uint64_t m = __ballot(hit);
+                    const uint32_t lower_row = dpp_from_lower(0xfffffffeu, static_cast<uint32_t>(sr[u]));
+                    const uint64_t starts = __ballot(static_cast<uint32_t>(sr[u]) != lower_row) | 1ull;
+                    if (!hit) continue;
+                    const uint64_t seg_lo = starts & (below | (1ull << lane));   // my segment start and below
+                    const int s0 = 63 - __clzll(static_cast<long long>(seg_lo));
+                    const uint64_t above = starts & ~(below | (1ull << lane));
+                    const int s1 = above ? __ffsll(static_cast<long long>(above)) - 1 : 64;
+                    const uint64_t seg = (s1 >= 64 ? ~0ull : ((1ull << s1) - 1ull)) & ~((1ull << s0) - 1ull);
+                    const bool first = (m & seg & below) == 0;
+                    const bool last = (m & seg & ~(below | (1ull << lane))) == 0;
+                    if (first) atomicMin(&boundary[2 * sr[u]], static_cast<uint32_t>(run_x0(run[u])));
+                    if (last) atomicMax(&boundary[2 * sr[u] + 1], static_cast<uint32_t>(run_x1(run[u]) + 1));
                 }
             }
         }
@@ -1391,24 +1457,32 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
     __syncthreads();
 
     {
+        // 8 consecutive slices of the boundary (multiples of four points,
+        // read four at a time) x 32 directions; slices are merged in order,
+        // so the result is still the first maximum in boundary order.
         const int dir = t & 31, part = t >> 5;
-        const int chunk = (points + 7) / 8;
+        const int chunk = (((points + 7) / 8) + 3) & ~3;
         const int begin = part * chunk;
         const int end = min(points, begin + chunk);
         const float ux = c_dir_cos[dir], uy = c_dir_sin[dir];
         float best = -FLT_MAX;
-        int bi = -1;
-        for (int i = begin; i < end; ++i) {
-            const uint32_t p = boundary[i];
-            if (p == 0xffffffffu) continue;
-            const float dd = static_cast<float>(p & 0xffffu) * ux + static_cast<float>(p >> 16) * uy;
-            if (dd > best) {
-                best = dd;
-                bi = i;
+        uint32_t bp = 0xffffffffu;
+        for (int i = begin; i < end; i += 4) {
+            const uint4 q = *reinterpret_cast<const uint4*>(boundary + i);
+            const uint32_t pv[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const uint32_t p = pv[c];
+                if (i + c >= end || p == 0xffffffffu) continue;
+                const float dd = static_cast<float>(p & 0xffffu) * ux + static_cast<float>(p >> 16) * uy;
+                if (dd > best) {
+                    best = dd;
+                    bp = p;
+                }
             }
         }
         pdot[part][dir] = best;
-        pidx[part][dir] = bi;
+        ppoint[part][dir] = bp;
     }
     __syncthreads();
     if (t < kFitDirections) {
@@ -1417,7 +1491,7 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
         for (int p = 0; p < 8; ++p) {
             if (pdot[p][t] > best) {
                 best = pdot[p][t];
-                point = boundary[pidx[p][t]];
+                point = ppoint[p][t];
             }
         }
         extreme[t] = point;
@@ -1578,7 +1652,7 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
                         const bool live = it < total;
                         const int s = live ? it / g.probes : 0;
                         const int i = live ? it - s * g.probes : 0;
-                        slot[u] = s * kMaxProbes + i;
+                        slot[u] = s * kProbeStride + i;
                         const float alpha = static_cast<float>(s + 1) / static_cast<float>(g.samples + 1);
                         const float px = g.ax + alpha * g.ex;
                         const float py = g.ay + alpha * g.ey;
@@ -1626,25 +1700,42 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
                     const float alpha = static_cast<float>(lane + 1) / static_cast<float>(g.samples + 1);
                     const float px = g.ax + alpha * g.ex;
                     const float py = g.ay + alpha * g.ey;
+                    // First strictly largest rise (:934-945), four probes per
+                    // LDS read.
                     float best_mag = 0.0f;
                     int best = -1;
-                    for (int i = 0; i < g.probes; ++i) {
-                        if (mg[i] > best_mag) {
-                            best_mag = mg[i];
-                            best = i;
+                    for (int q = 0; q < g.probes; q += 4) {
+                        const float4 v = *reinterpret_cast<const float4*>(mg + q);
+                        const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            if (q + c < g.probes && vv[c] > best_mag) {
+                                best_mag = vv[c];
+                                best = q + c;
+                            }
                         }
                     }
+                    // The nearest-peak rule (:952-966): the first i (in
+                    // increasing order) below best - 1 that is a local
+                    // maximum of at least 0.7 best_mag with a valley under
+                    // half its height before best.  Walking down from
+                    // best - 2 with the running minimum of mg[i+1 .. best-1]
+                    // finds the same i in one pass.
                     if (first_pass && best > 2) {
                         const float accept = 0.70f * best_mag;
-                        for (int i = 1; i + 1 < best; ++i) {
-                            if (mg[i] < accept || mg[i] < mg[i - 1] || mg[i] < mg[i + 1]) continue;
-                            float valley = FLT_MAX;
-                            for (int k = i + 1; k < best; ++k) valley = fminf(valley, mg[k]);
-                            if (valley < 0.5f * mg[i]) {
-                                best = i;
-                                best_mag = mg[i];
-                                break;
-                            }
+                        float valley = FLT_MAX;
+                        float right = mg[best - 1], here = mg[best - 2];
+                        int nearest = -1;
+                        for (int i = best - 2; i >= 1; --i) {
+                            const float left = mg[i - 1];
+                            valley = fminf(valley, right);
+                            if (!(here < accept || here < left || here < right) && valley < 0.5f * here) nearest = i;
+                            right = here;
+                            here = left;
+                        }
+                        if (nearest >= 0) {
+                            best = nearest;
+                            best_mag = mg[nearest];
                         }
                     }
                     if (best >= 0 && best_mag >= 6.0f) {
@@ -1740,13 +1831,21 @@ __device__ inline uint32_t next_item(uint32_t* cursor) {
     return s_next;
 }
 
+// Entry `item` of the batch candidate list: the head region (each frame's
+// largest candidates) first, then the tail region.
+__device__ inline uint32_t work_item(const Plan& plan, uint32_t item, uint32_t nhead) {
+    const uint32_t* head = plan.work + kWorkItems;
+    return item < nhead ? head[item] : head[kHeadRanks * plan.nframes + (item - nhead)];
+}
+
 // Persistent workgroups over the batch's candidate list (k_select appends
 // frame << 10 | candidate for every selected candidate of every frame).
 __global__ __launch_bounds__(256) void k_fit(Plan plan) {
-    const uint32_t total = plan.work[kWorkCount];
+    const uint32_t nhead = plan.work[kWorkHeadCount];
+    const uint32_t total = nhead + plan.work[kWorkCount];
     for (uint32_t item = blockIdx.x; item < total; item = next_item(&plan.work[kWorkFitCursor])) {
         // Uniform across the workgroup: scalar registers for the descriptor.
-        const uint32_t code = __builtin_amdgcn_readfirstlane(plan.work[kWorkItems + item]);
+        const uint32_t code = __builtin_amdgcn_readfirstlane(work_item(plan, item, nhead));
         // An opaque copy of the thread index keeps the compiler from hoisting
         // every per-thread address out of the item loop (which would hold
         // them in registers across it and halve occupancy).
@@ -1953,10 +2052,11 @@ __device__ void decode_candidate(const FrameDesc* d, int ci, int t) {
 }
 
 __global__ __launch_bounds__(512) void k_decode(Plan plan) {
-    const uint32_t total = plan.work[kWorkCount];
+    const uint32_t nhead = plan.work[kWorkHeadCount];
+    const uint32_t total = nhead + plan.work[kWorkCount];
     for (uint32_t item = blockIdx.x; item < total; item = next_item(&plan.work[kWorkDecodeCursor])) {
         // Uniform across the workgroup: scalar registers for the descriptor.
-        const uint32_t code = __builtin_amdgcn_readfirstlane(plan.work[kWorkItems + item]);
+        const uint32_t code = __builtin_amdgcn_readfirstlane(work_item(plan, item, nhead));
         int t = threadIdx.x;
         asm volatile("" : "+v"(t));
         decode_candidate(plan.frames + (code >> 10), static_cast<int>(code & 1023u), t);
