@@ -115,8 +115,8 @@ struct Plan {
 };
 
 // Kernels in pipeline order: tile_stats, frame_ref, binarize_runs, merge,
-// flatten, stats, collect, select, fit, decode, finalize.
-constexpr int kStageCount = 11;
+// stats, collect, select, fit, decode, finalize.
+constexpr int kStageCount = 10;
 
 // Enqueues one kernel (0..kStageCount-1) for every frame of the plan.
 hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream);

@@ -1,13 +1,13 @@
 // HIP kernels of the detection pipeline for gfx950 (CDNA4, wave64).
 //
-// Eleven kernels, each with a grid that spans every frame of the plan:
+// Ten kernels, each with a grid that spans every frame of the plan:
 //   k_tile_stats      8x8 tile min/max/mean, histograms, active extents   (a)
 //   k_frame_ref       quartile span + Otsu split, row ROI                 (a)
 //   k_binarize_runs   pooled threshold, 64-px bit packing, black runs per
 //                     row, band-local union-find in LDS                    (b)
 //   k_merge           8-connected union across band boundaries            (b)
-//   k_flatten         band roots resolved to component roots              (b)
-//   k_stats           root labels, component area and bounding box        (b)
+//   k_stats           band roots resolved to component roots, root labels,
+//                     component area and bounding box                     (b)
 //   k_collect         tag-shaped component filter                         (c)
 //   k_select          largest-first candidate order (LDS bitonic sort)    (c)
 //   k_fit             row extremes, hull probe, quad, edge refinement     (d)
@@ -764,17 +764,69 @@ __device__ inline void uf_union(uint32_t* label, uint32_t a, uint32_t b) {
     }
 }
 
-constexpr int kPairSlots = 2048;
+// k_merge LDS: a pair hash set of kPairSlots entries and kMergeStage runs
+// per staged row (rows with more runs are read from global memory); 20 KB
+// in all keeps eight workgroups resident per CU.
+constexpr int kPairSlots = 1024;
+constexpr int kMergeStage = 768;
+
+// Records the (smaller, larger) label pair in the LDS hash set, or joins
+// the pair directly when 16 probes find no room.
+__device__ inline void merge_pair(unsigned long long* pairs, uint32_t* label, uint32_t a, uint32_t b) {
+    const unsigned long long key = (static_cast<unsigned long long>(min(a, b)) << 32) | max(a, b);
+    const uint32_t h = static_cast<uint32_t>((key * 0x9e3779b97f4a7c15ull) >> 54);
+    for (int probe = 0; probe < 16; ++probe) {
+        const int e = static_cast<int>((h + static_cast<uint32_t>(probe)) & (kPairSlots - 1));
+        unsigned long long old = pairs[e];
+        if (old == ~0ull) old = atomicCAS(&pairs[e], ~0ull, key);
+        if (old == ~0ull || old == key) return;
+    }
+    uf_union(label, a, b);
+}
+
+// Joins run i of a row (runs cur, labels cur_lab) to the touching runs of
+// the row above (prev, prev_lab, pn runs): 8-connectivity, runs touching
+// [x0 - 1, x1 + 1] (reference k_merge_runs :509-557).  The arrays are LDS
+// copies or the global rows.
+template <class RunPtr, class LabPtr>
+__device__ inline void merge_run(unsigned long long* pairs, uint32_t* label, RunPtr cur, LabPtr cur_lab,
+                                 RunPtr prev, LabPtr prev_lab, int pn, int i) {
+    const uint32_t run = cur[i];
+    const int x0 = run_x0(run);
+    const int x1 = run_x1(run);
+    int lo = 0, hi = pn - 1, first = pn;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        if (run_x1(prev[mid]) >= x0 - 1) {
+            first = mid;
+            hi = mid - 1;
+        } else {
+            lo = mid + 1;
+        }
+    }
+    const uint32_t a = cur_lab[i];
+    uint32_t last_b = 0xffffffffu;
+    for (int k = first; k < pn; ++k) {
+        if (run_x0(prev[k]) > x1 + 1) break;
+        const uint32_t b = prev_lab[k];
+        if (b == a || b == last_b) continue;
+        last_b = b;
+        merge_pair(pairs, label, a, b);
+    }
+}
 
 // One workgroup per band.  Its first row is joined to the last row of the
 // band above (8-connectivity: runs touching [x0-1, x1+1]; reference
 // k_merge_runs :509-557); a band that was not joined in LDS also joins all
-// of its own rows here.  The (band root, band root) pairs of all touching
-// runs are first de-duplicated in an LDS hash set, so a large component
-// whose runs cross the boundary hundreds of times costs one union per
-// distinct pair instead of one per run.  Runs of one row never need
-// same-row joins: they are maximal, or in row 0 they are fragments the
-// reference leaves unjoined.
+// of its own rows here.  For the boundary row both rows' runs and labels
+// are staged in LDS by coalesced loads and the row's runs are spread over
+// the whole workgroup.  The (label, label) pairs of all touching runs are
+// first de-duplicated in an LDS hash set, so a large component whose runs
+// cross the boundary hundreds of times costs one union per distinct pair
+// instead of one per run.  Labels read at any time are ancestors of the
+// runs' roots, so the pairs join the same components whenever they are
+// read.  Runs of one row never need same-row joins: they are maximal, or in
+// row 0 they are fragments the reference leaves unjoined.
 __global__ __launch_bounds__(256) void k_merge(Plan plan) {
     int ty;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
@@ -791,56 +843,53 @@ __global__ __launch_bounds__(256) void k_merge(Plan plan) {
     const int t = threadIdx.x;
     const int wave = t / kWave, lane = t % kWave;
     __shared__ unsigned long long pairs[kPairSlots];
+    __shared__ uint32_t s_cur[kMergeStage], s_cur_lab[kMergeStage], s_prev[kMergeStage], s_prev_lab[kMergeStage];
     for (int e = t; e < kPairSlots; e += blockDim.x) pairs[e] = ~0ull;
-    __syncthreads();
     const uint32_t row_cap = static_cast<uint32_t>(d->row_cap);
     uint32_t* label = d->label;
-    for (int r = first_row + wave; r < last_row; r += static_cast<int>(blockDim.x) / kWave) {
+    if (first_row == 0) {
+        // The band's first row against the band above, whole workgroup.
+        const int y = y0;
+        const int n = static_cast<int>(d->row_count[y]);
+        const int pn = static_cast<int>(d->row_count[y - 1]);
+        const uint32_t cur_base = static_cast<uint32_t>(y) * row_cap;
+        const uint32_t prev_base = static_cast<uint32_t>(y - 1) * row_cap;
+        if (n > 0 && pn > 0) {
+            if (n <= kMergeStage && pn <= kMergeStage) {
+                for (int i = t; i < n; i += blockDim.x) {
+                    s_cur[i] = d->runs[cur_base + i];
+                    s_cur_lab[i] = label[cur_base + i];
+                }
+                for (int i = t; i < pn; i += blockDim.x) {
+                    s_prev[i] = d->runs[prev_base + i];
+                    s_prev_lab[i] = label[prev_base + i];
+                }
+                __syncthreads();
+                for (int i = t; i < n; i += blockDim.x)
+                    merge_run(pairs, label, s_cur, s_cur_lab, s_prev, s_prev_lab, pn, i);
+            } else {
+                __syncthreads();
+                for (int i = t; i < n; i += blockDim.x)
+                    merge_run(pairs, label, d->runs + cur_base, label + cur_base, d->runs + prev_base,
+                              label + prev_base, pn, i);
+            }
+        } else {
+            __syncthreads();
+        }
+    } else {
+        __syncthreads();
+    }
+    // Rows of a band that was not joined in LDS, one wave per row.
+    for (int r = max(first_row, 1) + wave; r < last_row; r += static_cast<int>(blockDim.x) / kWave) {
         const int y = y0 + r;
         const int n = static_cast<int>(d->row_count[y]);
         const int pn = static_cast<int>(d->row_count[y - 1]);
         if (n == 0 || pn == 0) continue;
-        const uint32_t* cur = d->runs + static_cast<size_t>(y) * row_cap;
-        const uint32_t* prev = d->runs + static_cast<size_t>(y - 1) * row_cap;
         const uint32_t cur_base = static_cast<uint32_t>(y) * row_cap;
         const uint32_t prev_base = static_cast<uint32_t>(y - 1) * row_cap;
-        for (int i = lane; i < n; i += kWave) {
-            const uint32_t run = cur[i];
-            const int x0 = run_x0(run);
-            const int x1 = run_x1(run);
-            int lo = 0, hi = pn - 1, first = pn;
-            while (lo <= hi) {
-                const int mid = (lo + hi) >> 1;
-                if (run_x1(prev[mid]) >= x0 - 1) {
-                    first = mid;
-                    hi = mid - 1;
-                } else {
-                    lo = mid + 1;
-                }
-            }
-            const uint32_t a = label[cur_base + i];
-            uint32_t last_b = 0xffffffffu;
-            for (int k = first; k < pn; ++k) {
-                if (run_x0(prev[k]) > x1 + 1) break;
-                const uint32_t b = label[prev_base + k];
-                if (b == a || b == last_b) continue;
-                last_b = b;
-                const unsigned long long key =
-                    (static_cast<unsigned long long>(min(a, b)) << 32) | max(a, b);
-                bool stored = false;
-                const uint32_t h = static_cast<uint32_t>((key * 0x9e3779b97f4a7c15ull) >> 53);
-                for (int probe = 0; probe < 16; ++probe) {
-                    const int e = static_cast<int>((h + static_cast<uint32_t>(probe)) & (kPairSlots - 1));
-                    unsigned long long old = pairs[e];
-                    if (old == ~0ull) old = atomicCAS(&pairs[e], ~0ull, key);
-                    if (old == ~0ull || old == key) {
-                        stored = true;
-                        break;
-                    }
-                }
-                if (!stored) uf_union(label, a, b);
-            }
-        }
+        for (int i = lane; i < n; i += kWave)
+            merge_run(pairs, label, d->runs + cur_base, label + cur_base, d->runs + prev_base, label + prev_base,
+                      pn, i);
     }
     __syncthreads();
     for (int e = t; e < kPairSlots; e += blockDim.x) {
@@ -850,56 +899,48 @@ __global__ __launch_bounds__(256) void k_merge(Plan plan) {
     }
 }
 
-// Band roots (flagged runs) walk to their component root; only band roots
-// carry global links, so this touches a small fraction of the runs.  Paths
-// are split with atomicMin, which can only lower a label towards the root,
-// so a concurrent walker never undoes another's final write.
-__global__ __launch_bounds__(256) void k_flatten(Plan plan) {
-    int blk;
-    const FrameDesc* d = plan.frames + frame_of(plan, SH_ROW4, blockIdx.x, &blk);
-    if (!d->active) return;
-    const int y = blk * 4 + static_cast<int>(threadIdx.x) / kWave;
-    const int lane = threadIdx.x % kWave;
-    if (y >= d->height) return;
-    const int n = static_cast<int>(d->row_count[y]);
-    const uint32_t base = static_cast<uint32_t>(y) * static_cast<uint32_t>(d->row_cap);
-    uint32_t* label = d->label;
-    for (int i = lane; i < n; i += kWave) {
-        const uint32_t slot = base + static_cast<uint32_t>(i);
-        if (!(d->runs[slot] & kRootFlag)) continue;
-        uint32_t x = slot;
-        uint32_t p = label[x];
-        while (p != x) {
-            const uint32_t gp = label[p];
-            if (gp != p) atomicMin(&label[x], gp);
-            x = p;
-            p = gp;
-        }
-        atomicMin(&label[slot], x);
-    }
-}
-
 constexpr int kStatSlots = 512;
 constexpr uint32_t kEmpty = 0xffffffffu;
 
-// One workgroup per band of 8 rows.  Every run takes its root in two hops
-// (run -> band root -> root; a band root's label is already final, so the
-// write-back races with no reader) and adds its length and extent into an
-// LDS table keyed by root; the table is then flushed with one set of global
-// atomics per (band, root), so a component with thousands of runs does not
-// serialise on its root's counters (reference k_region_stats :562-586 does
-// one atomic per run).
+// Walks x to its component root, splitting the path on the way: each
+// visited label is lowered to its grandparent with atomicMin, which can only
+// move it towards the root, so concurrent walkers never undo each other.
+__device__ inline uint32_t walk_to_root(uint32_t* label, uint32_t x) {
+    uint32_t p = label[x];
+    while (p != x) {
+        const uint32_t gp = label[p];
+        if (gp != p) atomicMin(&label[x], gp);
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+
+// One workgroup per band of 8 rows, after all joins.
+//  1. Band roots (flagged runs: the LDS-joined band's roots, or every run of
+//     a band joined in global memory) walk to their component root, which
+//     is written to their label.
+//  2. Every run takes its root (its own label if it is a band root, else its
+//     band root's label) and writes it to its label; the run's length and
+//     extent go into an LDS
+//     table keyed by root, flushed with one set of global atomics per (band,
+//     root), so a component with thousands of runs does not serialise on its
+//     root's counters (reference k_region_stats :562-586 does one atomic per
+//     run).  The root is the component's first run in raster order, so its
+//     row is the component's top row.
 __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     int ty;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
     if (!d->active) return;
-    if (!d->row_roi[4 * ty]) return;
+    const int32_t* roi = d->row_roi + 4 * ty;
+    if (!roi[0]) return;
     const int H = d->height;
     const int y0 = ty * kTile;
     const int nrows = min(kTile, H - y0);
+    const uint32_t row_cap = static_cast<uint32_t>(d->row_cap);
     const int t = threadIdx.x;
-    const int wave = t / kWave, lane = t % kWave;
     __shared__ uint32_t hkey[kStatSlots], harea[kStatSlots], hx0[kStatSlots], hx1[kStatSlots], hy1[kStatSlots];
+    __shared__ uint32_t s_off[kTile + 1];
     for (int e = t; e < kStatSlots; e += blockDim.x) {
         hkey[e] = kEmpty;
         harea[e] = 0;
@@ -907,19 +948,66 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         hx1[e] = 0;
         hy1[e] = 0;
     }
+    if (t == 0) {
+        uint32_t acc = 0;
+        for (int r = 0; r <= kTile; ++r) {
+            s_off[r] = acc;
+            if (r < nrows) acc += d->row_count[y0 + r];
+        }
+    }
     __syncthreads();
+    uint32_t so[kTile + 1];
+#pragma unroll
+    for (int q = 0; q <= kTile; ++q) so[q] = s_off[q];
+    const uint32_t nb = so[kTile];
+    const auto row_of = [so](uint32_t i) {
+        int r = 0;
+#pragma unroll
+        for (int q = 1; q < kTile; ++q) r += i >= so[q] ? 1 : 0;
+        return r;
+    };
+    const uint32_t band_base = static_cast<uint32_t>(y0) * row_cap;
     uint32_t* label = d->label;
-    for (int r = wave; r < nrows; r += static_cast<int>(blockDim.x) / kWave) {
-        const int y = y0 + r;
-        const int n = static_cast<int>(d->row_count[y]);
-        const uint32_t base = static_cast<uint32_t>(y) * static_cast<uint32_t>(d->row_cap);
-        for (int i = lane; i < n; i += kWave) {
-            const uint32_t slot = base + static_cast<uint32_t>(i);
-            const uint32_t root = label[label[slot]];
-            label[slot] = root;
-            if (root == slot) d->bbox[4 * static_cast<size_t>(root) + 1] = static_cast<uint32_t>(y);
-            const uint32_t run = d->runs[slot];
-            const uint32_t x0 = static_cast<uint32_t>(run_x0(run)), x1 = static_cast<uint32_t>(run_x1(run));
+
+    for (uint32_t i = t; i < nb; i += blockDim.x) {
+        const int r = row_of(i);
+        const uint32_t slot = band_base + static_cast<uint32_t>(r) * row_cap + (i - s_off[r]);
+        if (!(d->runs[slot] & kRootFlag)) continue;
+        atomicMin(&label[slot], walk_to_root(label, slot));
+    }
+    __syncthreads();
+
+    for (uint32_t i0 = t; i0 < nb; i0 += 4 * blockDim.x) {
+        constexpr int kItems = 4;
+        uint32_t slot[kItems], run[kItems], lab[kItems];
+        int row[kItems];
+#pragma unroll
+        for (int u = 0; u < kItems; ++u) {
+            const uint32_t i = i0 + u * blockDim.x;
+            row[u] = i < nb ? row_of(i) : -1;
+            slot[u] = i < nb ? band_base + static_cast<uint32_t>(row[u]) * row_cap + (i - s_off[row[u]]) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kItems; ++u) {
+            run[u] = row[u] >= 0 ? d->runs[slot[u]] : 0u;
+            lab[u] = row[u] >= 0 ? label[slot[u]] : 0u;
+        }
+        // A band root's label is now its root; other runs point at their
+        // band root.
+#pragma unroll
+        for (int u = 0; u < kItems; ++u) {
+            if (row[u] >= 0 && !(run[u] & kRootFlag)) lab[u] = label[lab[u]];
+        }
+        // One call per item with scalar arguments (a loop over the item
+        // arrays with the probe loop inside would not unroll and would put
+        // the arrays in scratch memory).
+        const auto account = [&](int rowu, uint32_t slotu, uint32_t runu, uint32_t labu) {
+            if (rowu < 0) return;
+            const uint32_t root = labu;
+            const int y = y0 + rowu;
+            label[slotu] = root;
+            if (root == slotu) d->bbox[4 * static_cast<size_t>(root) + 1] = static_cast<uint32_t>(y);
+            const uint32_t x0 = static_cast<uint32_t>(run_x0(runu)), x1 = static_cast<uint32_t>(run_x1(runu));
             int found = -1;
             const uint32_t h = (root * 2654435761u) >> 23;
             for (int probe = 0; probe < 32; ++probe) {
@@ -942,7 +1030,11 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
                 atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 2], x1);
                 atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 3], static_cast<uint32_t>(y));
             }
-        }
+        };
+        account(row[0], slot[0], run[0], lab[0]);
+        account(row[1], slot[1], run[1], lab[1]);
+        account(row[2], slot[2], run[2], lab[2]);
+        account(row[3], slot[3], run[3], lab[3]);
     }
     __syncthreads();
     for (int e = t; e < kStatSlots; e += blockDim.x) {
@@ -2233,13 +2325,12 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
         case 1: hipLaunchKernelGGL(k_frame_ref, dim3(nf), dim3(256), 0, stream, plan); break;
         case 2: hipLaunchKernelGGL(k_binarize_runs, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
         case 3: hipLaunchKernelGGL(k_merge, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
-        case 4: hipLaunchKernelGGL(k_flatten, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
-        case 5: hipLaunchKernelGGL(k_stats, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
-        case 6: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
-        case 7: hipLaunchKernelGGL(k_select, dim3(nf), dim3(1024), 0, stream, plan); break;
-        case 8: hipLaunchKernelGGL(k_fit, dim3(min(plan.work_cap, kFitGrid)), dim3(256), 0, stream, plan); break;
-        case 9: hipLaunchKernelGGL(k_decode, dim3(min(plan.work_cap, kDecodeGrid)), dim3(512), 0, stream, plan); break;
-        case 10: hipLaunchKernelGGL(k_finalize, dim3(nf), dim3(256), 0, stream, plan); break;
+        case 4: hipLaunchKernelGGL(k_stats, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
+        case 5: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
+        case 6: hipLaunchKernelGGL(k_select, dim3(nf), dim3(1024), 0, stream, plan); break;
+        case 7: hipLaunchKernelGGL(k_fit, dim3(min(plan.work_cap, kFitGrid)), dim3(256), 0, stream, plan); break;
+        case 8: hipLaunchKernelGGL(k_decode, dim3(min(plan.work_cap, kDecodeGrid)), dim3(512), 0, stream, plan); break;
+        case 9: hipLaunchKernelGGL(k_finalize, dim3(nf), dim3(256), 0, stream, plan); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
