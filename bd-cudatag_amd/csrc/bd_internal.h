@@ -24,6 +24,7 @@ constexpr int kFitIterations = 2;
 constexpr int kFitProbeHalf = 24;
 constexpr int kMaxProbes = 2 * kFitProbeHalf + 1;
 constexpr int kMaxSamples = 16;
+constexpr uint32_t kFitWindow = 8;   // runs checked at each end of a row range first
 
 // Device frames are addressed by 32-bit byte offsets from their base in the
 // sampling kernels; a frame must span less than 4 GiB.
@@ -78,6 +79,8 @@ struct FrameDesc {
     uint32_t* scal;       // S_COUNT scalars
     uint64_t* bits;       // height * words64
     uint32_t* row_count;  // height
+    uint32_t* run_dir;    // height * (2 * words64 + 1): per 32-px half word h,
+                          // runs ending before 32h | runs starting before 32h << 16
     uint32_t* runs;       // height * row_cap, x0 | x1 << 16
     uint32_t* label;      // height * row_cap
     uint32_t* area;       // height * row_cap

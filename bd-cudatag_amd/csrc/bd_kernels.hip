@@ -515,6 +515,8 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
             d->bits[static_cast<size_t>(y0 + r) * w64 + (i - r * w64)] = ~0ull;
         }
         if (t < nrows) d->row_count[y0 + t] = 0;
+        const int ds = 2 * w64 + 1;
+        for (int i = t; i < nrows * ds; i += blockDim.x) d->run_dir[static_cast<size_t>(y0) * ds + i] = 0;
         return;
     }
     __syncthreads();
@@ -639,6 +641,7 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
             uint32_t tot_s, tot_e;
             uint32_t ks = carry_s + wave_excl_scan(__popc(starts), lane, &tot_s);
             uint32_t ke = carry_e + wave_excl_scan(__popc(ends), lane, &tot_e);
+            if (hw < nh) d->run_dir[static_cast<size_t>(y) * (nh + 1) + hw] = ke | (ks << 16);
             const int xb = hw * 32;
             while (starts | ends) {
                 if (starts) {
@@ -670,6 +673,7 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
             carry_s += tot_s;
             carry_e += tot_e;
         }
+        if (lane == 0) d->run_dir[static_cast<size_t>(y) * (nh + 1) + nh] = carry_e | (carry_s << 16);
     }
     if (t == 0) roi[3] = local ? 1 : 0;
     if (!local) return;
@@ -1405,7 +1409,7 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
     // Rows padded to a multiple of four floats for 16-byte LDS reads.
     constexpr int kProbeStride = (kMaxProbes + 3) & ~3;
     __shared__ __align__(16) uint32_t work_lds[4 * kMaxSamples * kProbeStride];
-    static_assert(4 * kMaxSamples * kProbeStride >= 2 * kMaxBoundaryRows + 1, "row-extreme work arrays");
+    static_assert(4 * kMaxSamples * kProbeStride >= 3 * kMaxBoundaryRows + 1, "row-extreme work arrays");
     float (*mags)[kMaxSamples][kProbeStride] = reinterpret_cast<float (*)[kMaxSamples][kProbeStride]>(work_lds);
     __shared__ uint32_t wave_total[4];
     __shared__ float fit_quad[4][2];
@@ -1423,123 +1427,144 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
     const int kept = min(rows, kMaxBoundaryRows);
     const int points = 2 * kept;
 
-    // Row extremes in three parallel steps: (1) per sampled row, the index
-    // range of runs overlapping [bx0, bx1] by two interleaved binary
-    // searches; (2) an exclusive scan of the range lengths; (3) every
-    // (row, run) pair of the flattened ranges checked by its own thread, the
-    // component's runs folded into the row's left/right by LDS atomics.
-    // Runs are x-ordered and the component lies inside its bbox, so this
-    // finds the same extremes as a scan of the whole row.
-    uint32_t* row_slot = work_lds;                      // first run slot of the range
-    uint32_t* row_off = work_lds + kMaxBoundaryRows;    // range offsets, [kept] = total
-    for (int s = t; s < kept; s += blockDim.x) {
-        const int y = fit_row(s, kept, rows, by0);
-        const int n = static_cast<int>(d->row_count[y]);
-        const uint32_t* rr = d->runs + static_cast<size_t>(y) * row_cap;
-        int a_lo = 0, a_hi = n, b_lo = 0, b_hi = n;
-        while (a_lo < a_hi || b_lo < b_hi) {
-            const bool sa = a_lo < a_hi, sb = b_lo < b_hi;
-            const int am = (a_lo + a_hi) >> 1, bm = (b_lo + b_hi) >> 1;
-            const uint32_t ra = sa ? rr[am] : 0u;
-            const uint32_t rb = sb ? rr[bm] : 0u;
-            if (sa) {
-                if (run_x1(ra) >= bx0) a_hi = am; else a_lo = am + 1;
-            }
-            if (sb) {
-                if (run_x0(rb) > bx1) b_hi = bm; else b_lo = bm + 1;
-            }
+    // Row extremes: (1) per sampled row, a range of runs covering [bx0, bx1]
+    // from the run directory; (2) the component's first and last run in
+    // each range, found by progressive end windows.  Runs are x-ordered and
+    // the component lies inside its bbox, so these give the same extremes
+    // as a scan of the whole row; every row of the bbox holds a run of the
+    // (connected) component.  The component's matches fold into the row's
+    // left/right by LDS atomics.
+    uint32_t* row_slot = work_lds;                        // first run slot of the range
+    uint32_t* row_list = work_lds + kMaxBoundaryRows;     // rows of the current pass
+    uint32_t* row_len = work_lds + 2 * kMaxBoundaryRows + 1;  // runs in the range | window hit bits
+    constexpr uint32_t kLeftHit = 1u << 30, kRightHit = 1u << 31, kLenMask = kLeftHit - 1u;
+    {
+        // Range of each sampled row from the run directory: runs ending
+        // before bx0's half word to runs starting before the half word after
+        // bx1's.  The extra runs at either end lie outside the bbox and never
+        // carry the component's label.
+        const uint32_t dstride = static_cast<uint32_t>(2 * d->words64 + 1);
+        const uint32_t h0 = static_cast<uint32_t>(bx0) >> 5, h1 = (static_cast<uint32_t>(bx1) >> 5) + 1u;
+        for (int s = t; s < kept; s += blockDim.x) {
+            const int y = fit_row(s, kept, rows, by0);
+            const uint32_t* dir = d->run_dir + static_cast<size_t>(y) * dstride;
+            const uint32_t a_lo = dir[h0] & 0xffffu, b_lo = dir[h1] >> 16;
+            row_slot[s] = static_cast<uint32_t>(y) * row_cap + a_lo;
+            row_len[s] = b_lo - a_lo;
+            boundary[2 * s] = 0xffffffffu;  // min x0
+            boundary[2 * s + 1] = 0u;       // max x1 + 1
         }
-        row_slot[s] = static_cast<uint32_t>(y) * row_cap + static_cast<uint32_t>(a_lo);
-        row_off[s] = static_cast<uint32_t>(max(b_lo - a_lo, 0));
-        boundary[2 * s] = 0xffffffffu;  // min x0
-        boundary[2 * s + 1] = 0u;       // max x1 + 1
     }
     __syncthreads();
-    {
-        uint32_t c[4], sum = 0;
+    // Progressive end windows: pass p checks the next `width` runs at each
+    // end of every row range not yet resolved (depth runs already checked
+    // at each end; width = kFitWindow << p).  A row is resolved when both
+    // ends have found the component or its whole range has been checked.
+    // Rows of a pass are compacted into row_list and padded to 2 * width
+    // items, so an item's row is a shift away.
+    uint32_t depth = 0, width = kFitWindow;
+    for (int pass = 0;; ++pass) {
+        {
+            uint32_t need = 0, cnt = 0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int s = 4 * t + j;
-            c[j] = s < kept ? row_off[s] : 0u;
-            sum += c[j];
+            for (int jj = 0; jj < 4; ++jj) {
+                const int s = 4 * t + jj;
+                if (s >= kept) continue;
+                const uint32_t len = row_len[s], n = len & kLenMask;
+                const bool open = pass == 0 || (len & (kLeftHit | kRightHit)) != (kLeftHit | kRightHit);
+                if (n > 2u * depth && open) {
+                    need |= 1u << jj;
+                    ++cnt;
+                }
+            }
+            uint32_t wtot;
+            uint32_t ex = wave_excl_scan(cnt, lane, &wtot);
+            __syncthreads();  // row_list of the previous pass is no longer read
+            if (lane == 0) wave_total[wave] = wtot;
+            __syncthreads();
+            for (int w = 0; w < wave; ++w) ex += wave_total[w];
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                if (need & (1u << jj)) row_list[ex++] = static_cast<uint32_t>(4 * t + jj);
+            }
         }
-        uint32_t wtot;
-        uint32_t ex = wave_excl_scan(sum, lane, &wtot);
-        if (lane == 0) wave_total[wave] = wtot;
+        const uint32_t nlist = wave_total[0] + wave_total[1] + wave_total[2] + wave_total[3];
         __syncthreads();
-        for (int w = 0; w < wave; ++w) ex += wave_total[w];
+        if (nlist == 0) break;
+        const uint32_t per_row = 2u * width;
+        const int shift = 31 - __clz(static_cast<int>(per_row));
+        const uint32_t items = nlist << shift;
+        for (uint32_t j0 = static_cast<uint32_t>(t); j0 < items; j0 += 8u * blockDim.x) {
+            constexpr int kItems = 8;
+            uint32_t slot[kItems], ends[kItems];
+            int sr[kItems];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int s = 4 * t + j;
-            if (s < kept) row_off[s] = ex;
-            ex += c[j];
-        }
-        if (t == 0) row_off[kept] = wave_total[0] + wave_total[1] + wave_total[2] + wave_total[3];
-    }
-    __syncthreads();
-    {
-        // Item j goes to thread j % 256 (coalesced run and label loads); a
-        // thread's row only moves forward, from a binary search for its
-        // first item.
-        const uint32_t total = row_off[kept];
-        uint32_t j = static_cast<uint32_t>(t);
-        if (j < total) {
-            int lo = 0, hi = kept - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (row_off[mid] <= j) lo = mid; else hi = mid - 1;
-            }
-            int srow = lo;
-            uint32_t next_off = row_off[srow + 1];
-            const uint32_t step = blockDim.x;
-            while (j < total) {
-                constexpr int kItems = 8;
-                uint32_t slot[kItems];
-                int sr[kItems];
-#pragma unroll
-                for (int u = 0; u < kItems; ++u) {
-                    sr[u] = -1;
-                    slot[u] = 0;
-                    if (j < total) {
-                        while (j >= next_off) next_off = row_off[++srow + 1];
-                        sr[u] = srow;
-                        slot[u] = row_slot[srow] + (j - row_off[srow]);
-                        j += step;
+            for (int u = 0; u < kItems; ++u) {
+                sr[u] = -1;
+                slot[u] = 0;
+                ends[u] = 0;
+                const uint32_t j = j0 + u * blockDim.x;
+                if (j < items) {
+                    const uint32_t srow = row_list[j >> shift];
+                    const uint32_t q = j & (per_row - 1u);
+                    const uint32_t n = row_len[srow] & kLenMask;
+                    const uint32_t rem = n - 2u * depth;
+                    bool valid = true;
+                    uint32_t k;
+                    if (rem <= per_row) {
+                        valid = q < rem;  // the rest of the range, checked whole
+                        k = depth + q;
+                    } else if (q < width) {
+                        k = depth + q;
+                        ends[u] = kLeftHit;
+                    } else {
+                        k = n - depth - per_row + q;
+                        ends[u] = kRightHit;
+                    }
+                    if (valid) {
+                        sr[u] = static_cast<int>(srow);
+                        slot[u] = row_slot[srow] + k;
                     }
                 }
-                uint32_t lab[kItems], run[kItems];
+            }
+            uint32_t lab[kItems], run[kItems];
 #pragma unroll
-                for (int u = 0; u < kItems; ++u) {
-                    lab[u] = sr[u] >= 0 ? d->label[slot[u]] : ~root;
-                    run[u] = sr[u] >= 0 ? d->runs[slot[u]] : 0u;
-                }
-                const uint64_t below = (1ull << lane) - 1ull;
+            for (int u = 0; u < kItems; ++u) {
+                lab[u] = sr[u] >= 0 ? d->label[slot[u]] : ~root;
+                run[u] = sr[u] >= 0 ? d->runs[slot[u]] : 0u;
+            }
+            const uint64_t below = (1ull << lane) - 1ull;
 #pragma unroll
-                for (int u = 0; u < kItems; ++u) {
-                    // Consecutive lanes hold consecutive items, so a row's
-                    // items in this wave are a contiguous lane segment: only
-                    // the segment's first match (left) and last match (right)
-                    // update LDS.  Segment starts: lanes whose row differs
-                    // from the lower neighbour's.
-                    const bool hit = lab[u] == root;
-                    const uint64_t m = __ballot(hit);
-                    const uint32_t lower_row = dpp_from_lower(0xfffffffeu, static_cast<uint32_t>(sr[u]));
-                    const uint64_t starts = __ballot(static_cast<uint32_t>(sr[u]) != lower_row) | 1ull;
-                    if (!hit) continue;
-                    const uint64_t seg_lo = starts & (below | (1ull << lane));   // my segment start and below
-                    const int s0 = 63 - __clzll(static_cast<long long>(seg_lo));
-                    const uint64_t above = starts & ~(below | (1ull << lane));
-                    const int s1 = above ? __ffsll(static_cast<long long>(above)) - 1 : 64;
-                    const uint64_t seg = (s1 >= 64 ? ~0ull : ((1ull << s1) - 1ull)) & ~((1ull << s0) - 1ull);
-                    const bool first = (m & seg & below) == 0;
-                    const bool last = (m & seg & ~(below | (1ull << lane))) == 0;
-                    if (first) atomicMin(&boundary[2 * sr[u]], static_cast<uint32_t>(run_x0(run[u])));
-                    if (last) atomicMax(&boundary[2 * sr[u] + 1], static_cast<uint32_t>(run_x1(run[u]) + 1));
-                }
+            for (int u = 0; u < kItems; ++u) {
+                // Consecutive lanes hold consecutive items of a row (in run
+                // order), so a row's items in this wave are a contiguous
+                // lane segment: only the segment's first match (left) and
+                // last match (right) update LDS.  Segment starts: lanes whose
+                // row differs from the lower neighbour's.
+                const bool hit = lab[u] == root;
+                const uint64_t m = __ballot(hit);
+                const uint32_t lower_row = dpp_from_lower(0xfffffffeu, static_cast<uint32_t>(sr[u]));
+                const uint64_t starts = __ballot(static_cast<uint32_t>(sr[u]) != lower_row) | 1ull;
+                if (!hit) continue;
+                const uint64_t seg_lo = starts & (below | (1ull << lane));   // my segment start and below
+                const int s0 = 63 - __clzll(static_cast<long long>(seg_lo));
+                const uint64_t above = starts & ~(below | (1ull << lane));
+                const int s1 = above ? __ffsll(static_cast<long long>(above)) - 1 : 64;
+                const uint64_t seg = (s1 >= 64 ? ~0ull : ((1ull << s1) - 1ull)) & ~((1ull << s0) - 1ull);
+                const bool first = (m & seg & below) == 0;
+                const bool last = (m & seg & ~(below | (1ull << lane))) == 0;
+                if (first) atomicMin(&boundary[2 * sr[u]], static_cast<uint32_t>(run_x0(run[u])));
+                if (last) atomicMax(&boundary[2 * sr[u] + 1], static_cast<uint32_t>(run_x1(run[u]) + 1));
+                // A hit in the left window makes the segment's first hit a
+                // left-window item (items are in run order), and likewise
+                // for the right window and the last hit.
+                if (ends[u] && (first || last)) atomicOr(&row_len[sr[u]], ends[u]);
             }
         }
+        __syncthreads();
+        depth += width;
+        width *= 2u;
     }
-    __syncthreads();
     for (int s = t; s < kept; s += blockDim.x) {
         const uint32_t y = static_cast<uint32_t>(fit_row(s, kept, rows, by0)) << 16;
         const uint32_t l = boundary[2 * s], r = boundary[2 * s + 1];
