@@ -88,6 +88,11 @@ class ServeReport(ctypes.Structure):
 
 
 # Every symbol the public header declares, with (restype, argtypes).
+class Tune(ctypes.Structure):
+    _fields_ = [("fit_grid", ctypes.c_int32), ("decode_grid", ctypes.c_int32),
+                ("head_ranks", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
 SIGNATURES = {
@@ -118,6 +123,9 @@ SIGNATURES = {
     "bd_detector_batch": (_P, [_P]),
     "bd_batch_time_stages": (_I, [_P, _I, ctypes.POINTER(ctypes.c_float)]),
     "bd_batch_replay": (_I, [_P, _I, _I, ctypes.POINTER(ctypes.c_float)]),
+    "bd_batch_get_tune": (_I, [_P, ctypes.POINTER(Tune)]),
+    "bd_batch_set_tune": (_I, [_P, ctypes.POINTER(Tune)]),
+    "bd_batch_autotune": (_I, [_P, _I, ctypes.POINTER(Tune), ctypes.POINTER(ctypes.c_float)]),
     "bd_server_create": (_P, [_I, _I, ctypes.POINTER(ServerClass), ctypes.POINTER(Config),
                               ctypes.POINTER(_I)]),
     "bd_server_destroy": (None, [_P]),
@@ -201,6 +209,8 @@ class Batch:
                 "max_detections": info[7]})
             i += 1
         self.max_detections = self.frames[0]["max_detections"]
+        self.family = family
+        self.sizes = [(f["width"], f["height"]) for f in self.frames]
 
     def close(self):
         if getattr(self, "_owned", False) and self.handle:
@@ -296,6 +306,53 @@ class Batch:
         if rc != BD_OK:
             raise RuntimeError(f"bd_batch_replay failed: {last_error()}")
         return float(ms.value)
+
+    def get_tune(self) -> dict:
+        t = Tune()
+        if lib().bd_batch_get_tune(self.handle, ctypes.byref(t)) != BD_OK:
+            raise RuntimeError(f"bd_batch_get_tune failed: {last_error()}")
+        return {"fit_grid": t.fit_grid, "decode_grid": t.decode_grid, "head_ranks": t.head_ranks}
+
+    def set_tune(self, fit_grid: int, decode_grid: int, head_ranks: int) -> None:
+        t = Tune(fit_grid, decode_grid, head_ranks, 0)
+        rc = lib().bd_batch_set_tune(self.handle, ctypes.byref(t))
+        if rc != BD_OK:
+            raise (ValueError if rc == BD_ERROR_INVALID_ARGUMENT else RuntimeError)(
+                f"bd_batch_set_tune failed: {last_error()}")
+
+    def signature(self) -> str:
+        """Key of this batch's shape for a tuning cache."""
+        return f"family{self.family}|" + ",".join(
+            f"{f['width']}x{f['height']}c{f['max_candidates']}" for f in self.frames)
+
+    def autotune(self, reps: int = 10, cache: str | None = None) -> dict:
+        """Picks the fastest launch parameters for the frames of the last
+        detect call (bd_batch_autotune).  With `cache` (a JSON file), a
+        setting stored for this batch shape is applied instead of measuring,
+        and a new measurement is stored."""
+        import json
+        key = self.signature()
+        table = {}
+        if cache and os.path.exists(cache):
+            with open(cache) as f:
+                table = json.load(f)
+            if key in table:
+                t = table[key]
+                self.set_tune(t["fit_grid"], t["decode_grid"], t["head_ranks"])
+                return dict(t, cached=True)
+        best = Tune()
+        ms = ctypes.c_float()
+        rc = lib().bd_batch_autotune(self.handle, reps, ctypes.byref(best), ctypes.byref(ms))
+        if rc != BD_OK:
+            raise RuntimeError(f"bd_batch_autotune failed: {last_error()}")
+        out = {"fit_grid": best.fit_grid, "decode_grid": best.decode_grid, "head_ranks": best.head_ranks,
+               "pipeline_ms": round(float(ms.value), 4)}
+        if cache:
+            table[key] = out
+            os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+            with open(cache, "w") as f:
+                json.dump(table, f, indent=1, sort_keys=True)
+        return dict(out, cached=False)
 
 
 class Detector:

@@ -369,6 +369,50 @@ int32_t bd_batch_replay(bd_batch* batch, int32_t reps, int32_t device_path, floa
     return BD_OK;
 }
 
+int32_t bd_batch_get_tune(bd_batch* batch, bd_tune* out) {
+    if (!batch || !batch->engine || !out) {
+        set_error("null batch or output");
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    const bd::Tune t = batch->engine->tune();
+    *out = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, 0};
+    return BD_OK;
+}
+
+int32_t bd_batch_set_tune(bd_batch* batch, const bd_tune* tune) {
+    if (!batch || !batch->engine || !tune) {
+        set_error("null batch or parameters");
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    bd::Tune t;
+    t.fit_grid = tune->fit_grid;
+    t.decode_grid = tune->decode_grid;
+    t.head_ranks = tune->head_ranks;
+    std::string error;
+    if (!batch->engine->set_tune(t, &error)) {
+        set_error(error);
+        return error == "launch parameters out of range" ? BD_ERROR_INVALID_ARGUMENT : BD_ERROR_CUDA;
+    }
+    return BD_OK;
+}
+
+int32_t bd_batch_autotune(bd_batch* batch, int32_t reps, bd_tune* best, float* best_ms) {
+    if (!batch || !batch->engine || reps < 1) {
+        set_error("null batch or invalid repetition count");
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    bd::Tune t;
+    float ms = 0.0f;
+    std::string error;
+    if (!batch->engine->autotune(reps, &t, &ms, &error)) {
+        set_error(error);
+        return BD_ERROR_CUDA;
+    }
+    if (best) *best = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, 0};
+    if (best_ms) *best_ms = ms;
+    return BD_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

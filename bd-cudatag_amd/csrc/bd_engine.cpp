@@ -141,7 +141,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     }
     size_t work_cap = 0;
     for (int f = 0; f < nf; ++f) work_cap += static_cast<size_t>(maxc[f]);
-    const size_t work_words = kWorkItems + static_cast<size_t>(kHeadRanks) * nf + work_cap;
+    const size_t work_words = kWorkItems + static_cast<size_t>(kHeadRanksMax) * nf + work_cap;
     const size_t o_work = take(work_words * 4);
     const size_t o_scal = take(static_cast<size_t>(nf) * S_COUNT * 4);
     const size_t o_out = take(static_cast<size_t>(nf) * max_det_ * sizeof(bd_detection));
@@ -335,6 +335,7 @@ int Engine::detect(const bd_frame* frames, bool device, bd_detection* out, int o
     const std::lock_guard<std::mutex> lock(mutex_);
     const int nf = frame_count();
     const auto start = std::chrono::steady_clock::now();
+    last_device_ = device;
     if (device) {
         for (int f = 0; f < nf; ++f) {
             h_desc_dev_[f].gray = frames[f].gray;
@@ -559,6 +560,10 @@ bool Engine::time_stages(int reps, float* ms, std::string* error) {
 
 bool Engine::replay(int reps, bool device_path, float* ms, std::string* error) {
     const std::lock_guard<std::mutex> lock(mutex_);
+    return replay_locked(reps, device_path, ms, error);
+}
+
+bool Engine::replay_locked(int reps, bool device_path, float* ms, std::string* error) {
     if (reps < 1) {
         if (error) *error = "invalid repetition count";
         return false;
@@ -577,6 +582,83 @@ bool Engine::replay(int reps, bool device_path, float* ms, std::string* error) {
     float t = 0.0f;
     (void)hipEventElapsedTime(&t, ev_start_, ev_stop_);
     if (ms) *ms = t;
+    return true;
+}
+
+Tune Engine::tune() const { return plan_host_.tune; }
+
+bool Engine::set_tune(const Tune& t, std::string* error) {
+    const std::lock_guard<std::mutex> lock(mutex_);
+    return set_tune_locked(t, error);
+}
+
+bool Engine::set_tune_locked(const Tune& t, std::string* error) {
+    if (t.fit_grid < 64 || t.fit_grid > 16384 || t.decode_grid < 64 || t.decode_grid > 16384 ||
+        t.head_ranks < 0 || t.head_ranks > kHeadRanksMax) {
+        if (error) *error = "launch parameters out of range";
+        return false;
+    }
+    const hipError_t st = hipStreamSynchronize(stream_);
+    if (st != hipSuccess) {
+        if (error) *error = hip_error("hipStreamSynchronize", st);
+        return false;
+    }
+    plan_host_.tune = t;
+    plan_dev_.tune = t;
+    if (graph_host_) (void)hipGraphExecDestroy(graph_host_);
+    if (graph_dev_) (void)hipGraphExecDestroy(graph_dev_);
+    graph_host_ = graph_dev_ = nullptr;
+    if (p_.use_graph) {
+        std::string ignored;
+        if (!capture(false, &graph_host_, &ignored)) graph_host_ = nullptr;
+        if (!capture(true, &graph_dev_, &ignored)) graph_dev_ = nullptr;
+    }
+    return true;
+}
+
+bool Engine::autotune(int reps, Tune* best, float* best_ms, std::string* error) {
+    const std::lock_guard<std::mutex> lock(mutex_);
+    if (reps < 1) {
+        if (error) *error = "invalid repetition count";
+        return false;
+    }
+    const bool dev = last_device_;
+    const auto measure = [&](const Tune& t, float* ms) {
+        float warm = 0.0f;
+        return set_tune_locked(t, error) && replay_locked(2, dev, &warm, error) &&
+               replay_locked(reps, dev, ms, error);
+    };
+    Tune cur = plan_host_.tune;
+    float cur_ms = 0.0f;
+    if (!measure(cur, &cur_ms)) return false;
+    const auto consider = [&](Tune t) {
+        float ms = 0.0f;
+        if (!measure(t, &ms)) return false;
+        if (ms < cur_ms) {
+            cur = t;
+            cur_ms = ms;
+        }
+        return true;
+    };
+    static const int32_t kGrids[] = {256, 512, 1024, 2048, 4096};
+    for (const int32_t g : kGrids) {
+        Tune t = cur;
+        t.fit_grid = g;
+        if (!consider(t)) return false;
+    }
+    for (const int32_t g : kGrids) {
+        Tune t = cur;
+        t.decode_grid = g;
+        if (!consider(t)) return false;
+    }
+    for (int32_t h = 0; h <= kHeadRanksMax; ++h) {
+        Tune t = cur;
+        t.head_ranks = h;
+        if (!consider(t)) return false;
+    }
+    if (!set_tune_locked(cur, error)) return false;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms / static_cast<float>(reps);
     return true;
 }
 

@@ -68,6 +68,15 @@ public:
     bool replay(int reps, bool device_path, float* ms, std::string* error);
     hipStream_t stream() const { return stream_; }
 
+    // Launch parameters (results never depend on them).  set_tune waits for
+    // the stream and re-captures the graphs.
+    Tune tune() const;
+    bool set_tune(const Tune& t, std::string* error);
+    // Times graph replays of the last call's frames under a small set of
+    // launch parameters (one parameter at a time, keeping the best so far)
+    // and leaves the fastest set in place.  *best_ms is per pipeline run.
+    bool autotune(int reps, Tune* best, float* best_ms, std::string* error);
+
     // Asynchronous device-path launch of positions [0, n) (the rest are
     // inactive for this launch).  The caller must not launch again before
     // query_async() reports completion.
@@ -82,12 +91,15 @@ private:
     bool init(const Params& params, const std::vector<int>& widths,
               const std::vector<int>& heights, std::string* error);
     bool capture(bool device_path, hipGraphExec_t* exec, std::string* error);
+    bool set_tune_locked(const Tune& t, std::string* error);
+    bool replay_locked(int reps, bool device_path, float* ms, std::string* error);
     hipError_t enqueue(bool device_path);
     void release();
 
     Params p_;
     std::vector<int> w_, h_;
     int max_det_ = 0;
+    bool last_device_ = false;  // path of the last detect call
     std::vector<FrameDesc> desc_;      // host copy, gray -> engine input buffers
     std::vector<size_t> input_off_;    // per frame offset in the input region
     size_t input_bytes_ = 0;

@@ -100,20 +100,28 @@ enum Shape : int { SH_TILE_ROW = 0, SH_ROW4 = 1, SH_FRAME = 2, SH_TSTAT = 3, SH_
 // Batch candidate list: a small header, then one entry per selected
 // candidate of any frame (frame << 10 | candidate index), appended by
 // k_select and consumed by the persistent k_fit and k_decode grids.  The
-// first kHeadRanks candidates of every frame (its largest) sit in a head
-// region of kHeadRanks * nframes entries that is handed out first; the rest
-// follow in a tail region.
+// first Plan::head_ranks candidates of every frame (its largest) sit in a
+// head region of kHeadRanksMax * nframes entries that is handed out first;
+// the rest follow in a tail region.
 enum WorkSlot : int { kWorkCount = 0, kWorkFitCursor = 1, kWorkDecodeCursor = 2, kWorkHeadCount = 3, kWorkItems = 4 };
-constexpr int kHeadRanks = 2;
-constexpr int kFitGrid = 1024;     // 4 resident 256-thread workgroups per CU
-constexpr int kDecodeGrid = 1024;  // 4 resident 512-thread workgroups per CU
+constexpr int kHeadRanksMax = 4;
+
+// Launch parameters that change speed only, never results (autotuned per
+// batch shape): persistent grid sizes of k_fit and k_decode and the number
+// of each frame's largest candidates scheduled first.
+struct Tune {
+    int32_t fit_grid = 1024;     // 4 resident 256-thread workgroups per CU
+    int32_t decode_grid = 1024;  // 4 resident 512-thread workgroups per CU
+    int32_t head_ranks = 2;
+};
 
 struct Plan {
     const FrameDesc* frames;
     const int32_t* prefix;  // SH_COUNT * (nframes + 1)
-    uint32_t* work;         // kWorkItems + kHeadRanks * nframes + work_cap entries
+    uint32_t* work;         // kWorkItems + kHeadRanksMax * nframes + work_cap entries
     int32_t nframes;
     int32_t work_cap;       // sum of max_candidates over frames
+    Tune tune;
     int32_t blocks[SH_COUNT];
 };
 

@@ -1177,14 +1177,14 @@ __global__ __launch_bounds__(1024) void k_select(Plan plan) {
     // The frame's largest candidates go to the head of the batch list, so
     // that the longest fits of the batch start first.
     __shared__ uint32_t s_head, s_tail;
-    const uint32_t nhead = min(nsel, static_cast<uint32_t>(kHeadRanks));
+    const uint32_t nhead = min(nsel, static_cast<uint32_t>(plan.tune.head_ranks));
     if (t == 0) {
         s_head = atomicAdd(&plan.work[kWorkHeadCount], nhead);
         s_tail = atomicAdd(&plan.work[kWorkCount], nsel - nhead);
     }
     __syncthreads();
     uint32_t* head = plan.work + kWorkItems;
-    uint32_t* tail = head + kHeadRanks * plan.nframes;
+    uint32_t* tail = head + kHeadRanksMax * plan.nframes;
     for (uint32_t i = t; i < nsel; i += blockDim.x) {
         d->candidates[i] = static_cast<uint32_t>(keys[i]);
         const uint32_t code = (blockIdx.x << 10) | i;
@@ -1952,7 +1952,7 @@ __device__ inline uint32_t next_item(uint32_t* cursor) {
 // largest candidates) first, then the tail region.
 __device__ inline uint32_t work_item(const Plan& plan, uint32_t item, uint32_t nhead) {
     const uint32_t* head = plan.work + kWorkItems;
-    return item < nhead ? head[item] : head[kHeadRanks * plan.nframes + (item - nhead)];
+    return item < nhead ? head[item] : head[kHeadRanksMax * plan.nframes + (item - nhead)];
 }
 
 // Persistent workgroups over the batch's candidate list (k_select appends
@@ -2353,8 +2353,8 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
         case 4: hipLaunchKernelGGL(k_stats, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
         case 5: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
         case 6: hipLaunchKernelGGL(k_select, dim3(nf), dim3(1024), 0, stream, plan); break;
-        case 7: hipLaunchKernelGGL(k_fit, dim3(min(plan.work_cap, kFitGrid)), dim3(256), 0, stream, plan); break;
-        case 8: hipLaunchKernelGGL(k_decode, dim3(min(plan.work_cap, kDecodeGrid)), dim3(512), 0, stream, plan); break;
+        case 7: hipLaunchKernelGGL(k_fit, dim3(min(plan.work_cap, plan.tune.fit_grid)), dim3(256), 0, stream, plan); break;
+        case 8: hipLaunchKernelGGL(k_decode, dim3(min(plan.work_cap, plan.tune.decode_grid)), dim3(512), 0, stream, plan); break;
         case 9: hipLaunchKernelGGL(k_finalize, dim3(nf), dim3(256), 0, stream, plan); break;
         default: return hipErrorInvalidValue;
     }

@@ -10,15 +10,18 @@ returns every frame's detections to the host.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
                   [--single-frames N] [--cpu-seconds S] [--serve-seconds S]
-                  [--uneven-steps N]
+                  [--uneven-steps N] [--precision-frames N]
 
-Besides the headline line it reports an uneven batch (16 frames each of
-640x480, 1280x800, 1920x1200 and 4096x3072 in one launch) and a serving run
-(six Poisson camera streams, continuous batching, p50/p99/p99.9 latency).
+Besides the headline line it reports the same batch from host memory
+(H2D copy included), an uneven batch (16 frames each of 640x480, 1280x800,
+1920x1200 and 4096x3072 in one launch), a serving run (six Poisson camera
+streams, continuous batching, p50/p99/p99.9 latency) and the corner accuracy
+on 256 degraded frames against the rendered ground truth.
 
---single-frames 0 and --cpu-seconds 0 leave only batched launches, which is
-the command profiled under profiles/ so that per-kernel averages there match
-the stages_ms figures.
+--single-frames 0 --cpu-seconds 0 --uneven-steps 0 --serve-seconds 0
+--precision-frames 0 leave only launches on the headline batch, which is the
+command profiled under profiles/ so that per-kernel averages there match the
+stages_ms figures.
 
 Prints one JSON line.  Only one GPU is used; --gpus above 1 exits with a
 message.
@@ -46,12 +49,17 @@ def percentile(v, q):
 
 
 def cpu_baseline(frames, seconds: float):
-    """Scalar oracle on the host, one thread, over a bounded sample."""
+    """Scalar oracle on the host over a bounded sample: one thread (the
+    baseline), then frame-parallel threads on the host's cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import threading
+
     import pyoracle
 
     h, w = frames.shape[1:]
     o = pyoracle.Oracle(pyoracle.default_config(0, w, h))
+    for i in range(min(20, len(frames))):  # warm-up
+        o.run(frames[i % len(frames)])
     times = []
     t_end = time.perf_counter() + seconds
     i = 0
@@ -61,19 +69,44 @@ def cpu_baseline(frames, seconds: float):
         times.append(time.perf_counter() - t0)
         i += 1
     total = sum(times)
-    return {
+    out = {
         "value": round(len(times) * w * h / total / 1e6, 2),
         "unit": "Mpix/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{len(times)} frames of {w}x{h} (same seeded workload), {total:.1f} s, "
-                  f"median {percentile(times, 50) * 1e3:.2f} ms/frame",
+        "sample": f"{len(times)} frames of {w}x{h} (same seeded workload) after 20 warm-up frames, "
+                  f"{total:.1f} s; per frame median {percentile(times, 50) * 1e3:.2f} ms, "
+                  f"p95 {percentile(times, 95) * 1e3:.2f} ms, p99 {percentile(times, 99) * 1e3:.2f} ms",
     }
+    # Frame-parallel: one oracle per thread (ctypes releases the GIL).
+    threads = min(16, os.cpu_count() or 1)
+    done = [0] * threads
+    stop = time.perf_counter() + seconds / 2
+
+    def work(k):
+        ok = pyoracle.Oracle(pyoracle.default_config(0, w, h))
+        j = k
+        while time.perf_counter() < stop:
+            ok.run(frames[j % len(frames)])
+            done[k] += 1
+            j += threads
+
+    t0 = time.perf_counter()
+    pool = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
+    for th in pool:
+        th.start()
+    for th in pool:
+        th.join()
+    el = time.perf_counter() - t0
+    out["parallel"] = {"value": round(sum(done) * w * h / el / 1e6, 2), "unit": "Mpix/s", "cores": threads,
+                       "sample": f"{sum(done)} frames in {el:.1f} s on {threads} threads"}
+    return out
 
 
 def uneven_batch(bdcudatag, devmem, scenes, steps):
     """64 frames of four sizes (16 each of 640x480, 1280x800, 1920x1200,
-    4096x3072), 0-32 tags per frame, one graph launch per step."""
+    4096x3072), 0-32 tags per frame, one graph launch per step, after the
+    launch parameters are autotuned for this batch shape."""
     sizes = [(640, 480), (1280, 800), (1920, 1200), (4096, 3072)]
     per = 16
     threads = min(16, os.cpu_count() or 1)
@@ -86,6 +119,9 @@ def uneven_batch(bdcudatag, devmem, scenes, steps):
         all_sizes += [(w, h)] * per
     px = sum(w * h for w, h in all_sizes)
     batch = bdcudatag.Batch(0, all_sizes)
+    batch.detect_device(ptrs)
+    default_ms = batch.replay(steps, device=True) / steps
+    tune = batch.autotune(reps=5)
     batch.prepare_device(ptrs)
     for _ in range(2):
         batch.detect_raw()
@@ -98,7 +134,44 @@ def uneven_batch(bdcudatag, devmem, scenes, steps):
     dev_ms = batch.replay(steps, device=True) / steps
     return {"frames": len(all_sizes), "mpix_per_step": round(px / 1e6, 2),
             "mpix_s": round(px * steps / t / 1e6, 1), "ms_per_step": round(t / steps * 1e3, 3),
-            "device_ms_per_step": round(dev_ms, 3)}
+            "device_ms_per_step": round(dev_ms, 3), "device_ms_default_launch": round(default_ms, 3),
+            "tune": {k: tune[k] for k in ("fit_grid", "decode_grid", "head_ranks")}}
+
+
+def precision(bdcudatag, devmem, scenes, frames_total):
+    """Corner accuracy on the degraded workload (Gaussian blur 1.5, noise 8,
+    20-80 px tags, tilt up to 60 degrees): RMS corner error of every decoded
+    tag against the rendered ground truth, batches of 64 1280x800 frames."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    W, H, B = 1280, 800, 64
+    threads = min(16, os.cpu_count() or 1)
+    batch = bdcudatag.Batch(0, [(W, H)] * B)
+    rms, tags_total, found, false_ids = [], 0, 0, 0
+    with ThreadPoolExecutor(max_workers=threads) as pool:  # ctypes releases the GIL
+        for first in range(0, frames_total, B):
+            n = min(B, frames_total - first)
+            rendered = list(pool.map(lambda k: scenes.scene(3, W, H, first + k), range(B)))
+            res = batch.detect([img for img, _ in rendered])
+            for k in range(n):
+                # ids can repeat within a frame: a detection matches the
+                # nearest ground-truth tag of its id
+                want = [t for t in rendered[k][1] if not t["decoy"]]
+                tags_total += len(want)
+                for det in res[k]:
+                    c = det["corners"].astype(np.float64)
+                    same = [t["corners"] for t in want if t["id"] == det["id"]]
+                    if not same:
+                        false_ids += 1
+                        continue
+                    gt = min(same, key=lambda g: float(((g.mean(0) - c.mean(0)) ** 2).sum()))
+                    found += 1
+                    rms.append(float(np.sqrt(((c - gt) ** 2).sum(1).mean())))
+    r = np.asarray(rms)
+    return {"frames": frames_total, "tags": tags_total, "decoded": found, "false_ids": false_ids,
+            "rms_px_mean": round(float(r.mean()), 4) if len(r) else None,
+            "rms_px_median": round(float(np.median(r)), 4) if len(r) else None,
+            "rms_px_p90": round(float(np.percentile(r, 90)), 4) if len(r) else None}
 
 
 def serving(bdcudatag, devmem, scenes, seconds):
@@ -137,6 +210,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--uneven-steps", type=int, default=10)
+    ap.add_argument("--precision-frames", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames shard "
@@ -160,6 +234,7 @@ def main():
     stages = batch.time_stages(reps=5)
     res = batch.detect_device(ptrs)
     found = sum(len(r) for r in res)
+    tune = batch.autotune(reps=5)  # launch parameters for this batch shape
     batch.prepare_device(ptrs)
     for _ in range(WU):
         batch.detect_raw()
@@ -171,6 +246,13 @@ def main():
     t_batch = time.perf_counter() - t0
     batched_mpix = B * px * K / t_batch / 1e6
     replay_ms = batch.replay(K, device=True)
+    # Same batch from host memory: pinned staging, H2D copy inside the graph.
+    host_list = list(frames)
+    batch.detect(host_list)
+    t0 = time.perf_counter()
+    for _ in range(K):
+        batch.detect(host_list)
+    t_host = time.perf_counter() - t0
 
     # --- single-frame path -------------------------------------------------
     single = None
@@ -228,7 +310,12 @@ def main():
         "single_frame": single,
         "batched_vs_single": round(batched_mpix / single["mpix_s"], 2) if single else None,
         "batch_device_ms_per_step": round(replay_ms / K, 4),
+        "batch_host_input": {"mpix_s": round(B * px * K / t_host / 1e6, 1),
+                             "ms_per_step": round(t_host / K * 1e3, 4),
+                             "note": "frames in pageable host memory: copy to pinned staging, H2D, "
+                                     "kernels, D2H and per-frame result conversion in Python"},
         "detections_per_step": found,
+        "tune": {k: tune[k] for k in ("fit_grid", "decode_grid", "head_ranks")},
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": {
             "bound": "hbm",
@@ -246,6 +333,8 @@ def main():
         out["uneven_batch"] = uneven_batch(bdcudatag, devmem, scenes, args.uneven_steps)
     if args.serve_seconds > 0:
         out["serving"] = serving(bdcudatag, devmem, scenes, args.serve_seconds)
+    if args.precision_frames > 0:
+        out["precision"] = precision(bdcudatag, devmem, scenes, args.precision_frames)
     out["cpu_baseline"] = cpu_baseline(frames, args.cpu_seconds) if args.cpu_seconds > 0 else None
     print(json.dumps(out), flush=True)
 

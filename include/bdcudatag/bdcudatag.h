@@ -233,6 +233,29 @@ BD_API bd_batch* bd_detector_batch(bd_detector* detector);
 BD_API int32_t bd_batch_time_stages(bd_batch* batch, int32_t reps, float* ms);
 BD_API int32_t bd_batch_replay(bd_batch* batch, int32_t reps, int32_t device_path, float* ms);
 
+/*
+ * Launch parameters of a batch (no counterpart in the reference).  They
+ * change speed only: every setting gives bit-identical detections.
+ */
+typedef struct bd_tune {
+    int32_t fit_grid;    /* persistent quad-fit workgroups, 64..16384 (default 1024) */
+    int32_t decode_grid; /* persistent decode workgroups, 64..16384 (default 1024) */
+    int32_t head_ranks;  /* each frame's largest candidates scheduled first, 0..4 (default 2) */
+    int32_t reserved;
+} bd_tune;
+
+BD_API int32_t bd_batch_get_tune(bd_batch* batch, bd_tune* out);
+/* Waits for the batch's stream and re-captures its graphs. */
+BD_API int32_t bd_batch_set_tune(bd_batch* batch, const bd_tune* tune);
+/*
+ * Times `reps` graph replays of the frames of the last detect call on this
+ * batch (host frames stay staged; device frames must still be valid) for a
+ * small set of launch parameters, one parameter at a time keeping the best
+ * so far, and leaves the fastest set in place.  Writes it to *best and the
+ * mean pipeline time of that set to *best_ms.
+ */
+BD_API int32_t bd_batch_autotune(bd_batch* batch, int32_t reps, bd_tune* best, float* best_ms);
+
 /* ---------------------------------------------------------------------- */
 /* Serving: continuous batching over a preallocated arena (version 3).    */
 /* ---------------------------------------------------------------------- */

@@ -1,0 +1,44 @@
+"""Launch parameters (persistent grid sizes, largest-first scheduling)
+change speed only: every setting must give bit-identical detections, and
+the autotuner must leave a valid setting in place and honour its cache."""
+import numpy as np
+import pytest
+
+import bdcudatag
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def same(a, b):
+    assert len(a) == len(b)
+    for x, y in zip(a, b):
+        assert [d["id"] for d in x] == [d["id"] for d in y]
+        for p, q in zip(x, y):
+            assert np.array_equal(p["corners"], q["corners"])
+            assert np.array_equal(p["homography"], q["homography"])
+            assert np.float32(p["decision_margin"]) == np.float32(q["decision_margin"])
+
+
+def test_launch_parameters_keep_results(tmp_path):
+    sizes = [(1280, 800)] * 4 + [(640, 480)] * 2 + [(1920, 1200)]
+    imgs = [scenes.scene(2, w, h, 40 + i)[0] for i, (w, h) in enumerate(sizes)]
+    batch = bdcudatag.Batch(0, sizes)
+    assert batch.get_tune() == {"fit_grid": 1024, "decode_grid": 1024, "head_ranks": 2}
+    ref = batch.detect(imgs)
+    assert sum(len(r) for r in ref) > 0
+    for fg, dg, hr in [(64, 64, 0), (256, 2048, 4), (4096, 512, 1)]:
+        batch.set_tune(fg, dg, hr)
+        assert batch.get_tune() == {"fit_grid": fg, "decode_grid": dg, "head_ranks": hr}
+        same(batch.detect(imgs), ref)
+    for bad in [(10, 1024, 2), (1024, 1 << 20, 2), (1024, 1024, 5)]:
+        with pytest.raises(ValueError):
+            batch.set_tune(*bad)
+    cache = str(tmp_path / "tune.json")
+    first = batch.autotune(reps=2, cache=cache)
+    assert not first["cached"] and first["pipeline_ms"] > 0
+    assert batch.get_tune() == {k: first[k] for k in ("fit_grid", "decode_grid", "head_ranks")}
+    same(batch.detect(imgs), ref)
+    again = batch.autotune(reps=2, cache=cache)
+    assert again["cached"] and {k: again[k] for k in ("fit_grid", "decode_grid", "head_ranks")} == \
+        {k: first[k] for k in ("fit_grid", "decode_grid", "head_ranks")}
