@@ -26,6 +26,10 @@ BDM_API int32_t bdm_download(void* dst, const void* src, size_t bytes) {
     return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
 }
 
+BDM_API int32_t bdm_copy(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice) == hipSuccess ? 0 : -1;
+}
+
 BDM_API int32_t bdm_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -1; }
 
 BDM_API int32_t bdm_device_count(void) {

@@ -23,10 +23,17 @@ def lib():
         L.bdm_free.argtypes = [ctypes.c_void_p]
         L.bdm_upload.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         L.bdm_download.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
-        for f in (L.bdm_free, L.bdm_upload, L.bdm_download, L.bdm_sync, L.bdm_device_count):
+        L.bdm_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        for f in (L.bdm_free, L.bdm_upload, L.bdm_download, L.bdm_copy, L.bdm_sync, L.bdm_device_count):
             f.restype = ctypes.c_int32
         _lib = L
     return _lib
+
+
+def copy(dst: int, src: int, nbytes: int) -> None:
+    """Device-to-device copy of nbytes."""
+    if lib().bdm_copy(dst, src, nbytes) != 0:
+        raise RuntimeError("bdm_copy failed")
 
 
 def sync():

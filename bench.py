@@ -42,6 +42,12 @@ sys.path.insert(0, os.path.join(ROOT, "bd-cudatag_amd"))
 METRIC = "Mpix/s batched vs single-frame at 1280x800 u8; p50/p99 per-frame latency"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 BYTES_PER_PIXEL = 1.0  # algorithmic bytes: the u8 frame read once
+# HBM bytes per input pixel that each kernel moves on the headline workload
+# (64 x 1280x800), from rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by the
+# same counters on a device copy of known size (profiles/traffic_batch64_1280x800.md).
+TRAFFIC_B_PER_PX = {"tile_stats": 1.088, "frame_ref": 0.007, "binarize_runs": 2.416, "merge": 0.895,
+                    "stats": 2.429, "collect": 0.641, "select": 0.031, "fit": 2.189, "decode": 0.397,
+                    "finalize": 0.010}
 
 
 def percentile(v, q):
@@ -211,6 +217,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--uneven-steps", type=int, default=10)
     ap.add_argument("--precision-frames", type=int, default=256)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 10 device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames shard "
@@ -220,6 +228,16 @@ def main():
     import bdcudatag
     import devmem
     import scenes
+
+    if args.calibrate_copy > 0:
+        nbytes = args.calibrate_copy << 20
+        src = devmem.DeviceBuffer(np.zeros(nbytes, dtype=np.uint8))
+        dst = devmem.DeviceBuffer(np.zeros(nbytes, dtype=np.uint8))
+        for _ in range(10):
+            devmem.copy(dst.ptr, src.ptr, nbytes)
+        devmem.sync()
+        print(json.dumps({"calibration": "device-to-device copy", "bytes_per_copy": nbytes, "copies": 10}))
+        return
 
     W, H, B, K, WU = args.width, args.height, args.batch, args.steps, args.warmup
     px = W * H
@@ -324,7 +342,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": (round(TRAFFIC_B_PER_PX[dom] * B * px)
+                        if (B, W, H) == (64, 1280, 800) and dom in TRAFFIC_B_PER_PX else None),
             "stream_kernel": {"kernel": "tile_stats", "achieved": round(stream_achieved, 1),
                               "frac": round(stream_achieved / HBM_PEAK_GBS, 4)},
         },
