@@ -769,10 +769,10 @@ __device__ inline void uf_union(uint32_t* label, uint32_t a, uint32_t b) {
 }
 
 // k_merge LDS: a pair hash set of kPairSlots entries and kMergeStage runs
-// per staged row (rows with more runs are read from global memory); 20 KB
-// in all keeps eight workgroups resident per CU.
+// per staged row (rows with more runs are read from global memory); under
+// 20 KB in all keeps eight workgroups resident per CU.
 constexpr int kPairSlots = 1024;
-constexpr int kMergeStage = 768;
+constexpr int kMergeStage = 760;
 
 // Records the (smaller, larger) label pair in the LDS hash set, or joins
 // the pair directly when 16 probes find no room.
@@ -896,29 +896,30 @@ __global__ __launch_bounds__(256) void k_merge(Plan plan) {
                       pn, i);
     }
     __syncthreads();
+    // Distinct pairs are gathered to the front of the table (each thread
+    // holds its slots' keys in registers meanwhile), then joined one per
+    // thread.
+    __shared__ uint32_t s_npairs;
+    if (t == 0) s_npairs = 0;
+    __syncthreads();
+    unsigned long long mine[kPairSlots / 256];
+    int nmine = 0;
     for (int e = t; e < kPairSlots; e += blockDim.x) {
         const unsigned long long key = pairs[e];
-        if (key == ~0ull) continue;
+        if (key != ~0ull) mine[nmine++] = key;
+    }
+    __syncthreads();
+    for (int q = 0; q < nmine; ++q) pairs[atomicAdd(&s_npairs, 1u)] = mine[q];
+    __syncthreads();
+    const uint32_t npairs = s_npairs;
+    for (uint32_t k = t; k < npairs; k += blockDim.x) {
+        const unsigned long long key = pairs[k];
         uf_union(label, static_cast<uint32_t>(key >> 32), static_cast<uint32_t>(key & 0xffffffffu));
     }
 }
 
 constexpr int kStatSlots = 512;
 constexpr uint32_t kEmpty = 0xffffffffu;
-
-// Walks x to its component root, splitting the path on the way: each
-// visited label is lowered to its grandparent with atomicMin, which can only
-// move it towards the root, so concurrent walkers never undo each other.
-__device__ inline uint32_t walk_to_root(uint32_t* label, uint32_t x) {
-    uint32_t p = label[x];
-    while (p != x) {
-        const uint32_t gp = label[p];
-        if (gp != p) atomicMin(&label[x], gp);
-        x = p;
-        p = gp;
-    }
-    return x;
-}
 
 // One workgroup per band of 8 rows, after all joins.
 //  1. Band roots (flagged runs: the LDS-joined band's roots, or every run of
@@ -945,6 +946,8 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     const int t = threadIdx.x;
     __shared__ uint32_t hkey[kStatSlots], harea[kStatSlots], hx0[kStatSlots], hx1[kStatSlots], hy1[kStatSlots];
     __shared__ uint32_t s_off[kTile + 1];
+    __shared__ uint32_t s_flush[kStatSlots];
+    __shared__ uint32_t s_nflush;
     for (int e = t; e < kStatSlots; e += blockDim.x) {
         hkey[e] = kEmpty;
         harea[e] = 0;
@@ -952,12 +955,13 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         hx1[e] = 0;
         hy1[e] = 0;
     }
-    if (t == 0) {
-        uint32_t acc = 0;
-        for (int r = 0; r <= kTile; ++r) {
-            s_off[r] = acc;
-            if (r < nrows) acc += d->row_count[y0 + r];
-        }
+    if (t < kWave) {
+        // Row offsets of the band: one lane per row, a wave prefix sum.
+        const uint32_t c = t < nrows ? d->row_count[y0 + t] : 0u;
+        uint32_t total;
+        const uint32_t ex = wave_excl_scan(c, t, &total);
+        if (t <= kTile) s_off[t] = ex;
+        if (t == 0) s_nflush = 0;
     }
     __syncthreads();
     uint32_t so[kTile + 1];
@@ -973,11 +977,48 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     const uint32_t band_base = static_cast<uint32_t>(y0) * row_cap;
     uint32_t* label = d->label;
 
-    for (uint32_t i = t; i < nb; i += blockDim.x) {
-        const int r = row_of(i);
-        const uint32_t slot = band_base + static_cast<uint32_t>(r) * row_cap + (i - s_off[r]);
-        if (!(d->runs[slot] & kRootFlag)) continue;
-        atomicMin(&label[slot], walk_to_root(label, slot));
+    // Band roots walk to their root, splitting the path on the way: each
+    // visited label is lowered to its grandparent with atomicMin, which can
+    // only move it towards the root, so concurrent walkers never undo each
+    // other.  A thread advances four walks in lockstep so their loads
+    // overlap.
+    for (uint32_t i0 = t; i0 < nb; i0 += 4 * blockDim.x) {
+        constexpr int kWalks = 4;
+        uint32_t start[kWalks], x[kWalks], p[kWalks], run[kWalks];
+#pragma unroll
+        for (int u = 0; u < kWalks; ++u) {
+            const uint32_t i = i0 + u * blockDim.x;
+            start[u] = 0xffffffffu;
+            if (i < nb) {
+                const int r = row_of(i);
+                start[u] = band_base + static_cast<uint32_t>(r) * row_cap + (i - s_off[r]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kWalks; ++u) {
+            run[u] = start[u] != 0xffffffffu ? d->runs[start[u]] : 0u;
+            p[u] = start[u] != 0xffffffffu && (run[u] & kRootFlag) ? label[start[u]] : 0u;
+            x[u] = (run[u] & kRootFlag) ? start[u] : 0u;   // x == p ends a walk
+        }
+        bool live = true;
+        while (live) {
+            live = false;
+            uint32_t gp[kWalks];
+#pragma unroll
+            for (int u = 0; u < kWalks; ++u) gp[u] = p[u] != x[u] ? label[p[u]] : p[u];
+#pragma unroll
+            for (int u = 0; u < kWalks; ++u) {
+                if (p[u] == x[u]) continue;
+                if (gp[u] != p[u]) atomicMin(&label[x[u]], gp[u]);
+                x[u] = p[u];
+                p[u] = gp[u];
+                live |= p[u] != x[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kWalks; ++u) {
+            if (run[u] & kRootFlag) atomicMin(&label[start[u]], x[u]);
+        }
     }
     __syncthreads();
 
@@ -1041,9 +1082,16 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         account(row[3], slot[3], run[3], lab[3]);
     }
     __syncthreads();
+    // Occupied table entries are compacted first so every thread flushes
+    // about the same number of roots.
     for (int e = t; e < kStatSlots; e += blockDim.x) {
+        if (hkey[e] != kEmpty) s_flush[atomicAdd(&s_nflush, 1u)] = static_cast<uint32_t>(e);
+    }
+    __syncthreads();
+    const uint32_t nflush = s_nflush;
+    for (uint32_t k = t; k < nflush; k += blockDim.x) {
+        const uint32_t e = s_flush[k];
         const uint32_t root = hkey[e];
-        if (root == kEmpty) continue;
         atomicAdd(&d->area[root], harea[e]);
         atomicMin(&d->bbox[4 * static_cast<size_t>(root) + 0], hx0[e]);
         atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 2], hx1[e]);
