@@ -24,7 +24,8 @@ constexpr int kFitIterations = 2;
 constexpr int kFitProbeHalf = 24;
 constexpr int kMaxProbes = 2 * kFitProbeHalf + 1;
 constexpr int kMaxSamples = 16;
-constexpr uint32_t kFitWindow = 8;   // runs checked at each end of a row range first
+constexpr uint32_t kFitWindow = 8;
+constexpr int kMaxCodes = 587;       // largest codebook (tag36h11)   // runs checked at each end of a row range first
 
 // Device frames are addressed by 32-bit byte offsets from their base in the
 // sampling kernels; a frame must span less than 4 GiB.
@@ -103,7 +104,7 @@ enum Shape : int { SH_TILE_ROW = 0, SH_ROW4 = 1, SH_FRAME = 2, SH_TSTAT = 3, SH_
 // first Plan::head_ranks candidates of every frame (its largest) sit in a
 // head region of kHeadRanksMax * nframes entries that is handed out first;
 // the rest follow in a tail region.
-enum WorkSlot : int { kWorkCount = 0, kWorkFitCursor = 1, kWorkDecodeCursor = 2, kWorkHeadCount = 3, kWorkItems = 4 };
+enum WorkSlot : int { kWorkCount = 0, kWorkFitCursor = 1, kWorkHeadCount = 3, kWorkItems = 4 };  // slot 2 unused
 constexpr int kHeadRanksMax = 4;
 
 // Launch parameters that change speed only, never results (autotuned per

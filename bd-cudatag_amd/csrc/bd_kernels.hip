@@ -124,6 +124,37 @@ __device__ inline uint32_t min_lanes_0_31(uint32_t v) {
     return wave_lane(v, 31);
 }
 
+// Minimum / maximum of v over the whole wave (every lane active), by the
+// same DPP row shifts and broadcasts as wave_incl_scan; lane 63 ends up
+// holding the result, which is returned to every lane.
+template <int kCtrl, int kRowMask>
+__device__ inline uint32_t dpp_min(uint32_t x) {
+    return min(x, static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(0xffffffffu), static_cast<int>(x),
+                                                                    kCtrl, kRowMask, 0xf, false)));
+}
+template <int kCtrl, int kRowMask>
+__device__ inline uint32_t dpp_max(uint32_t x) {
+    return max(x, static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), kCtrl, kRowMask, 0xf, false)));
+}
+__device__ inline uint32_t wave_min(uint32_t v) {
+    v = dpp_min<0x111, 0xf>(v);
+    v = dpp_min<0x112, 0xf>(v);
+    v = dpp_min<0x114, 0xf>(v);
+    v = dpp_min<0x118, 0xf>(v);
+    v = dpp_min<0x142, 0xa>(v);
+    v = dpp_min<0x143, 0xc>(v);
+    return wave_lane(v, kWave - 1);
+}
+__device__ inline uint32_t wave_max(uint32_t v) {
+    v = dpp_max<0x111, 0xf>(v);
+    v = dpp_max<0x112, 0xf>(v);
+    v = dpp_max<0x114, 0xf>(v);
+    v = dpp_max<0x118, 0xf>(v);
+    v = dpp_max<0x142, 0xa>(v);
+    v = dpp_max<0x143, 0xc>(v);
+    return wave_lane(v, kWave - 1);
+}
+
 // ---------------------------------------------------------------------------
 // (a) tile statistics
 // ---------------------------------------------------------------------------
@@ -1987,7 +2018,9 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
 }
 
 // Next work-list item of a persistent workgroup: the first is its block
-// index, later ones come from the kernel's shared cursor.
+// index, later ones come from the kernel's shared cursor, claimed only when
+// the previous item is done (a claim made earlier would queue an item behind
+// a long one while other workgroups run dry).
 __device__ inline uint32_t next_item(uint32_t* cursor) {
     __shared__ uint32_t s_next;
     __syncthreads();
@@ -2054,7 +2087,7 @@ __device__ inline bool homography_from_unit_quad(const float (*q)[2], float h[9]
 // of wave 0 applies the reference's preference (refined first, highest
 // score, first on ties).  reference decode_quad :1291-1430 and
 // k_decode_quads :1438-1537.
-__device__ void decode_candidate(const FrameDesc* d, int ci, int t) {
+__device__ void decode_candidate(const FrameDesc* d, int ci, int t, const uint64_t* codes, int ncodes) {
     const int wave = t / kWave, lane = t % kWave;
     bd_detection* res = d->dec + ci;
     const float* qsrc = d->quads + 8 * static_cast<size_t>(ci);
@@ -2063,7 +2096,6 @@ __device__ void decode_candidate(const FrameDesc* d, int ci, int t) {
         return;
     }
     __shared__ float s_diff[8][64];
-    __shared__ float s_sharp[8][64];
     __shared__ int r_id[8], r_ham[8];
     __shared__ float r_margin[8];
     __shared__ float r_corners[8][4][2];
@@ -2098,14 +2130,12 @@ __device__ void decode_candidate(const FrameDesc* d, int ci, int t) {
     }
     const uint64_t bad_cells = __ballot(!cell_ok);
     ok = ok && bad_cells == 0ull;
-    float mn = lane < cells ? raw : 255.0f;
-    float mx = lane < cells ? raw : 0.0f;
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        mn = fminf(mn, __shfl_xor(mn, off, kWave));
-        mx = fmaxf(mx, __shfl_xor(mx, off, kWave));
-    }
-    mn = fminf(255.0f, mn);
-    mx = fmaxf(0.0f, mx);
+    // With every cell inside the frame the samples are in [0, 255], where
+    // float order is the order of the bit patterns (otherwise ok is false
+    // and the extremes are not used).
+    const bool sampled = lane < cells && raw >= 0.0f;
+    const float mn = fminf(255.0f, __uint_as_float(wave_min(sampled ? __float_as_uint(raw) : __float_as_uint(255.0f))));
+    const float mx = fmaxf(0.0f, __uint_as_float(wave_max(sampled ? __float_as_uint(raw) : 0u)));
     ok = ok && !(mx - mn < 10.0f);
     const float thresh = 0.5f * (mn + mx);
     const bool border = lane < cells && (cx == 0 || cy == 0 || cx == dim - 1 || cy == dim - 1);
@@ -2121,19 +2151,17 @@ __device__ void decode_candidate(const FrameDesc* d, int ci, int t) {
         const float lap = 4.0f * diff - df[lane - dim] - df[lane + dim] - df[lane - 1] - df[lane + 1];
         sharp = diff + d->sharpening * lap;
     }
-    s_sharp[wave][lane] = sharp;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-
-    // Code bits and class sums in the reference's row-major order.
+    // Code bits and class sums in the reference's row-major order: every
+    // lane reads the interior cells' values by readlane and adds them in
+    // that order (the same roundings as one serial loop).
     uint64_t code = 0;
     float margin = 0.0f;
-    if (lane == 0 && ok) {
+    if (ok) {
         float white_sum = 0.0f, black_sum = 0.0f;
         int white_count = 0, black_count = 0;
         for (int yy = 1; yy <= dim - 2; ++yy) {
             for (int xx = 1; xx <= dim - 2; ++xx) {
-                const float dv = s_sharp[wave][yy * dim + xx];
+                const float dv = __uint_as_float(wave_lane(__float_as_uint(sharp), yy * dim + xx));
                 code = (code << 1) | (dv > 0.0f ? 1ull : 0ull);
                 if (dv > 0.0f) {
                     white_sum += dv;
@@ -2148,27 +2176,18 @@ __device__ void decode_candidate(const FrameDesc* d, int ci, int t) {
         const float black_avg = black_sum / static_cast<float>(max(black_count, 1));
         margin = fminf(white_avg, black_avg);
     }
-    code = __shfl(code, 0, kWave);
-    // First codeword at the smallest distance, if within max_hamming.
-    int best_d = 1000, best_i = 1 << 30;
+    // First codeword at the smallest distance (codebook in LDS), if within
+    // max_hamming: (distance << 16 | index) orders by distance, then index.
+    uint32_t key = 0xffffffffu;
     if (ok) {
-        const uint64_t* codes = d->codes;
-        for (int i = lane; i < d->code_count; i += kWave) {
-            const int dd = __popcll(codes[i] ^ code);
-            if (dd < best_d) {
-                best_d = dd;
-                best_i = i;
-            }
+        for (int i = lane; i < ncodes; i += kWave) {
+            const uint32_t k = (static_cast<uint32_t>(__popcll(codes[i] ^ code)) << 16) | static_cast<uint32_t>(i);
+            key = min(key, k);
         }
     }
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        const int od = __shfl_xor(best_d, off, kWave);
-        const int oi = __shfl_xor(best_i, off, kWave);
-        if (od < best_d || (od == best_d && oi < best_i)) {
-            best_d = od;
-            best_i = oi;
-        }
-    }
+    key = wave_min(key);
+    const int best_d = key == 0xffffffffu ? 1000 : static_cast<int>(key >> 16);
+    const int best_i = key == 0xffffffffu ? (1 << 30) : static_cast<int>(key & 0xffffu);
     if (lane == 0) {
         int id = -1;
         if (ok && best_d <= d->max_hamming) {
@@ -2216,15 +2235,24 @@ __device__ void decode_candidate(const FrameDesc* d, int ci, int t) {
     }
 }
 
+// Decoding costs about the same per candidate, so workgroups take items by
+// a fixed stride (no cursor).  The codebook (shared by every frame of the
+// batch) is copied to LDS once per persistent workgroup.
 __global__ __launch_bounds__(512) void k_decode(Plan plan) {
     const uint32_t nhead = plan.work[kWorkHeadCount];
     const uint32_t total = nhead + plan.work[kWorkCount];
-    for (uint32_t item = blockIdx.x; item < total; item = next_item(&plan.work[kWorkDecodeCursor])) {
+    if (blockIdx.x >= total) return;
+    __shared__ uint64_t s_codes[kMaxCodes];
+    const int ncodes = min(plan.frames[0].code_count, kMaxCodes);
+    for (int i = threadIdx.x; i < ncodes; i += blockDim.x) s_codes[i] = plan.frames[0].codes[i];
+    __syncthreads();
+    for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
         // Uniform across the workgroup: scalar registers for the descriptor.
         const uint32_t code = __builtin_amdgcn_readfirstlane(work_item(plan, item, nhead));
         int t = threadIdx.x;
         asm volatile("" : "+v"(t));
-        decode_candidate(plan.frames + (code >> 10), static_cast<int>(code & 1023u), t);
+        decode_candidate(plan.frames + (code >> 10), static_cast<int>(code & 1023u), t, s_codes, ncodes);
+        __syncthreads();  // LDS of this item is no longer read
     }
 }
 
