@@ -445,11 +445,17 @@ __device__ inline int run_x0(uint32_t r) { return static_cast<int>(r & 0xffffu);
 __device__ inline int run_x1(uint32_t r) { return static_cast<int>((r >> 16) & 0x7fffu); }
 constexpr uint32_t kRootFlag = 0x80000000u;
 
-__device__ inline uint32_t lds_find(const uint32_t* lab, uint32_t x) {
+__device__ inline uint32_t lds_find(uint32_t* lab, uint32_t x) {
+    // Path splitting: each visited label is lowered to its grandparent.
+    // Parents have smaller indices than their children, so atomicMin only
+    // moves a label towards the root and concurrent finds and unions stay
+    // consistent.
     uint32_t p = lab[x];
     while (p != x) {
+        const uint32_t g = lab[p];
+        if (g != p) atomicMin(&lab[x], g);
         x = p;
-        p = lab[x];
+        p = g;
     }
     return x;
 }
