@@ -133,3 +133,136 @@ def scene_batch(config, width, height, first, count, threads=8):
     if lib().bds_scene_batch(config, width, height, first, count, imgs.ctypes.data, threads) != count:
         raise ValueError("bds_scene_batch failed")
     return imgs
+
+
+# ---------------------------------------------------------------------------
+# Fixture formats of the reference's harnesses
+# ---------------------------------------------------------------------------
+
+def write_pgm(path: str, img: np.ndarray) -> None:
+    """Binary 8-bit PGM (reference scenes.hpp:212-220)."""
+    img = np.ascontiguousarray(img, dtype=np.uint8)
+    with open(path, "wb") as f:
+        f.write(b"P5\n%d %d\n255\n" % (img.shape[1], img.shape[0]))
+        f.write(img.tobytes())
+
+
+def read_pgm(path: str) -> np.ndarray:
+    with open(path, "rb") as f:
+        data = f.read()
+    fields, pos = [], 0
+    while len(fields) < 4:  # magic, width, height, maxval (comments not used)
+        while data[pos:pos + 1].isspace():
+            pos += 1
+        end = pos
+        while not data[end:end + 1].isspace():
+            end += 1
+        fields.append(data[pos:end])
+        pos = end
+    if fields[0] != b"P5" or int(fields[3]) != 255:
+        raise ValueError(f"{path}: not an 8-bit binary PGM")
+    w, h = int(fields[1]), int(fields[2])
+    return np.frombuffer(data[pos + 1:pos + 1 + w * h], dtype=np.uint8).reshape(h, w).copy()
+
+
+def write_scene_set(scene_list, directory: str, prefix: str, family: str = "tag36h11") -> str:
+    """`<directory>/scenes_<prefix>.json` plus one PGM per scene, in the
+    reference exporter's layout (scenes.hpp:222-261), which
+    compare_detectors.py reads.  scene_list: (image, tags) pairs as returned
+    by bench_scene / scene.  Returns the manifest path."""
+    os.makedirs(directory, exist_ok=True)
+    manifest = os.path.join(directory, f"scenes_{prefix}.json")
+    parts = [f'{{\n  "family": "{family}",\n  "scenes": [\n']
+    for i, (img, tags) in enumerate(scene_list):
+        name = f"{prefix}_{i:03d}.pgm"
+        write_pgm(os.path.join(directory, name), img)
+        parts.append(f'    {{"file": "{name}", "width": {img.shape[1]}, "height": {img.shape[0]}, "tags": [')
+        for k, t in enumerate(tags):
+            corners = ", ".join(f"[{c[0]:.3f}, {c[1]:.3f}]" for c in t["corners"])
+            parts.append(f'{"," if k else ""}\n      {{"id": {t["id"]}, "corners": [{corners}]}}')
+        parts.append(f'\n    ]}}{"," if i + 1 < len(scene_list) else ""}\n')
+    parts.append("  ]\n}\n")
+    with open(manifest, "w") as f:
+        f.write("".join(parts))
+    return manifest
+
+
+def read_scene_sets(directory: str) -> list:
+    """Every scenes_*.json of a directory as compare_detectors.py loads them
+    (:317-354), images read without OpenCV."""
+    import glob
+    import json
+    sets = []
+    for path in sorted(glob.glob(os.path.join(directory, "scenes_*.json"))):
+        with open(path) as f:
+            manifest = json.load(f)
+        items = [{"image": read_pgm(os.path.join(directory, e["file"])), "tags": e["tags"],
+                  "width": e["width"], "height": e["height"]} for e in manifest["scenes"]]
+        if items:
+            sets.append({"label": f"{items[0]['width']}x{items[0]['height']}", "width": items[0]["width"],
+                         "height": items[0]["height"], "family": manifest.get("family"), "scenes": items})
+    sets.sort(key=lambda s: (s["width"], s["height"]))
+    return sets
+
+
+def export_reference_sets(directory: str, count: int = 32, seed: int = 6324) -> list:
+    """What the reference's bdcudatag_scenes exporter writes (scene_gen.cpp):
+    `count` benchmark scenes at 640x480, 1280x720, 1280x800 and 1920x1080."""
+    out = []
+    for w, h in ((640, 480), (1280, 720), (1280, 800), (1920, 1080)):
+        out.append(write_scene_set([bench_scene(w, h, i, seed) for i in range(count)], directory, f"{w}x{h}"))
+    return out
+
+
+def write_napa_group(root: str, group: str, scene_list, scenario: str = "synthetic") -> str:
+    """Images plus `<root>/data/<group>/metadata.jsonl` in the layout
+    eval_napa_dataset.py reads (:90-95): one record per image with
+    file_name, width, height, scenario, tag_ids and corners."""
+    import json
+    folder = os.path.join(root, "data", group)
+    os.makedirs(folder, exist_ok=True)
+    path = os.path.join(folder, "metadata.jsonl")
+    with open(path, "w") as f:
+        for i, (img, tags) in enumerate(scene_list):
+            name = f"{group}_{i:05d}.pgm"
+            write_pgm(os.path.join(folder, name), img)
+            real = [t for t in tags if not t.get("decoy", False)]
+            f.write(json.dumps({"file_name": name, "width": int(img.shape[1]), "height": int(img.shape[0]),
+                                "scenario": scenario, "tag_ids": [int(t["id"]) for t in real],
+                                "corners": [np.asarray(t["corners"]).round(3).tolist() for t in real]}) + "\n")
+    return path
+
+
+def score_scene(tags, detections):
+    """The harness's scoring (compare_detectors.py:284-314): a tag is found
+    when a detection of its id has its centre within 0.75 x the tag side of
+    the true centre; corner error is the mean corner distance, corners
+    matched as an unordered set; wrong ids are detections of ids absent from
+    the scene.  detections: dicts with "id" and "corners" (4x2).
+    Returns (found, total, wrong_ids, errors)."""
+    import itertools
+    orders = list(itertools.permutations(range(4)))
+    truth = [(t["id"], np.asarray(t["corners"], dtype=np.float64)) for t in tags]
+    known = {i for i, _ in truth}
+    wrong = sum(1 for d in detections if d["id"] not in known)
+    found, errors = 0, []
+    for tag_id, corners in truth:
+        centre = corners.mean(axis=0)
+        side = 0.5 * (np.linalg.norm(corners[1] - corners[0]) + np.linalg.norm(corners[2] - corners[1]))
+        hits = [np.asarray(d["corners"], dtype=np.float64) for d in detections if d["id"] == tag_id]
+        hits = [c for c in hits if np.linalg.norm(c.mean(axis=0) - centre) <= 0.75 * side]
+        if not hits:
+            continue
+        found += 1
+        errors.append(min(min(float(np.linalg.norm(c[list(o)] - corners, axis=1).mean()) for o in orders)
+                          for c in hits))
+    return found, len(truth), wrong, errors
+
+
+if __name__ == "__main__":
+    import sys
+    # python scenes.py [output-dir] [scene-count]: the reference exporter's sets
+    target = sys.argv[1] if len(sys.argv) > 1 else "scenes"
+    n = int(sys.argv[2]) if len(sys.argv) > 2 else 32
+    for m in export_reference_sets(target, n):
+        print(f"wrote {m}")

@@ -200,3 +200,21 @@ def test_batch_of_uneven_frames_matches_single():
             assert np.array_equal(a["corners"], b["corners"])
         o = pyoracle.Oracle(pyoracle.default_config(0, w, h)).run(imgs[i])
         assert [d["id"] for d in o["detections"]] == [d["id"] for d in got[i]]
+
+
+def test_reference_harness_scoring(tmp_path):
+    # The reference's scene sets (detector/bench/scene_gen.cpp), exported in
+    # its file format and scored by its harness rules
+    # (compare_detectors.py:284-314): every tag found, no wrong id (README
+    # accuracy table, README.md:140-155).
+    scenes.export_reference_sets(str(tmp_path), 4)
+    for s in scenes.read_scene_sets(str(tmp_path)):
+        det = bdcudatag.Detector(0, s["width"], s["height"], max_detections=32)
+        found = total = wrong = 0
+        errs = []
+        for sc in s["scenes"]:
+            f, n, w, e = scenes.score_scene(sc["tags"], det.detect(sc["image"]))
+            found, total, wrong = found + f, total + n, wrong + w
+            errs += e
+        assert found == total and wrong == 0, s["label"]
+        assert np.median(errs) < 0.3, (s["label"], np.median(errs))
