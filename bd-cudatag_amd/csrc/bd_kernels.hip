@@ -515,6 +515,10 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
     uint8_t* pool_min = reinterpret_cast<uint8_t*>(b_run);
     uint8_t* pool_max = pool_min + 3 * 1026;
     uint64_t (*masks)[128] = reinterpret_cast<uint64_t (*)[128]>(b_lab);
+    // Per row and 32-px half word: runs of the row ending before it (pass B),
+    // kept until the band union starts, after the masks in b_lab.
+    uint16_t (*end_before)[256] = reinterpret_cast<uint16_t (*)[256]>(b_lab + kTile * 128 * 2);
+    static_assert(kTile * 128 * 8 + kTile * 256 * 2 <= kBandRuns * 4, "masks and end counts fit in b_lab");
 
     // Pooled 3x3 thresholds for this tile row (k_tile_threshold): the three
     // tile rows are staged in LDS (padded with neutral values), then pooled.
@@ -678,7 +682,10 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
             uint32_t tot_s, tot_e;
             uint32_t ks = carry_s + wave_excl_scan(__popc(starts), lane, &tot_s);
             uint32_t ke = carry_e + wave_excl_scan(__popc(ends), lane, &tot_e);
-            if (hw < nh) d->run_dir[static_cast<size_t>(y) * (nh + 1) + hw] = ke | (ks << 16);
+            if (hw < nh) {
+                d->run_dir[static_cast<size_t>(y) * (nh + 1) + hw] = ke | (ks << 16);
+                end_before[r][hw] = static_cast<uint16_t>(ke);
+            }
             const int xb = hw * 32;
             while (starts | ends) {
                 if (starts) {
@@ -730,14 +737,53 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
         for (int q = 1; q < kTile; ++q) r += i >= so[q] ? 1 : 0;
         return r;
     };
+    // The first run of the row above that touches run i (x1' >= x0 - 1) is
+    // that row's first run plus its runs ending before x0 - 1: the row's end
+    // count before x0 - 1's half word plus the end bits below x0 - 1 in it.
+    // Computed from the masks (still in b_lab) for a thread's first
+    // kFirstAhead runs; later runs (bands of more than 1024 + row-0 runs)
+    // search the row above instead.
+    constexpr int kFirstAhead = 4;
+    int first_ahead[kFirstAhead];
+#pragma unroll
+    for (int k = 0; k < kFirstAhead; ++k) {
+        const uint32_t i = so[1] + t + k * blockDim.x;
+        first_ahead[k] = 0;
+        if (i < nb) {
+            const int r = row_of(i);
+            const int p = run_x0(b_run[i]) - 1;
+            uint32_t before = 0;
+            if (p > 0) {
+                const int h = p >> 5, bit = p & 31;
+                uint64_t st, en;
+                edges(r - 1, h >> 1, &st, &en);
+                const uint32_t ends = static_cast<uint32_t>(en >> (32 * (h & 1)));
+                before = end_before[r - 1][h] + static_cast<uint32_t>(__popc(ends & ((1u << bit) - 1u)));
+            }
+            first_ahead[k] = static_cast<int>(s_off[r - 1] + before);
+        }
+    }
+    __syncthreads();
     for (uint32_t i = t; i < nb; i += blockDim.x) b_lab[i] = i;
     __syncthreads();
-    for (uint32_t i = so[1] + t; i < nb; i += blockDim.x) {
+    const auto join_above = [&](uint32_t i, int first) {
         const int r = row_of(i);
-        const uint32_t beg = s_off[r], pbeg = s_off[r - 1];
-        const uint32_t run = b_run[i];
-        const int x0 = run_x0(run), x1 = run_x1(run);
-        int lo = static_cast<int>(pbeg), hi = static_cast<int>(beg) - 1, first = static_cast<int>(beg);
+        const int beg = static_cast<int>(s_off[r]);
+        const int x1 = run_x1(b_run[i]);
+        for (int k = first; k < beg; ++k) {
+            if (run_x0(b_run[k]) > x1 + 1) break;
+            lds_union(b_lab, i, static_cast<uint32_t>(k));
+        }
+    };
+#pragma unroll
+    for (int k = 0; k < kFirstAhead; ++k) {
+        const uint32_t i = so[1] + t + k * blockDim.x;
+        if (i < nb) join_above(i, first_ahead[k]);
+    }
+    for (uint32_t i = so[1] + t + kFirstAhead * blockDim.x; i < nb; i += blockDim.x) {
+        const int r = row_of(i);
+        const int x0 = run_x0(b_run[i]);
+        int lo = static_cast<int>(s_off[r - 1]), hi = static_cast<int>(s_off[r]) - 1, first = hi + 1;
         while (lo <= hi) {
             const int mid = (lo + hi) >> 1;
             if (run_x1(b_run[mid]) >= x0 - 1) {
@@ -747,10 +793,7 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
                 lo = mid + 1;
             }
         }
-        for (int k = first; k < static_cast<int>(beg); ++k) {
-            if (run_x0(b_run[k]) > x1 + 1) break;
-            lds_union(b_lab, i, static_cast<uint32_t>(k));
-        }
+        join_above(i, first);
     }
     __syncthreads();
     for (uint32_t i = t; i < nb; i += blockDim.x) b_lab[i] = lds_find(b_lab, i);
