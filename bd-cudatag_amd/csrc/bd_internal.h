@@ -24,8 +24,8 @@ constexpr int kFitIterations = 2;
 constexpr int kFitProbeHalf = 24;
 constexpr int kMaxProbes = 2 * kFitProbeHalf + 1;
 constexpr int kMaxSamples = 16;
-constexpr uint32_t kFitWindow = 8;
-constexpr int kMaxCodes = 587;       // largest codebook (tag36h11)   // runs checked at each end of a row range first
+constexpr uint32_t kFitWindow = 8;   // runs checked at each end of a row range first
+constexpr int kMaxCodes = 587;       // largest codebook (tag36h11)
 
 // Device frames are addressed by 32-bit byte offsets from their base in the
 // sampling kernels; a frame must span less than 4 GiB.
@@ -38,11 +38,12 @@ inline bool frame_span_fits(int width, int height, int stride) {
 // radix select over global memory.
 constexpr int kSortCap = 8192;
 // Runs of one 8-row band joined in LDS by k_binarize_runs; a band with more
-// runs is joined row by row in global memory by k_merge instead.  4096 runs
-// (32 KB of band arrays) keeps four workgroups resident per CU and covers
-// noisy 1280-2048 px wide frames (about 1100 runs per band on average, 2100
-// at most, on the eight-tag 1280x800 workload).
-constexpr int kBandRuns = 4096;
+// runs is joined row by row in global memory by k_merge instead.  3072 runs
+// (24 KB of band arrays, the least that holds the band's pixel masks and end
+// counts) keeps six workgroups resident per CU and covers noisy 1280-1920 px
+// wide frames (about 930 runs per band on average, 2000 at most, on the
+// eight-tag 1280x800 workload).
+constexpr int kBandRuns = 3072;
 
 // Per-frame scalar block (u32 each).  Accumulators are cleared by the kernel
 // that consumes them, so a replayed graph needs no memset nodes.

@@ -493,7 +493,7 @@ __device__ inline void lds_union(uint32_t* lab, uint32_t a, uint32_t b) {
 //     word and only they get component statistics initialised.  A band with
 //     more runs than the LDS buffer leaves every run a root and lets k_merge
 //     join all of its rows.
-__global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_binarize_runs(Plan plan) {
     int ty;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
     if (!d->active) return;
@@ -591,6 +591,9 @@ __global__ __launch_bounds__(256) void k_binarize_runs(Plan plan) {
                     }
                 }
             } else {
+                // Frame edge or unaligned frame: one pixel at a time (kept
+                // rolled so the unrolled form's registers do not spill).
+#pragma unroll 1
                 for (int k = 0; k < 32; ++k) {
                     const int x = xs + k;
                     uint32_t white = 1;
