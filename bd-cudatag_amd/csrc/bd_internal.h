@@ -112,7 +112,7 @@ constexpr int kHeadRanksMax = 4;
 // batch shape): persistent grid sizes of k_fit and k_decode and the number
 // of each frame's largest candidates scheduled first.
 struct Tune {
-    int32_t fit_grid = 1024;     // 4 resident 256-thread workgroups per CU
+    int32_t fit_grid = 1280;     // 5 resident 256-thread workgroups per CU
     int32_t decode_grid = 1024;  // 4 resident 512-thread workgroups per CU
     int32_t head_ranks = 2;
 };

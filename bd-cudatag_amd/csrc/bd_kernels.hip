@@ -1064,33 +1064,43 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     // visited label is lowered to its grandparent with atomicMin, which can
     // only move it towards the root, so concurrent walkers never undo each
     // other.  A thread advances four walks in lockstep so their loads
-    // overlap.
-    for (uint32_t i0 = t; i0 < nb; i0 += 4 * blockDim.x) {
-        constexpr int kWalks = 4;
-        uint32_t start[kWalks], x[kWalks], p[kWalks], run[kWalks];
+    // overlap.  Walks only visit band roots (a band root's label always
+    // names another band root), so the labels of the other runs are final
+    // from the start.
+    constexpr int kItems = 4;
+    const auto locate = [&](uint32_t i0, int* row, uint32_t* slot) {
 #pragma unroll
-        for (int u = 0; u < kWalks; ++u) {
+        for (int u = 0; u < kItems; ++u) {
             const uint32_t i = i0 + u * blockDim.x;
-            start[u] = 0xffffffffu;
-            if (i < nb) {
-                const int r = row_of(i);
-                start[u] = band_base + static_cast<uint32_t>(r) * row_cap + (i - s_off[r]);
-            }
+            row[u] = i < nb ? row_of(i) : -1;
+            slot[u] = row[u] >= 0 ? band_base + static_cast<uint32_t>(row[u]) * row_cap + (i - s_off[row[u]]) : 0u;
         }
+    };
+    const auto load = [&](const int* row, const uint32_t* slot, uint32_t* run, uint32_t* lab) {
 #pragma unroll
-        for (int u = 0; u < kWalks; ++u) {
-            run[u] = start[u] != 0xffffffffu ? d->runs[start[u]] : 0u;
-            p[u] = start[u] != 0xffffffffu && (run[u] & kRootFlag) ? label[start[u]] : 0u;
-            x[u] = (run[u] & kRootFlag) ? start[u] : 0u;   // x == p ends a walk
+        for (int u = 0; u < kItems; ++u) {
+            run[u] = row[u] >= 0 ? d->runs[slot[u]] : 0u;
+            lab[u] = row[u] >= 0 ? label[slot[u]] : 0u;
+        }
+    };
+    // On return a band root's lab[] is its component root, written to its
+    // label as well.
+    const auto walk = [&](const uint32_t* slot, const uint32_t* run, uint32_t* lab) {
+        uint32_t x[kItems], p[kItems];
+#pragma unroll
+        for (int u = 0; u < kItems; ++u) {
+            const bool br = (run[u] & kRootFlag) != 0;
+            x[u] = br ? slot[u] : 0u;   // x == p ends a walk
+            p[u] = br ? lab[u] : 0u;
         }
         bool live = true;
         while (live) {
             live = false;
-            uint32_t gp[kWalks];
+            uint32_t gp[kItems];
 #pragma unroll
-            for (int u = 0; u < kWalks; ++u) gp[u] = p[u] != x[u] ? label[p[u]] : p[u];
+            for (int u = 0; u < kItems; ++u) gp[u] = p[u] != x[u] ? label[p[u]] : p[u];
 #pragma unroll
-            for (int u = 0; u < kWalks; ++u) {
+            for (int u = 0; u < kItems; ++u) {
                 if (p[u] == x[u]) continue;
                 if (gp[u] != p[u]) atomicMin(&label[x[u]], gp[u]);
                 x[u] = p[u];
@@ -1099,66 +1109,82 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
             }
         }
 #pragma unroll
-        for (int u = 0; u < kWalks; ++u) {
-            if (run[u] & kRootFlag) atomicMin(&label[start[u]], x[u]);
+        for (int u = 0; u < kItems; ++u) {
+            if (run[u] & kRootFlag) {
+                atomicMin(&label[slot[u]], x[u]);
+                lab[u] = x[u];
+            }
         }
+    };
+    // The band's first kItems * blockDim.x runs stay in registers from the
+    // walk to the accumulation; a longer band reloads the rest.
+    int row0[kItems];
+    uint32_t slot0[kItems], run0[kItems], lab0[kItems];
+    locate(t, row0, slot0);
+    load(row0, slot0, run0, lab0);
+    walk(slot0, run0, lab0);
+    for (uint32_t i0 = t + kItems * blockDim.x; i0 < nb; i0 += kItems * blockDim.x) {
+        int row[kItems];
+        uint32_t slot[kItems], run[kItems], lab[kItems];
+        locate(i0, row, slot);
+        load(row, slot, run, lab);
+        walk(slot, run, lab);
     }
     __syncthreads();
 
-    for (uint32_t i0 = t; i0 < nb; i0 += 4 * blockDim.x) {
-        constexpr int kItems = 4;
-        uint32_t slot[kItems], run[kItems], lab[kItems];
-        int row[kItems];
-#pragma unroll
-        for (int u = 0; u < kItems; ++u) {
-            const uint32_t i = i0 + u * blockDim.x;
-            row[u] = i < nb ? row_of(i) : -1;
-            slot[u] = i < nb ? band_base + static_cast<uint32_t>(row[u]) * row_cap + (i - s_off[row[u]]) : 0;
+    // Every run takes its root and is accounted.  One call per item with
+    // scalar arguments (a loop over the item arrays with the probe loop
+    // inside would not unroll and would put the arrays in scratch memory).
+    const auto account = [&](int rowu, uint32_t slotu, uint32_t runu, uint32_t labu) {
+        if (rowu < 0) return;
+        const uint32_t root = labu;
+        const int y = y0 + rowu;
+        label[slotu] = root;
+        if (root == slotu) d->bbox[4 * static_cast<size_t>(root) + 1] = static_cast<uint32_t>(y);
+        const uint32_t x0 = static_cast<uint32_t>(run_x0(runu)), x1 = static_cast<uint32_t>(run_x1(runu));
+        int found = -1;
+        const uint32_t h = (root * 2654435761u) >> 23;
+        for (int probe = 0; probe < 32; ++probe) {
+            const int e = static_cast<int>((h + static_cast<uint32_t>(probe)) & (kStatSlots - 1));
+            uint32_t k = hkey[e];
+            if (k == kEmpty) k = atomicCAS(&hkey[e], kEmpty, root);
+            if (k == kEmpty || k == root) {
+                found = e;
+                break;
+            }
         }
-#pragma unroll
-        for (int u = 0; u < kItems; ++u) {
-            run[u] = row[u] >= 0 ? d->runs[slot[u]] : 0u;
-            lab[u] = row[u] >= 0 ? label[slot[u]] : 0u;
+        if (found >= 0) {
+            atomicAdd(&harea[found], x1 - x0 + 1);
+            atomicMin(&hx0[found], x0);
+            atomicMax(&hx1[found], x1);
+            atomicMax(&hy1[found], static_cast<uint32_t>(y));
+        } else {
+            atomicAdd(&d->area[root], x1 - x0 + 1);
+            atomicMin(&d->bbox[4 * static_cast<size_t>(root) + 0], x0);
+            atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 2], x1);
+            atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 3], static_cast<uint32_t>(y));
         }
-        // A band root's label is now its root; other runs point at their
-        // band root.
+    };
+    // A run that is not a band root points at its band root, whose label
+    // is now the component root.
+    const auto resolve = [&](const int* row, const uint32_t* run, uint32_t* lab) {
 #pragma unroll
         for (int u = 0; u < kItems; ++u) {
             if (row[u] >= 0 && !(run[u] & kRootFlag)) lab[u] = label[lab[u]];
         }
-        // One call per item with scalar arguments (a loop over the item
-        // arrays with the probe loop inside would not unroll and would put
-        // the arrays in scratch memory).
-        const auto account = [&](int rowu, uint32_t slotu, uint32_t runu, uint32_t labu) {
-            if (rowu < 0) return;
-            const uint32_t root = labu;
-            const int y = y0 + rowu;
-            label[slotu] = root;
-            if (root == slotu) d->bbox[4 * static_cast<size_t>(root) + 1] = static_cast<uint32_t>(y);
-            const uint32_t x0 = static_cast<uint32_t>(run_x0(runu)), x1 = static_cast<uint32_t>(run_x1(runu));
-            int found = -1;
-            const uint32_t h = (root * 2654435761u) >> 23;
-            for (int probe = 0; probe < 32; ++probe) {
-                const int e = static_cast<int>((h + static_cast<uint32_t>(probe)) & (kStatSlots - 1));
-                uint32_t k = hkey[e];
-                if (k == kEmpty) k = atomicCAS(&hkey[e], kEmpty, root);
-                if (k == kEmpty || k == root) {
-                    found = e;
-                    break;
-                }
-            }
-            if (found >= 0) {
-                atomicAdd(&harea[found], x1 - x0 + 1);
-                atomicMin(&hx0[found], x0);
-                atomicMax(&hx1[found], x1);
-                atomicMax(&hy1[found], static_cast<uint32_t>(y));
-            } else {
-                atomicAdd(&d->area[root], x1 - x0 + 1);
-                atomicMin(&d->bbox[4 * static_cast<size_t>(root) + 0], x0);
-                atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 2], x1);
-                atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 3], static_cast<uint32_t>(y));
-            }
-        };
+    };
+    resolve(row0, run0, lab0);
+    account(row0[0], slot0[0], run0[0], lab0[0]);
+    account(row0[1], slot0[1], run0[1], lab0[1]);
+    account(row0[2], slot0[2], run0[2], lab0[2]);
+    account(row0[3], slot0[3], run0[3], lab0[3]);
+    for (uint32_t i0 = t + kItems * blockDim.x; i0 < nb; i0 += kItems * blockDim.x) {
+        int row[kItems];
+        uint32_t slot[kItems], run[kItems], lab[kItems];
+        locate(i0, row, slot);
+        load(row, slot, run, lab);
+        // A band root's label is its root (written by the walk).
+        resolve(row, run, lab);
         account(row[0], slot[0], run[0], lab[0]);
         account(row[1], slot[1], run[1], lab[1]);
         account(row[2], slot[2], run[2], lab[2]);
@@ -2090,7 +2116,7 @@ __device__ inline uint32_t work_item(const Plan& plan, uint32_t item, uint32_t n
 
 // Persistent workgroups over the batch's candidate list (k_select appends
 // frame << 10 | candidate for every selected candidate of every frame).
-__global__ __launch_bounds__(256) void k_fit(Plan plan) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_fit(Plan plan) {
     const uint32_t nhead = plan.work[kWorkHeadCount];
     const uint32_t total = nhead + plan.work[kWorkCount];
     for (uint32_t item = blockIdx.x; item < total; item = next_item(&plan.work[kWorkFitCursor])) {

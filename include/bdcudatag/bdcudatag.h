@@ -238,7 +238,7 @@ BD_API int32_t bd_batch_replay(bd_batch* batch, int32_t reps, int32_t device_pat
  * change speed only: every setting gives bit-identical detections.
  */
 typedef struct bd_tune {
-    int32_t fit_grid;    /* persistent quad-fit workgroups, 64..16384 (default 1024) */
+    int32_t fit_grid;    /* persistent quad-fit workgroups, 64..16384 (default 1280) */
     int32_t decode_grid; /* persistent decode workgroups, 64..16384 (default 1024) */
     int32_t head_ranks;  /* each frame's largest candidates scheduled first, 0..4 (default 2) */
     int32_t reserved;

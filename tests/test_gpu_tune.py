@@ -24,7 +24,7 @@ def test_launch_parameters_keep_results(tmp_path):
     sizes = [(1280, 800)] * 4 + [(640, 480)] * 2 + [(1920, 1200)]
     imgs = [scenes.scene(2, w, h, 40 + i)[0] for i, (w, h) in enumerate(sizes)]
     batch = bdcudatag.Batch(0, sizes)
-    assert batch.get_tune() == {"fit_grid": 1024, "decode_grid": 1024, "head_ranks": 2}
+    assert batch.get_tune() == {"fit_grid": 1280, "decode_grid": 1024, "head_ranks": 2}
     ref = batch.detect(imgs)
     assert sum(len(r) for r in ref) > 0
     for fg, dg, hr in [(64, 64, 0), (256, 2048, 4), (4096, 512, 1)]:
