@@ -375,7 +375,7 @@ int32_t bd_batch_get_tune(bd_batch* batch, bd_tune* out) {
         return BD_ERROR_INVALID_ARGUMENT;
     }
     const bd::Tune t = batch->engine->tune();
-    *out = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, 0};
+    *out = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, t.streams};
     return BD_OK;
 }
 
@@ -388,6 +388,7 @@ int32_t bd_batch_set_tune(bd_batch* batch, const bd_tune* tune) {
     t.fit_grid = tune->fit_grid;
     t.decode_grid = tune->decode_grid;
     t.head_ranks = tune->head_ranks;
+    t.streams = tune->streams == 0 ? bd::Tune{}.streams : tune->streams;
     std::string error;
     if (!batch->engine->set_tune(t, &error)) {
         set_error(error);
@@ -408,7 +409,7 @@ int32_t bd_batch_autotune(bd_batch* batch, int32_t reps, bd_tune* best, float* b
         set_error(error);
         return BD_ERROR_CUDA;
     }
-    if (best) *best = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, 0};
+    if (best) *best = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, t.streams};
     if (best_ms) *best_ms = ms;
     return BD_OK;
 }

@@ -53,14 +53,23 @@ Engine::~Engine() { release(); }
 
 void Engine::release() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    if (stream2_) (void)hipStreamSynchronize(stream2_);
     if (graph_host_) (void)hipGraphExecDestroy(graph_host_);
     if (graph_dev_) (void)hipGraphExecDestroy(graph_dev_);
+    for (int l = 0; l < 2; ++l) {
+        if (lane_graph_host_[l]) (void)hipGraphExecDestroy(lane_graph_host_[l]);
+        if (lane_graph_dev_[l]) (void)hipGraphExecDestroy(lane_graph_dev_[l]);
+        lane_graph_host_[l] = lane_graph_dev_[l] = nullptr;
+    }
     if (ev_start_) (void)hipEventDestroy(ev_start_);
     if (ev_stop_) (void)hipEventDestroy(ev_stop_);
     for (hipEvent_t& e : stage_events_) {
         if (e) (void)hipEventDestroy(e);
         e = nullptr;
     }
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    if (ev_join_) (void)hipEventDestroy(ev_join_);
+    if (stream2_) (void)hipStreamDestroy(stream2_);
     if (stream_) (void)hipStreamDestroy(stream_);
     if (arena_) (void)hipFree(arena_);
     if (h_input_) (void)hipHostFree(h_input_);
@@ -69,6 +78,8 @@ void Engine::release() {
     if (h_desc_dev_) (void)hipHostFree(h_desc_dev_);
     graph_host_ = graph_dev_ = nullptr;
     ev_start_ = ev_stop_ = nullptr;
+    ev_fork_ = ev_join_ = nullptr;
+    stream2_ = nullptr;
     stream_ = nullptr;
     arena_ = nullptr;
     h_input_ = nullptr;
@@ -143,6 +154,34 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     for (int f = 0; f < nf; ++f) work_cap += static_cast<size_t>(maxc[f]);
     const size_t work_words = kWorkItems + static_cast<size_t>(kHeadRanksMax) * nf + work_cap;
     const size_t o_work = take(work_words * 4);
+    // Lanes: frames [0, split) and [split, nf), split where the two pixel
+    // counts are closest.
+    int split = 0;
+    if (nf >= 2) {
+        uint64_t total_px = 0, best = ~0ull, run = 0;
+        for (int f = 0; f < nf; ++f) total_px += static_cast<uint64_t>(w_[f]) * h_[f];
+        for (int f = 1; f < nf; ++f) {
+            run += static_cast<uint64_t>(w_[f - 1]) * h_[f - 1];
+            const uint64_t diff = run * 2 > total_px ? run * 2 - total_px : total_px - run * 2;
+            if (diff < best) {
+                best = diff;
+                split = f;
+            }
+        }
+    }
+    lanes_ = split > 0 ? 2 : 1;
+    lane_first_[0] = 0;
+    lane_first_[1] = split;
+    lane_count_[0] = split;
+    lane_count_[1] = nf - split;
+    const int* lane_first = lane_first_;
+    const int* lane_count = lane_count_;
+    size_t lane_cap[2] = {0, 0}, o_lane_work[2] = {0, 0}, o_lane_prefix[2] = {0, 0};
+    for (int l = 0; l < 2 && lanes_ == 2; ++l) {
+        for (int f = lane_first[l]; f < lane_first[l] + lane_count[l]; ++f) lane_cap[l] += static_cast<size_t>(maxc[f]);
+        o_lane_work[l] = take((kWorkItems + static_cast<size_t>(kHeadRanksMax) * lane_count[l] + lane_cap[l]) * 4);
+        o_lane_prefix[l] = take(static_cast<size_t>(SH_COUNT) * (lane_count[l] + 1) * 4);
+    }
     const size_t o_scal = take(static_cast<size_t>(nf) * S_COUNT * 4);
     const size_t o_out = take(static_cast<size_t>(nf) * max_det_ * sizeof(bd_detection));
     const size_t o_desc_h = take(static_cast<size_t>(nf) * sizeof(FrameDesc));
@@ -234,10 +273,27 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
             prefix[SH_TSTAT * (nf + 1) + f] + (d.tile_h + rows_per_wg - 1) / rows_per_wg;
     }
     std::memcpy(h_desc_dev_, desc_.data(), desc_.size() * sizeof(FrameDesc));
+    // Per-lane prefix tables: the whole-batch table restricted to the lane's
+    // frames and rebased to zero.
+    std::vector<int32_t> lane_prefix[2];
+    for (int l = 0; l < 2 && lanes_ == 2; ++l) {
+        const int n = lane_count[l], first = lane_first[l];
+        lane_prefix[l].assign(static_cast<size_t>(SH_COUNT) * (n + 1), 0);
+        for (int k = 0; k < SH_COUNT; ++k) {
+            const int32_t* src = prefix.data() + static_cast<size_t>(k) * (nf + 1);
+            for (int i = 0; i <= n; ++i) lane_prefix[l][static_cast<size_t>(k) * (n + 1) + i] = src[first + i] - src[first];
+        }
+    }
 
     st = hipMemcpy(d_desc_host_, desc_.data(), desc_.size() * sizeof(FrameDesc), hipMemcpyHostToDevice);
     if (st == hipSuccess) st = hipMemcpy(d_desc_dev_, desc_.data(), desc_.size() * sizeof(FrameDesc), hipMemcpyHostToDevice);
     if (st == hipSuccess) st = hipMemcpy(d_prefix_, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice);
+    for (int l = 0; l < 2 && lanes_ == 2 && st == hipSuccess; ++l) {
+        st = hipMemcpy(base + o_lane_prefix[l], lane_prefix[l].data(), lane_prefix[l].size() * 4, hipMemcpyHostToDevice);
+        if (st == hipSuccess)
+            st = hipMemset(base + o_lane_work[l], 0,
+                           (kWorkItems + static_cast<size_t>(kHeadRanksMax) * lane_count[l] + lane_cap[l]) * 4);
+    }
     if (st == hipSuccess) st = hipMemcpy(d_codes_, codes, static_cast<size_t>(code_count) * 8, hipMemcpyHostToDevice);
     // Accumulators start cleared; afterwards each consumer kernel clears them.
     for (int f = 0; f < nf && st == hipSuccess; ++f) {
@@ -264,8 +320,25 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     for (int k = 0; k < SH_COUNT; ++k) plan_host_.blocks[k] = prefix[k * (nf + 1) + nf];
     plan_dev_ = plan_host_;
     plan_dev_.frames = d_desc_dev_;
+    for (int l = 0; l < 2 && lanes_ == 2; ++l) {
+        const int n = lane_count[l];
+        Plan& p = lane_host_[l];
+        p = plan_host_;
+        p.frames = d_desc_host_ + lane_first[l];
+        p.prefix = reinterpret_cast<int32_t*>(base + o_lane_prefix[l]);
+        p.work = reinterpret_cast<uint32_t*>(base + o_lane_work[l]);
+        p.nframes = n;
+        p.work_cap = static_cast<int32_t>(lane_cap[l]);
+        for (int k = 0; k < SH_COUNT; ++k) p.blocks[k] = lane_prefix[l][static_cast<size_t>(k) * (n + 1) + n];
+        lane_dev_[l] = p;
+        lane_dev_[l].frames = d_desc_dev_ + lane_first[l];
+    }
 
+    // Every stream at the default priority.
     st = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
+    if (st == hipSuccess && lanes_ == 2) st = hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking);
+    if (st == hipSuccess && lanes_ == 2) st = hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming);
+    if (st == hipSuccess && lanes_ == 2) st = hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming);
     if (st == hipSuccess) st = hipEventCreate(&ev_start_);
     if (st == hipSuccess) st = hipEventCreate(&ev_stop_);
     if (st == hipSuccess) st = hipDeviceSynchronize();
@@ -275,59 +348,97 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     }
     stats_.assign(nf, FrameStats{});
 
-    if (p_.use_graph) {
-        // Graph capture is an optimisation: on failure the engine falls back
-        // to direct launches, as the reference does (gpu_detector.cu:2051-2056).
-        std::string ignored;
-        if (!capture(false, &graph_host_, &ignored)) graph_host_ = nullptr;
-        if (!capture(true, &graph_dev_, &ignored)) graph_dev_ = nullptr;
-    }
+    capture_all();
     return true;
 }
 
-hipError_t Engine::enqueue(bool device_path) {
+hipError_t Engine::enqueue_lane(bool device_path, int lane) {
     const int nf = frame_count();
+    const int first = lane < 0 ? 0 : lane_first_[lane];
+    const int n = lane < 0 ? nf : lane_count_[lane];
+    const Plan& plan = lane < 0 ? (device_path ? plan_dev_ : plan_host_) : (device_path ? lane_dev_[lane] : lane_host_[lane]);
+    hipStream_t s = lane == 1 ? stream2_ : stream_;
     hipError_t st;
     if (device_path) {
-        st = hipMemcpyAsync(d_desc_dev_, h_desc_dev_, static_cast<size_t>(nf) * sizeof(FrameDesc),
-                            hipMemcpyHostToDevice, stream_);
-        if (st == hipSuccess) st = launch_pipeline(plan_dev_, stream_);
+        st = hipMemcpyAsync(d_desc_dev_ + first, h_desc_dev_ + first, static_cast<size_t>(n) * sizeof(FrameDesc),
+                            hipMemcpyHostToDevice, s);
     } else {
-        st = hipMemcpyAsync(d_input_, h_input_, input_bytes_, hipMemcpyHostToDevice, stream_);
-        if (st == hipSuccess) st = launch_pipeline(plan_host_, stream_);
+        const size_t begin = input_off_[first];
+        const size_t end = first + n < nf ? input_off_[first + n] : input_bytes_;
+        st = hipMemcpyAsync(d_input_ + begin, h_input_ + begin, end - begin, hipMemcpyHostToDevice, s);
     }
+    if (st == hipSuccess) st = launch_pipeline(plan, s);
     if (st == hipSuccess)
-        st = hipMemcpyAsync(h_scal_, d_scal_, static_cast<size_t>(nf) * S_COUNT * 4,
-                            hipMemcpyDeviceToHost, stream_);
+        st = hipMemcpyAsync(h_scal_ + static_cast<size_t>(first) * S_COUNT, d_scal_ + static_cast<size_t>(first) * S_COUNT,
+                            static_cast<size_t>(n) * S_COUNT * 4, hipMemcpyDeviceToHost, s);
     if (st == hipSuccess)
-        st = hipMemcpyAsync(h_out_, d_out_, static_cast<size_t>(nf) * max_det_ * sizeof(bd_detection),
-                            hipMemcpyDeviceToHost, stream_);
+        st = hipMemcpyAsync(h_out_ + static_cast<size_t>(first) * max_det_, d_out_ + static_cast<size_t>(first) * max_det_,
+                            static_cast<size_t>(n) * max_det_ * sizeof(bd_detection), hipMemcpyDeviceToHost, s);
     return st;
 }
 
-bool Engine::capture(bool device_path, hipGraphExec_t* exec, std::string* error) {
-    hipError_t st = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
-    if (st != hipSuccess) {
-        if (error) *error = hip_error("hipStreamBeginCapture", st);
-        return false;
+hipError_t Engine::launch(bool device_path) {
+    if (!two_lanes()) {
+        hipGraphExec_t g = device_path ? graph_dev_ : graph_host_;
+        return g ? hipGraphLaunch(g, stream_) : enqueue_lane(device_path, -1);
     }
-    const hipError_t queued = enqueue(device_path);
+    // The second lane starts after everything queued on stream_ so far;
+    // stream_ waits for it before anything queued later.
+    hipError_t st = hipEventRecord(ev_fork_, stream_);
+    if (st == hipSuccess) st = hipStreamWaitEvent(stream2_, ev_fork_, 0);
+    for (int l = 0; l < 2 && st == hipSuccess; ++l) {
+        hipGraphExec_t g = device_path ? lane_graph_dev_[l] : lane_graph_host_[l];
+        st = g ? hipGraphLaunch(g, l == 1 ? stream2_ : stream_) : enqueue_lane(device_path, l);
+    }
+    if (st == hipSuccess) st = hipEventRecord(ev_join_, stream2_);
+    if (st == hipSuccess) st = hipStreamWaitEvent(stream_, ev_join_, 0);
+    return st;
+}
+
+bool Engine::capture(bool device_path, int lane, hipGraphExec_t* exec) {
+    *exec = nullptr;
+    hipStream_t s = lane == 1 ? stream2_ : stream_;
+    hipError_t st = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (st != hipSuccess) return false;
+    const hipError_t queued = enqueue_lane(device_path, lane);
     hipGraph_t graph = nullptr;
-    st = hipStreamEndCapture(stream_, &graph);
+    st = hipStreamEndCapture(s, &graph);
     if (queued != hipSuccess || st != hipSuccess || !graph) {
         if (graph) (void)hipGraphDestroy(graph);
-        if (error) *error = hip_error("hipStreamEndCapture", st);
         (void)hipGetLastError();
         return false;
     }
     st = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (st != hipSuccess) {
-        if (error) *error = hip_error("hipGraphInstantiate", st);
         *exec = nullptr;
         return false;
     }
     return true;
+}
+
+void Engine::capture_all() {
+    if (graph_host_) (void)hipGraphExecDestroy(graph_host_);
+    if (graph_dev_) (void)hipGraphExecDestroy(graph_dev_);
+    graph_host_ = graph_dev_ = nullptr;
+    for (int l = 0; l < 2; ++l) {
+        if (lane_graph_host_[l]) (void)hipGraphExecDestroy(lane_graph_host_[l]);
+        if (lane_graph_dev_[l]) (void)hipGraphExecDestroy(lane_graph_dev_[l]);
+        lane_graph_host_[l] = lane_graph_dev_[l] = nullptr;
+    }
+    // Graph capture is an optimisation: a graph that fails to capture leaves
+    // direct launches in its place, as the reference does
+    // (gpu_detector.cu:2051-2056).
+    if (!p_.use_graph) return;
+    if (two_lanes()) {
+        for (int l = 0; l < 2; ++l) {
+            (void)capture(false, l, &lane_graph_host_[l]);
+            (void)capture(true, l, &lane_graph_dev_[l]);
+        }
+    } else {
+        (void)capture(false, -1, &graph_host_);
+        (void)capture(true, -1, &graph_dev_);
+    }
 }
 
 int Engine::detect(const bd_frame* frames, bool device, bd_detection* out, int out_stride,
@@ -356,11 +467,10 @@ int Engine::detect(const bd_frame* frames, bool device, bd_detection* out, int o
         }
     }
     hipError_t st = hipEventRecord(ev_start_, stream_);
-    hipGraphExec_t g = device ? graph_dev_ : graph_host_;
-    if (st == hipSuccess) st = g ? hipGraphLaunch(g, stream_) : enqueue(device);
+    if (st == hipSuccess) st = launch(device);
     if (st == hipSuccess) st = hipEventRecord(ev_stop_, stream_);
     if (st != hipSuccess) {
-        if (error) *error = hip_error(g ? "hipGraphLaunch" : "pipeline launch", st);
+        if (error) *error = hip_error("pipeline launch", st);
         return -1;
     }
     const hipError_t sync = hipEventSynchronize(ev_stop_);
@@ -486,7 +596,7 @@ bool Engine::launch_async(const uint8_t* const* gray, const int32_t* stride, int
             h_desc_dev_[f].stride = stride[f];
         }
     }
-    hipError_t st = graph_dev_ ? hipGraphLaunch(graph_dev_, stream_) : enqueue(true);
+    hipError_t st = launch(true);
     if (st == hipSuccess) st = hipEventRecord(ev_stop_, stream_);
     if (st != hipSuccess) {
         if (error) *error = hip_error("async launch", st);
@@ -569,10 +679,7 @@ bool Engine::replay_locked(int reps, bool device_path, float* ms, std::string* e
         return false;
     }
     hipError_t st = hipEventRecord(ev_start_, stream_);
-    for (int r = 0; r < reps && st == hipSuccess; ++r) {
-        hipGraphExec_t g = device_path ? graph_dev_ : graph_host_;
-        st = g ? hipGraphLaunch(g, stream_) : enqueue(device_path);
-    }
+    for (int r = 0; r < reps && st == hipSuccess; ++r) st = launch(device_path);
     if (st == hipSuccess) st = hipEventRecord(ev_stop_, stream_);
     if (st == hipSuccess) st = hipEventSynchronize(ev_stop_);
     if (st != hipSuccess) {
@@ -585,7 +692,11 @@ bool Engine::replay_locked(int reps, bool device_path, float* ms, std::string* e
     return true;
 }
 
-Tune Engine::tune() const { return plan_host_.tune; }
+Tune Engine::tune() const {
+    Tune t = plan_host_.tune;
+    if (lanes_ < 2) t.streams = 1;
+    return t;
+}
 
 bool Engine::set_tune(const Tune& t, std::string* error) {
     const std::lock_guard<std::mutex> lock(mutex_);
@@ -594,25 +705,23 @@ bool Engine::set_tune(const Tune& t, std::string* error) {
 
 bool Engine::set_tune_locked(const Tune& t, std::string* error) {
     if (t.fit_grid < 64 || t.fit_grid > 16384 || t.decode_grid < 64 || t.decode_grid > 16384 ||
-        t.head_ranks < 0 || t.head_ranks > kHeadRanksMax) {
+        t.head_ranks < 0 || t.head_ranks > kHeadRanksMax || t.streams < 1 || t.streams > 2) {
         if (error) *error = "launch parameters out of range";
         return false;
     }
-    const hipError_t st = hipStreamSynchronize(stream_);
+    hipError_t st = hipStreamSynchronize(stream_);
+    if (st == hipSuccess && stream2_) st = hipStreamSynchronize(stream2_);
     if (st != hipSuccess) {
         if (error) *error = hip_error("hipStreamSynchronize", st);
         return false;
     }
     plan_host_.tune = t;
     plan_dev_.tune = t;
-    if (graph_host_) (void)hipGraphExecDestroy(graph_host_);
-    if (graph_dev_) (void)hipGraphExecDestroy(graph_dev_);
-    graph_host_ = graph_dev_ = nullptr;
-    if (p_.use_graph) {
-        std::string ignored;
-        if (!capture(false, &graph_host_, &ignored)) graph_host_ = nullptr;
-        if (!capture(true, &graph_dev_, &ignored)) graph_dev_ = nullptr;
+    for (int l = 0; l < 2; ++l) {
+        lane_host_[l].tune = t;
+        lane_dev_[l].tune = t;
     }
+    capture_all();
     return true;
 }
 
@@ -654,6 +763,11 @@ bool Engine::autotune(int reps, Tune* best, float* best_ms, std::string* error) 
     for (int32_t h = 0; h <= kHeadRanksMax; ++h) {
         Tune t = cur;
         t.head_ranks = h;
+        if (!consider(t)) return false;
+    }
+    if (lanes_ == 2) {
+        Tune t = cur;
+        t.streams = 3 - cur.streams;
         if (!consider(t)) return false;
     }
     if (!set_tune_locked(cur, error)) return false;

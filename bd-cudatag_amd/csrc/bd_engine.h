@@ -90,10 +90,18 @@ private:
     Engine() = default;
     bool init(const Params& params, const std::vector<int>& widths,
               const std::vector<int>& heights, std::string* error);
-    bool capture(bool device_path, hipGraphExec_t* exec, std::string* error);
+    bool two_lanes() const { return lanes_ == 2 && plan_host_.tune.streams == 2; }
+    // Copies in, kernels, copies out for lane 0 or 1 (on its own stream) or,
+    // with lane = -1, for the whole batch on stream_.
+    hipError_t enqueue_lane(bool device_path, int lane);
+    bool capture(bool device_path, int lane, hipGraphExec_t* exec);
+    // (Re)captures the graphs the current launch parameters use.
+    void capture_all();
+    // One pipeline run on the staged frames, by graph where one was
+    // captured; everything later queued on stream_ runs after it.
+    hipError_t launch(bool device_path);
     bool set_tune_locked(const Tune& t, std::string* error);
     bool replay_locked(int reps, bool device_path, float* ms, std::string* error);
-    hipError_t enqueue(bool device_path);
     void release();
 
     Params p_;
@@ -121,6 +129,19 @@ private:
 
     Plan plan_host_{};
     Plan plan_dev_{};
+    // Two-lane execution: the frames split into two contiguous runs of about
+    // equal pixel count, each with its own plan (prefix tables and candidate
+    // list) and its own graphs (copies in, kernels, copies out of its own
+    // frames), launched on stream_ and stream2_ and joined on stream_.  The
+    // lanes share no buffer.  The whole-batch plans above serve one-lane
+    // launches and the per-kernel timing.
+    int lanes_ = 1;
+    int lane_first_[2] = {0, 0}, lane_count_[2] = {0, 0};
+    Plan lane_host_[2]{};
+    Plan lane_dev_[2]{};
+    hipGraphExec_t lane_graph_host_[2] = {}, lane_graph_dev_[2] = {};
+    hipStream_t stream2_ = nullptr;
+    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr;
     hipEvent_t stage_events_[kStageCount + 1] = {};

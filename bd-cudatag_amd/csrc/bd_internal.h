@@ -109,12 +109,14 @@ enum WorkSlot : int { kWorkCount = 0, kWorkFitCursor = 1, kWorkHeadCount = 3, kW
 constexpr int kHeadRanksMax = 4;
 
 // Launch parameters that change speed only, never results (autotuned per
-// batch shape): persistent grid sizes of k_fit and k_decode and the number
-// of each frame's largest candidates scheduled first.
+// batch shape): persistent grid sizes of k_fit and k_decode, the number of
+// each frame's largest candidates scheduled first, and whether a batch of
+// two or more frames runs as two concurrent lanes (streams = 2) or as one.
 struct Tune {
     int32_t fit_grid = 1280;     // 5 resident 256-thread workgroups per CU
     int32_t decode_grid = 1024;  // 4 resident 512-thread workgroups per CU
     int32_t head_ranks = 2;
+    int32_t streams = 2;
 };
 
 struct Plan {

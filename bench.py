@@ -141,7 +141,7 @@ def uneven_batch(bdcudatag, devmem, scenes, steps):
     return {"frames": len(all_sizes), "mpix_per_step": round(px / 1e6, 2),
             "mpix_s": round(px * steps / t / 1e6, 1), "ms_per_step": round(t / steps * 1e3, 3),
             "device_ms_per_step": round(dev_ms, 3), "device_ms_default_launch": round(default_ms, 3),
-            "tune": {k: tune[k] for k in ("fit_grid", "decode_grid", "head_ranks")}}
+            "tune": {k: tune[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams")}}
 
 
 def precision(bdcudatag, devmem, scenes, frames_total):
@@ -333,7 +333,7 @@ def main():
                              "note": "frames in pageable host memory: copy to pinned staging, H2D, "
                                      "kernels, D2H and per-frame result conversion in Python"},
         "detections_per_step": found,
-        "tune": {k: tune[k] for k in ("fit_grid", "decode_grid", "head_ranks")},
+        "tune": {k: tune[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams")},
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": {
             "bound": "hbm",

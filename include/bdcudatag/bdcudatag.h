@@ -241,7 +241,10 @@ typedef struct bd_tune {
     int32_t fit_grid;    /* persistent quad-fit workgroups, 64..16384 (default 1280) */
     int32_t decode_grid; /* persistent decode workgroups, 64..16384 (default 1024) */
     int32_t head_ranks;  /* each frame's largest candidates scheduled first, 0..4 (default 2) */
-    int32_t reserved;
+    int32_t streams;     /* 2: a batch of two or more frames runs as two concurrent halves of
+                            about equal pixel count, on two streams; 1: as one launch sequence
+                            (default 2; 0 in bd_batch_set_tune selects the default; a one-frame
+                            batch reports 1) */
 } bd_tune;
 
 BD_API int32_t bd_batch_get_tune(bd_batch* batch, bd_tune* out);
