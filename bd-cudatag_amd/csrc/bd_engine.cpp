@@ -317,6 +317,12 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     plan_host_.nframes = nf;
     plan_host_.work = d_work_;
     plan_host_.work_cap = static_cast<int32_t>(work_cap);
+    const auto band_runs_of = [this](int first, int n) {
+        int widest = 0;
+        for (int f = first; f < first + n; ++f) widest = std::max(widest, w_[f]);
+        return band_runs_for_width(widest);
+    };
+    plan_host_.band_runs = band_runs_of(0, nf);
     for (int k = 0; k < SH_COUNT; ++k) plan_host_.blocks[k] = prefix[k * (nf + 1) + nf];
     plan_dev_ = plan_host_;
     plan_dev_.frames = d_desc_dev_;
@@ -329,6 +335,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         p.work = reinterpret_cast<uint32_t*>(base + o_lane_work[l]);
         p.nframes = n;
         p.work_cap = static_cast<int32_t>(lane_cap[l]);
+        p.band_runs = band_runs_of(lane_first[l], n);
         for (int k = 0; k < SH_COUNT; ++k) p.blocks[k] = lane_prefix[l][static_cast<size_t>(k) * (n + 1) + n];
         lane_dev_[l] = p;
         lane_dev_[l].frames = d_desc_dev_ + lane_first[l];

@@ -37,13 +37,21 @@ inline bool frame_span_fits(int width, int height, int stride) {
 // Candidate keys sorted inside one workgroup's LDS before falling back to a
 // radix select over global memory.
 constexpr int kSortCap = 8192;
-// Runs of one 8-row band joined in LDS by k_binarize_runs; a band with more
-// runs is joined row by row in global memory by k_merge instead.  3072 runs
-// (24 KB of band arrays, the least that holds the band's pixel masks and end
-// counts) keeps six workgroups resident per CU and covers noisy 1280-1920 px
-// wide frames (about 930 runs per band on average, 2000 at most, on the
-// eight-tag 1280x800 workload).
-constexpr int kBandRuns = 3072;
+// Runs of one 8-row band joined in LDS by k_binarize_runs (dynamic LDS of
+// Plan::band_runs runs, 8 bytes each); a band with more runs is joined row by
+// row in global memory by k_merge instead.  The capacity follows the widest
+// frame of the plan, band_runs = clamp(1.6 * width rounded up to 512,
+// kBandRunsMin, kBandRunsMax): 3072 runs (24 KB, the least that holds the
+// band's pixel masks and end counts; six workgroups per CU) up to 1920 px,
+// where noisy scenes reach 930 runs per band on average and 2200 at most;
+// 6656 runs at 4096 px (3500 on average, about 1 % of bands above).  The
+// maximum keeps the workgroup within 64 KB of LDS.
+constexpr int kBandRunsMin = 3072;
+constexpr int kBandRunsMax = 7680;
+inline int band_runs_for_width(int width) {
+    const int want = (width * 8 / 5 + 511) / 512 * 512;
+    return want < kBandRunsMin ? kBandRunsMin : want > kBandRunsMax ? kBandRunsMax : want;
+}
 
 // Per-frame scalar block (u32 each).  Accumulators are cleared by the kernel
 // that consumes them, so a replayed graph needs no memset nodes.
@@ -125,6 +133,7 @@ struct Plan {
     uint32_t* work;         // kWorkItems + kHeadRanksMax * nframes + work_cap entries
     int32_t nframes;
     int32_t work_cap;       // sum of max_candidates over frames
+    int32_t band_runs;      // LDS band capacity of k_binarize_runs (runs)
     Tune tune;
     int32_t blocks[SH_COUNT];
 };

@@ -507,8 +507,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 
     __shared__ uint8_t thr[1024];
     __shared__ uint32_t s_off[kTile + 1];
-    __shared__ __align__(16) uint32_t b_run[kBandRuns];
-    __shared__ __align__(16) uint32_t b_lab[kBandRuns];
+    // Band arrays in dynamic LDS: plan.band_runs runs each.
+    extern __shared__ __align__(16) uint32_t band_lds[];
+    const int band_cap = plan.band_runs;
+    uint32_t* b_run = band_lds;
+    uint32_t* b_lab = band_lds + band_cap;
     // Short-lived buffers share storage with the band arrays: the tile
     // staging for the pooled thresholds lives in b_run until the runs are
     // written, the pixel masks live in b_lab until the band union starts.
@@ -518,7 +521,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     // Per row and 32-px half word: runs of the row ending before it (pass B),
     // kept until the band union starts, after the masks in b_lab.
     uint16_t (*end_before)[256] = reinterpret_cast<uint16_t (*)[256]>(b_lab + kTile * 128 * 2);
-    static_assert(kTile * 128 * 8 + kTile * 256 * 2 <= kBandRuns * 4, "masks and end counts fit in b_lab");
+    static_assert(kTile * 128 * 8 + kTile * 256 * 2 <= kBandRunsMin * 4, "masks and end counts fit in b_lab");
 
     // Pooled 3x3 thresholds for this tile row (k_tile_threshold): the three
     // tile rows are staged in LDS (padded with neutral values), then pooled.
@@ -656,7 +659,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     }
     __syncthreads();
     const uint32_t nb = s_off[nrows];
-    const bool local = nb <= static_cast<uint32_t>(kBandRuns);
+    const bool local = nb <= static_cast<uint32_t>(band_cap);
     if (lane == 0 && block_frags) atomicAdd(&d->scal[S_COUNTS + 2], block_frags);
     if (t == 0 && nb) atomicAdd(&d->scal[S_MAXRUNS], nb);
 
@@ -2502,7 +2505,10 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
     switch (stage) {
         case 0: hipLaunchKernelGGL(k_tile_stats, dim3(plan.blocks[SH_TSTAT]), dim3(256), 0, stream, plan); break;
         case 1: hipLaunchKernelGGL(k_frame_ref, dim3(nf), dim3(256), 0, stream, plan); break;
-        case 2: hipLaunchKernelGGL(k_binarize_runs, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
+        case 2:
+            hipLaunchKernelGGL(k_binarize_runs, dim3(plan.blocks[SH_TILE_ROW]), dim3(256),
+                               static_cast<size_t>(plan.band_runs) * 8, stream, plan);
+            break;
         case 3: hipLaunchKernelGGL(k_merge, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
         case 4: hipLaunchKernelGGL(k_stats, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
         case 5: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
