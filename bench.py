@@ -45,8 +45,8 @@ BYTES_PER_PIXEL = 1.0  # algorithmic bytes: the u8 frame read once
 # HBM bytes per input pixel that each kernel moves on the headline workload
 # (64 x 1280x800), from rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by the
 # same counters on a device copy of known size (profiles/traffic_batch64_1280x800.md).
-TRAFFIC_B_PER_PX = {"tile_stats": 1.088, "frame_ref": 0.007, "binarize_runs": 2.416, "merge": 0.895,
-                    "stats": 2.429, "collect": 0.641, "select": 0.031, "fit": 2.189, "decode": 0.397,
+TRAFFIC_B_PER_PX = {"tile_stats": 1.088, "frame_ref": 0.007, "binarize_runs": 2.416, "merge": 0.601,
+                    "stats": 2.277, "collect": 0.641, "select": 0.031, "fit": 3.101, "decode": 0.397,
                     "finalize": 0.010}
 
 
@@ -217,12 +217,16 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--uneven-steps", type=int, default=10)
     ap.add_argument("--precision-frames", type=int, default=256)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch parameters of the headline batch")
+    ap.add_argument("--streams", type=int, default=0, choices=(0, 1, 2),
+                    help="force one launch sequence (1) or two concurrent lanes (2) for the headline batch")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 10 device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames shard "
-              "across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+              "across GPUs with no exchange: run one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import bdcudatag
@@ -248,11 +252,18 @@ def main():
 
     # --- batched path: B frames per step, device-resident inputs -----------
     batch = bdcudatag.Batch(0, [(W, H)] * B)
+    if args.streams:
+        t0_ = batch.get_tune()
+        batch.set_tune(t0_["fit_grid"], t0_["decode_grid"], t0_["head_ranks"], args.streams)
     batch.detect(list(frames))  # stage frames for the per-kernel timing below
     stages = batch.time_stages(reps=5)
     res = batch.detect_device(ptrs)
     found = sum(len(r) for r in res)
-    tune = batch.autotune(reps=5)  # launch parameters for this batch shape
+    # Launch parameters for this batch shape.
+    tune = batch.get_tune() if args.no_autotune else batch.autotune(reps=5)
+    if args.streams:
+        batch.set_tune(tune["fit_grid"], tune["decode_grid"], tune["head_ranks"], args.streams)
+        tune = batch.get_tune()
     batch.prepare_device(ptrs)
     for _ in range(WU):
         batch.detect_raw()
