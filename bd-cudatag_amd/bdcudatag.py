@@ -90,7 +90,8 @@ class ServeReport(ctypes.Structure):
 # Every symbol the public header declares, with (restype, argtypes).
 class Tune(ctypes.Structure):
     _fields_ = [("fit_grid", ctypes.c_int32), ("decode_grid", ctypes.c_int32),
-                ("head_ranks", ctypes.c_int32), ("streams", ctypes.c_int32)]
+                ("head_ranks", ctypes.c_int32), ("streams", ctypes.c_int32),
+                ("decode_waves", ctypes.c_int32)]
 
 
 _P = ctypes.c_void_p
@@ -312,10 +313,11 @@ class Batch:
         if lib().bd_batch_get_tune(self.handle, ctypes.byref(t)) != BD_OK:
             raise RuntimeError(f"bd_batch_get_tune failed: {last_error()}")
         return {"fit_grid": t.fit_grid, "decode_grid": t.decode_grid, "head_ranks": t.head_ranks,
-                "streams": t.streams}
+                "streams": t.streams, "decode_waves": t.decode_waves}
 
-    def set_tune(self, fit_grid: int, decode_grid: int, head_ranks: int, streams: int = 2) -> None:
-        t = Tune(fit_grid, decode_grid, head_ranks, streams)
+    def set_tune(self, fit_grid: int, decode_grid: int, head_ranks: int, streams: int = 2,
+                 decode_waves: int = 8) -> None:
+        t = Tune(fit_grid, decode_grid, head_ranks, streams, decode_waves)
         rc = lib().bd_batch_set_tune(self.handle, ctypes.byref(t))
         if rc != BD_OK:
             raise (ValueError if rc == BD_ERROR_INVALID_ARGUMENT else RuntimeError)(
@@ -339,7 +341,8 @@ class Batch:
                 table = json.load(f)
             if key in table:
                 t = table[key]
-                self.set_tune(t["fit_grid"], t["decode_grid"], t["head_ranks"], t.get("streams", 2))
+                self.set_tune(t["fit_grid"], t["decode_grid"], t["head_ranks"], t.get("streams", 2),
+                              t.get("decode_waves", 8))
                 return dict(t, cached=True)
         best = Tune()
         ms = ctypes.c_float()
@@ -347,7 +350,8 @@ class Batch:
         if rc != BD_OK:
             raise RuntimeError(f"bd_batch_autotune failed: {last_error()}")
         out = {"fit_grid": best.fit_grid, "decode_grid": best.decode_grid, "head_ranks": best.head_ranks,
-               "streams": best.streams, "pipeline_ms": round(float(ms.value), 4)}
+               "streams": best.streams, "decode_waves": best.decode_waves,
+               "pipeline_ms": round(float(ms.value), 4)}
         if cache:
             table[key] = out
             os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)

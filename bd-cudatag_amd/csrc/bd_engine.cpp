@@ -712,7 +712,8 @@ bool Engine::set_tune(const Tune& t, std::string* error) {
 
 bool Engine::set_tune_locked(const Tune& t, std::string* error) {
     if (t.fit_grid < 64 || t.fit_grid > 16384 || t.decode_grid < 64 || t.decode_grid > 16384 ||
-        t.head_ranks < 0 || t.head_ranks > kHeadRanksMax || t.streams < 1 || t.streams > 2) {
+        t.head_ranks < 0 || t.head_ranks > kHeadRanksMax || t.streams < 1 || t.streams > 2 ||
+        (t.decode_waves != 4 && t.decode_waves != 8 && t.decode_waves != 16)) {
         if (error) *error = "launch parameters out of range";
         return false;
     }
@@ -765,6 +766,12 @@ bool Engine::autotune(int reps, Tune* best, float* best_ms, std::string* error) 
     for (const int32_t g : kGrids) {
         Tune t = cur;
         t.decode_grid = g;
+        if (!consider(t)) return false;
+    }
+    for (const int32_t w : {4, 8, 16}) {
+        if (w == cur.decode_waves) continue;
+        Tune t = cur;
+        t.decode_waves = w;
         if (!consider(t)) return false;
     }
     for (int32_t h = 0; h <= kHeadRanksMax; ++h) {

@@ -125,6 +125,7 @@ struct Tune {
     int32_t decode_grid = 1024;  // 4 resident 512-thread workgroups per CU
     int32_t head_ranks = 2;
     int32_t streams = 2;
+    int32_t decode_waves = 8;    // 4, 8 or 16 waves per decode workgroup
 };
 
 struct Plan {

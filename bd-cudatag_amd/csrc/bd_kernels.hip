@@ -2162,29 +2162,24 @@ __device__ inline bool homography_from_unit_quad(const float (*q)[2], float h[9]
     return true;
 }
 
-// One workgroup of 8 waves per candidate: wave v decodes variant v (refined
-// quad for v < 4, hull quad otherwise) at rotation v & 3, one lane per grid
-// cell; the code is matched with one lane per group of codewords.  Lane 0
-// of wave 0 applies the reference's preference (refined first, highest
-// score, first on ties).  reference decode_quad :1291-1430 and
-// k_decode_quads :1438-1537.
-__device__ void decode_candidate(const FrameDesc* d, int ci, int t, const uint64_t* codes, int ncodes) {
-    const int wave = t / kWave, lane = t % kWave;
-    bd_detection* res = d->dec + ci;
-    const float* qsrc = d->quads + 8 * static_cast<size_t>(ci);
-    if (qsrc[0] != qsrc[0]) {
-        if (t == 0) res->id = -1;
-        return;
-    }
-    __shared__ float s_diff[8][64];
-    __shared__ int r_id[8], r_ham[8];
-    __shared__ float r_margin[8];
-    __shared__ float r_corners[8][4][2];
+// Result of one decode variant (reference decode_quad :1291-1430).
+struct VariantResult {
+    int id, ham;
+    float margin;
+    float corners[4][2];
+};
 
+// Variant v of candidate ci on one wave: the refined quad for v < 4, the
+// hull quad otherwise, at rotation v & 3; one lane per grid cell, the code
+// matched with one lane per group of codewords.  `diff` is the wave's own
+// 64-entry LDS row (`reused`: the wave's previous variant read it).  Lane 0
+// writes the result.
+__device__ void decode_variant(const FrameDesc* d, int ci, int v, int lane, float* diff_row, bool reused,
+                               const uint64_t* codes, int ncodes, VariantResult* out) {
     const int dim = d->dim;
     const int cells = dim * dim;
-    const float* base = (wave < 4 ? d->quads : d->hull_quads) + 8 * static_cast<size_t>(ci);
-    const int rot = wave & 3;
+    const float* base = (v < 4 ? d->quads : d->hull_quads) + 8 * static_cast<size_t>(ci);
+    const int rot = v & 3;
     float corners[4][2];
     for (int k = 0; k < 4; ++k) {
         const int src = (rot + k) & 3;
@@ -2198,13 +2193,13 @@ __device__ void decode_candidate(const FrameDesc* d, int ci, int t, const uint64
     bool cell_ok = true;
     if (ok && lane < cells) {
         const float u = (static_cast<float>(cx) + 0.5f) / static_cast<float>(dim);
-        const float v = (static_cast<float>(cy) + 0.5f) / static_cast<float>(dim);
-        const float den = h[6] * u + h[7] * v + 1.0f;
+        const float vv = (static_cast<float>(cy) + 0.5f) / static_cast<float>(dim);
+        const float den = h[6] * u + h[7] * vv + 1.0f;
         if (fabsf(den) < 1e-9f) {
             cell_ok = false;
         } else {
-            const float x = (h[0] * u + h[1] * v + h[2]) / den;
-            const float y = (h[3] * u + h[4] * v + h[5]) / den;
+            const float x = (h[0] * u + h[1] * vv + h[2]) / den;
+            const float y = (h[3] * u + h[4] * vv + h[5]) / den;
             raw = sample_bilinear(frame_bytes(d->gray), d->width, d->height, d->stride, x - 0.0f, y - 0.0f);
             if (raw < 0.0f) cell_ok = false;
         }
@@ -2223,13 +2218,16 @@ __device__ void decode_candidate(const FrameDesc* d, int ci, int t, const uint64
     const uint64_t bad_border = __ballot(border && raw >= thresh);
     ok = ok && bad_border == 0ull;
     const float diff = raw - thresh;
-    s_diff[wave][lane] = diff;
+    if (reused) {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    }
+    diff_row[lane] = diff;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
     float sharp = diff;
     if (lane < cells && cx >= 1 && cx <= dim - 2 && cy >= 1 && cy <= dim - 2) {
-        const float* df = s_diff[wave];
-        const float lap = 4.0f * diff - df[lane - dim] - df[lane + dim] - df[lane - 1] - df[lane + 1];
+        const float lap = 4.0f * diff - diff_row[lane - dim] - diff_row[lane + dim] - diff_row[lane - 1] - diff_row[lane + 1];
         sharp = diff + d->sharpening * lap;
     }
     // Code bits and class sums in the reference's row-major order: every
@@ -2277,63 +2275,90 @@ __device__ void decode_candidate(const FrameDesc* d, int ci, int t, const uint64
                 if (score > -1.0f) id = best_i;
             }
         }
-        r_id[wave] = id;
-        r_ham[wave] = best_d;
-        r_margin[wave] = margin;
+        out->id = id;
+        out->ham = best_d;
+        out->margin = margin;
         for (int k = 0; k < 4; ++k) {
-            r_corners[wave][k][0] = corners[k][0];
-            r_corners[wave][k][1] = corners[k][1];
-        }
-    }
-    __syncthreads();
-    if (t == 0) {
-        int best = -1;
-        float best_score = -1.0f;
-        for (int variant = 0; variant < 2 && best < 0; ++variant) {
-            for (int rotation = 0; rotation < 4; ++rotation) {
-                const int r = variant * 4 + rotation;
-                if (r_id[r] < 0) continue;
-                const float score = static_cast<float>(4 - r_ham[r]) * 1000.0f + r_margin[r];
-                if (score > best_score) {
-                    best_score = score;
-                    best = r;
-                }
-            }
-        }
-        if (best < 0) {
-            res->id = -1;
-        } else {
-            res->id = r_id[best];
-            res->hamming = r_ham[best];
-            res->decision_margin = r_margin[best];
-            for (int k = 0; k < 4; ++k) {
-                res->corners[2 * k] = r_corners[best][k][0];
-                res->corners[2 * k + 1] = r_corners[best][k][1];
-            }
-            res->reserved = 0;
-            atomicAdd(&d->scal[S_DECODED], 1u);
+            out->corners[k][0] = corners[k][0];
+            out->corners[k][1] = corners[k][1];
         }
     }
 }
 
-// Decoding costs about the same per candidate, so workgroups take items by
-// a fixed stride (no cursor).  The codebook (shared by every frame of the
-// batch) is copied to LDS once per persistent workgroup.
-__global__ __launch_bounds__(512) void k_decode(Plan plan) {
+// The reference's preference among the 8 variants (refined first, highest
+// score, first on ties; k_decode_quads :1438-1537), written to the
+// candidate's decode record.
+__device__ void decode_choose(const FrameDesc* d, int ci, bool have_quad, const VariantResult* r) {
+    bd_detection* res = d->dec + ci;
+    int best = -1;
+    float best_score = -1.0f;
+    for (int variant = 0; variant < 2 && best < 0 && have_quad; ++variant) {
+        for (int rotation = 0; rotation < 4; ++rotation) {
+            const int v = variant * 4 + rotation;
+            if (r[v].id < 0) continue;
+            const float score = static_cast<float>(4 - r[v].ham) * 1000.0f + r[v].margin;
+            if (score > best_score) {
+                best_score = score;
+                best = v;
+            }
+        }
+    }
+    if (best < 0) {
+        res->id = -1;
+    } else {
+        res->id = r[best].id;
+        res->hamming = r[best].ham;
+        res->decision_margin = r[best].margin;
+        for (int k = 0; k < 4; ++k) {
+            res->corners[2 * k] = r[best].corners[k][0];
+            res->corners[2 * k + 1] = r[best].corners[k][1];
+        }
+        res->reserved = 0;
+        atomicAdd(&d->scal[S_DECODED], 1u);
+    }
+}
+
+// Persistent decode workgroups of kWaves waves over the work list (the
+// decode_waves launch parameter): 4 waves decode one candidate two variants
+// per wave, 8 waves one candidate one variant per wave, 16 waves two
+// candidates at once.  Decoding costs about the same per candidate, so
+// workgroups take items by a fixed stride (no cursor).  The codebook (shared
+// by every frame of the batch) is copied to LDS once per workgroup.
+template <int kWaves>
+__global__ __launch_bounds__(kWaves * 64) void k_decode(Plan plan) {
+    constexpr int kPerBlock = kWaves >= 16 ? kWaves / 8 : 1;   // candidates per workgroup
+    constexpr int kWavesPerCand = kWaves / kPerBlock;           // 4 or 8
+    constexpr int kVariantsPerWave = 8 / kWavesPerCand;         // 2 or 1
     const uint32_t nhead = plan.work[kWorkHeadCount];
     const uint32_t total = nhead + plan.work[kWorkCount];
-    if (blockIdx.x >= total) return;
+    if (blockIdx.x * kPerBlock >= total) return;
     __shared__ uint64_t s_codes[kMaxCodes];
+    __shared__ float s_diff[kWaves][kWave];
+    __shared__ VariantResult s_res[kPerBlock][8];
     const int ncodes = min(plan.frames[0].code_count, kMaxCodes);
     for (int i = threadIdx.x; i < ncodes; i += blockDim.x) s_codes[i] = plan.frames[0].codes[i];
     __syncthreads();
-    for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
-        // Uniform across the workgroup: scalar registers for the descriptor.
-        const uint32_t code = __builtin_amdgcn_readfirstlane(work_item(plan, item, nhead));
-        int t = threadIdx.x;
-        asm volatile("" : "+v"(t));
-        decode_candidate(plan.frames + (code >> 10), static_cast<int>(code & 1023u), t, s_codes, ncodes);
-        __syncthreads();  // LDS of this item is no longer read
+    const int wave = static_cast<int>(threadIdx.x) / kWave, lane = static_cast<int>(threadIdx.x) % kWave;
+    const int slot = wave / kWavesPerCand, wslot = wave % kWavesPerCand;
+    for (uint32_t first = blockIdx.x * kPerBlock; first < total; first += gridDim.x * kPerBlock) {
+        const uint32_t item = first + static_cast<uint32_t>(slot);
+        const bool have = item < total;
+        // Uniform across the wave: scalar registers for the descriptor.
+        const uint32_t code = have ? __builtin_amdgcn_readfirstlane(work_item(plan, item, nhead)) : 0u;
+        const FrameDesc* d = plan.frames + (code >> 10);
+        const int ci = static_cast<int>(code & 1023u);
+        // A candidate without a quad decodes to nothing.
+        const bool have_quad = have && d->quads[8 * static_cast<size_t>(ci)] == d->quads[8 * static_cast<size_t>(ci)];
+        if (have_quad) {
+#pragma unroll
+            for (int k = 0; k < kVariantsPerWave; ++k) {
+                const int v = wslot + k * kWavesPerCand;
+                decode_variant(d, ci, v, lane, s_diff[wave], k > 0, s_codes, ncodes, &s_res[slot][v]);
+            }
+        }
+        __syncthreads();
+        if (have && wslot == 0 && lane == 0) decode_choose(d, ci, have_quad, s_res[slot]);
+        __syncthreads();  // s_res of this item is no longer read
     }
 }
 
@@ -2514,7 +2539,15 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
         case 5: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
         case 6: hipLaunchKernelGGL(k_select, dim3(nf), dim3(1024), 0, stream, plan); break;
         case 7: hipLaunchKernelGGL(k_fit, dim3(min(plan.work_cap, plan.tune.fit_grid)), dim3(256), 0, stream, plan); break;
-        case 8: hipLaunchKernelGGL(k_decode, dim3(min(plan.work_cap, plan.tune.decode_grid)), dim3(512), 0, stream, plan); break;
+        case 8: {
+            const int w = plan.tune.decode_waves;
+            const int per = w >= 16 ? w / 8 : 1;
+            const dim3 grid(min((plan.work_cap + per - 1) / per, plan.tune.decode_grid));
+            if (w == 4) hipLaunchKernelGGL(k_decode<4>, grid, dim3(256), 0, stream, plan);
+            else if (w == 16) hipLaunchKernelGGL(k_decode<16>, grid, dim3(1024), 0, stream, plan);
+            else hipLaunchKernelGGL(k_decode<8>, grid, dim3(512), 0, stream, plan);
+            break;
+        }
         case 9: hipLaunchKernelGGL(k_finalize, dim3(nf), dim3(256), 0, stream, plan); break;
         default: return hipErrorInvalidValue;
     }

@@ -245,6 +245,9 @@ typedef struct bd_tune {
                             about equal pixel count, on two streams; 1: as one launch sequence
                             (default 2; 0 in bd_batch_set_tune selects the default; a one-frame
                             batch reports 1) */
+    int32_t decode_waves; /* waves per decode workgroup: 4 (two variants per wave), 8 (one variant
+                             per wave) or 16 (two candidates per workgroup); default 8, 0 in
+                             bd_batch_set_tune selects the default */
 } bd_tune;
 
 BD_API int32_t bd_batch_get_tune(bd_batch* batch, bd_tune* out);

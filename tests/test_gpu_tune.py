@@ -24,24 +24,27 @@ def test_launch_parameters_keep_results(tmp_path):
     sizes = [(1280, 800)] * 4 + [(640, 480)] * 2 + [(1920, 1200)]
     imgs = [scenes.scene(2, w, h, 40 + i)[0] for i, (w, h) in enumerate(sizes)]
     batch = bdcudatag.Batch(0, sizes)
-    assert batch.get_tune() == {"fit_grid": 1280, "decode_grid": 1024, "head_ranks": 2, "streams": 2}
+    assert batch.get_tune() == {"fit_grid": 1280, "decode_grid": 1024, "head_ranks": 2, "streams": 2,
+                                "decode_waves": 8}
     ref = batch.detect(imgs)
     assert sum(len(r) for r in ref) > 0
-    for fg, dg, hr, ns in [(64, 64, 0, 1), (256, 2048, 4, 2), (4096, 512, 1, 1), (1280, 1024, 2, 2)]:
-        batch.set_tune(fg, dg, hr, ns)
-        assert batch.get_tune() == {"fit_grid": fg, "decode_grid": dg, "head_ranks": hr, "streams": ns}
+    for fg, dg, hr, ns, dw in [(64, 64, 0, 1, 4), (256, 2048, 4, 2, 16), (4096, 512, 1, 1, 8),
+                               (1280, 1024, 2, 2, 4), (1280, 100, 2, 1, 16)]:
+        batch.set_tune(fg, dg, hr, ns, dw)
+        assert batch.get_tune() == {"fit_grid": fg, "decode_grid": dg, "head_ranks": hr, "streams": ns,
+                                    "decode_waves": dw}
         same(batch.detect(imgs), ref)
-    for bad in [(10, 1024, 2), (1024, 1 << 20, 2), (1024, 1024, 5), (1024, 1024, 2, 3)]:
+    for bad in [(10, 1024, 2), (1024, 1 << 20, 2), (1024, 1024, 5), (1024, 1024, 2, 3), (1024, 1024, 2, 2, 12)]:
         with pytest.raises(ValueError):
             batch.set_tune(*bad)
     cache = str(tmp_path / "tune.json")
     first = batch.autotune(reps=2, cache=cache)
     assert not first["cached"] and first["pipeline_ms"] > 0
-    assert batch.get_tune() == {k: first[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams")}
+    assert batch.get_tune() == {k: first[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams", "decode_waves")}
     same(batch.detect(imgs), ref)
     again = batch.autotune(reps=2, cache=cache)
-    assert again["cached"] and {k: again[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams")} == \
-        {k: first[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams")}
+    assert again["cached"] and {k: again[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams", "decode_waves")} == \
+        {k: first[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams", "decode_waves")}
 
 
 def test_one_frame_batch_runs_one_lane():
