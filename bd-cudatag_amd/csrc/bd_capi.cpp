@@ -375,7 +375,7 @@ int32_t bd_batch_get_tune(bd_batch* batch, bd_tune* out) {
         return BD_ERROR_INVALID_ARGUMENT;
     }
     const bd::Tune t = batch->engine->tune();
-    *out = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, t.streams, t.decode_waves};
+    *out = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, t.streams, t.decode_waves, t.chunk_px};
     return BD_OK;
 }
 
@@ -390,6 +390,7 @@ int32_t bd_batch_set_tune(bd_batch* batch, const bd_tune* tune) {
     t.head_ranks = tune->head_ranks;
     t.streams = tune->streams == 0 ? bd::Tune{}.streams : tune->streams;
     t.decode_waves = tune->decode_waves == 0 ? bd::Tune{}.decode_waves : tune->decode_waves;
+    t.chunk_px = tune->chunk_px == 0 ? bd::Tune{}.chunk_px : tune->chunk_px;
     std::string error;
     if (!batch->engine->set_tune(t, &error)) {
         set_error(error);
@@ -410,7 +411,7 @@ int32_t bd_batch_autotune(bd_batch* batch, int32_t reps, bd_tune* best, float* b
         set_error(error);
         return BD_ERROR_CUDA;
     }
-    if (best) *best = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, t.streams, t.decode_waves};
+    if (best) *best = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, t.streams, t.decode_waves, t.chunk_px};
     if (best_ms) *best_ms = ms;
     return BD_OK;
 }

@@ -713,7 +713,8 @@ bool Engine::set_tune(const Tune& t, std::string* error) {
 bool Engine::set_tune_locked(const Tune& t, std::string* error) {
     if (t.fit_grid < 64 || t.fit_grid > 16384 || t.decode_grid < 64 || t.decode_grid > 16384 ||
         t.head_ranks < 0 || t.head_ranks > kHeadRanksMax || t.streams < 1 || t.streams > 2 ||
-        (t.decode_waves != 4 && t.decode_waves != 8 && t.decode_waves != 16)) {
+        (t.decode_waves != 4 && t.decode_waves != 8 && t.decode_waves != 16) ||
+        (t.chunk_px != 64 && t.chunk_px != 128)) {
         if (error) *error = "launch parameters out of range";
         return false;
     }
@@ -772,6 +773,11 @@ bool Engine::autotune(int reps, Tune* best, float* best_ms, std::string* error) 
         if (w == cur.decode_waves) continue;
         Tune t = cur;
         t.decode_waves = w;
+        if (!consider(t)) return false;
+    }
+    {
+        Tune t = cur;
+        t.chunk_px = cur.chunk_px == 64 ? 128 : 64;
         if (!consider(t)) return false;
     }
     for (int32_t h = 0; h <= kHeadRanksMax; ++h) {

@@ -126,6 +126,7 @@ struct Tune {
     int32_t head_ranks = 2;
     int32_t streams = 2;
     int32_t decode_waves = 8;    // 4, 8 or 16 waves per decode workgroup
+    int32_t chunk_px = 64;       // pixels packed per k_binarize_runs item: 64 or 128
 };
 
 struct Plan {

@@ -493,6 +493,7 @@ __device__ inline void lds_union(uint32_t* lab, uint32_t a, uint32_t b) {
 //     word and only they get component statistics initialised.  A band with
 //     more runs than the LDS buffer leaves every run a root and lets k_merge
 //     join all of its rows.
+template <int kChunkWords>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_binarize_runs(Plan plan) {
     int ty;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
@@ -567,47 +568,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 
     const uint8_t* gray = d->gray;
     const bool vec = ((reinterpret_cast<uintptr_t>(gray) | static_cast<uintptr_t>(stride)) & 15u) == 0;
-    const int items = nrows * w64;
+    // kChunkWords consecutive 64-px words of one row per item (the chunk
+    // width launch parameter: 64 or 128 px).
+    const int wchunks = (w64 + kChunkWords - 1) / kChunkWords;
+    const int items = nrows * wchunks;
     for (int it = t; it < items; it += blockDim.x) {
-        const int r = it / w64;
-        const int w = it - r * w64;
+        const int r = it / wchunks;
         const int y = y0 + r;
-        uint64_t m = ~0ull;
-        for (int g = 0; g < 2; ++g) {
-            const int grp = 2 * w + g;
-            const int xs = grp * 32;
-            if (xs >= W) continue;
-            const int tile0 = grp * 4;
-            if (!(tile0 <= re && tile0 + 3 >= rs)) continue;  // 32-px ROI test (:376-377)
-            const uint8_t* src = gray + static_cast<size_t>(y) * stride + xs;
-            uint32_t m32 = 0;
-            if (vec && xs + 32 <= W) {
-                for (int q = 0; q < 2; ++q) {
-                    const uint4 v = *reinterpret_cast<const uint4*>(src + 16 * q);
-                    const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
-                    for (int j = 0; j < 4; ++j) {
-                        const uint32_t tv = thr[(xs >> 3) + 2 * q + (j >> 1)];
-                        for (int b = 0; b < 4; ++b) {
-                            const uint32_t px = (wd[j] >> (8 * b)) & 0xffu;
-                            m32 |= static_cast<uint32_t>(px > tv) << (16 * q + 4 * j + b);
+#pragma unroll
+        for (int cw = 0; cw < kChunkWords; ++cw) {
+            const int w = (it - r * wchunks) * kChunkWords + cw;
+            if (w >= w64) break;
+            uint64_t m = ~0ull;
+            for (int g = 0; g < 2; ++g) {
+                const int grp = 2 * w + g;
+                const int xs = grp * 32;
+                if (xs >= W) continue;
+                const int tile0 = grp * 4;
+                if (!(tile0 <= re && tile0 + 3 >= rs)) continue;  // 32-px ROI test (:376-377)
+                const uint8_t* src = gray + static_cast<size_t>(y) * stride + xs;
+                uint32_t m32 = 0;
+                if (vec && xs + 32 <= W) {
+                    for (int q = 0; q < 2; ++q) {
+                        const uint4 v = *reinterpret_cast<const uint4*>(src + 16 * q);
+                        const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+                        for (int j = 0; j < 4; ++j) {
+                            const uint32_t tv = thr[(xs >> 3) + 2 * q + (j >> 1)];
+                            for (int b = 0; b < 4; ++b) {
+                                const uint32_t px = (wd[j] >> (8 * b)) & 0xffu;
+                                m32 |= static_cast<uint32_t>(px > tv) << (16 * q + 4 * j + b);
+                            }
                         }
                     }
+                } else {
+                    // Frame edge or unaligned frame: one pixel at a time (kept
+                    // rolled so the unrolled form's registers do not spill).
+    #pragma unroll 1
+                    for (int k = 0; k < 32; ++k) {
+                        const int x = xs + k;
+                        uint32_t white = 1;
+                        if (x < W) white = src[k] > thr[x >> 3] ? 1u : 0u;
+                        m32 |= white << k;
+                    }
                 }
-            } else {
-                // Frame edge or unaligned frame: one pixel at a time (kept
-                // rolled so the unrolled form's registers do not spill).
-#pragma unroll 1
-                for (int k = 0; k < 32; ++k) {
-                    const int x = xs + k;
-                    uint32_t white = 1;
-                    if (x < W) white = src[k] > thr[x >> 3] ? 1u : 0u;
-                    m32 |= white << k;
-                }
+                m = (m & ~(0xffffffffull << (32 * g))) | (static_cast<uint64_t>(m32) << (32 * g));
             }
-            m = (m & ~(0xffffffffull << (32 * g))) | (static_cast<uint64_t>(m32) << (32 * g));
+            masks[r][w] = m;
+            d->bits[static_cast<size_t>(y) * w64 + w] = m;
         }
-        masks[r][w] = m;
-        d->bits[static_cast<size_t>(y) * w64 + w] = m;
     }
     __syncthreads();
 
@@ -2531,8 +2539,12 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
         case 0: hipLaunchKernelGGL(k_tile_stats, dim3(plan.blocks[SH_TSTAT]), dim3(256), 0, stream, plan); break;
         case 1: hipLaunchKernelGGL(k_frame_ref, dim3(nf), dim3(256), 0, stream, plan); break;
         case 2:
-            hipLaunchKernelGGL(k_binarize_runs, dim3(plan.blocks[SH_TILE_ROW]), dim3(256),
-                               static_cast<size_t>(plan.band_runs) * 8, stream, plan);
+            if (plan.tune.chunk_px == 128)
+                hipLaunchKernelGGL(k_binarize_runs<2>, dim3(plan.blocks[SH_TILE_ROW]), dim3(256),
+                                   static_cast<size_t>(plan.band_runs) * 8, stream, plan);
+            else
+                hipLaunchKernelGGL(k_binarize_runs<1>, dim3(plan.blocks[SH_TILE_ROW]), dim3(256),
+                                   static_cast<size_t>(plan.band_runs) * 8, stream, plan);
             break;
         case 3: hipLaunchKernelGGL(k_merge, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
         case 4: hipLaunchKernelGGL(k_stats, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;

@@ -248,6 +248,8 @@ typedef struct bd_tune {
     int32_t decode_waves; /* waves per decode workgroup: 4 (two variants per wave), 8 (one variant
                              per wave) or 16 (two candidates per workgroup); default 8, 0 in
                              bd_batch_set_tune selects the default */
+    int32_t chunk_px;     /* pixels a binarization thread packs per item: 64 or 128 (default 64;
+                             0 in bd_batch_set_tune selects the default) */
 } bd_tune;
 
 BD_API int32_t bd_batch_get_tune(bd_batch* batch, bd_tune* out);

@@ -25,26 +25,27 @@ def test_launch_parameters_keep_results(tmp_path):
     imgs = [scenes.scene(2, w, h, 40 + i)[0] for i, (w, h) in enumerate(sizes)]
     batch = bdcudatag.Batch(0, sizes)
     assert batch.get_tune() == {"fit_grid": 1280, "decode_grid": 1024, "head_ranks": 2, "streams": 2,
-                                "decode_waves": 8}
+                                "decode_waves": 8, "chunk_px": 64}
     ref = batch.detect(imgs)
     assert sum(len(r) for r in ref) > 0
-    for fg, dg, hr, ns, dw in [(64, 64, 0, 1, 4), (256, 2048, 4, 2, 16), (4096, 512, 1, 1, 8),
-                               (1280, 1024, 2, 2, 4), (1280, 100, 2, 1, 16)]:
-        batch.set_tune(fg, dg, hr, ns, dw)
+    for fg, dg, hr, ns, dw, cp in [(64, 64, 0, 1, 4, 128), (256, 2048, 4, 2, 16, 64), (4096, 512, 1, 1, 8, 128),
+                                   (1280, 1024, 2, 2, 4, 64), (1280, 100, 2, 1, 16, 128)]:
+        batch.set_tune(fg, dg, hr, ns, dw, cp)
         assert batch.get_tune() == {"fit_grid": fg, "decode_grid": dg, "head_ranks": hr, "streams": ns,
-                                    "decode_waves": dw}
+                                    "decode_waves": dw, "chunk_px": cp}
         same(batch.detect(imgs), ref)
-    for bad in [(10, 1024, 2), (1024, 1 << 20, 2), (1024, 1024, 5), (1024, 1024, 2, 3), (1024, 1024, 2, 2, 12)]:
+    for bad in [(10, 1024, 2), (1024, 1 << 20, 2), (1024, 1024, 5), (1024, 1024, 2, 3), (1024, 1024, 2, 2, 12),
+                (1024, 1024, 2, 2, 8, 96)]:
         with pytest.raises(ValueError):
             batch.set_tune(*bad)
     cache = str(tmp_path / "tune.json")
     first = batch.autotune(reps=2, cache=cache)
     assert not first["cached"] and first["pipeline_ms"] > 0
-    assert batch.get_tune() == {k: first[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams", "decode_waves")}
+    assert batch.get_tune() == {k: first[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams", "decode_waves", "chunk_px")}
     same(batch.detect(imgs), ref)
     again = batch.autotune(reps=2, cache=cache)
-    assert again["cached"] and {k: again[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams", "decode_waves")} == \
-        {k: first[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams", "decode_waves")}
+    assert again["cached"] and {k: again[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams", "decode_waves", "chunk_px")} == \
+        {k: first[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams", "decode_waves", "chunk_px")}
 
 
 def test_one_frame_batch_runs_one_lane():
