@@ -35,8 +35,11 @@ inline bool frame_span_fits(int width, int height, int stride) {
 }
 
 // Candidate keys sorted inside one workgroup's LDS before falling back to a
-// radix select over global memory.
-constexpr int kSortCap = 8192;
+// radix select over global memory (at most max_candidates <= 1024 keys are
+// kept, so the select always leaves them within the cap).  16 KB keeps
+// k_select small enough to start on CUs that the other lane's persistent
+// k_fit workgroups hold.
+constexpr int kSortCap = 2048;
 // Runs of one 8-row band joined in LDS by k_binarize_runs (dynamic LDS of
 // Plan::band_runs runs, 8 bytes each); a band with more runs is joined row by
 // row in global memory by k_merge instead.  The capacity follows the widest

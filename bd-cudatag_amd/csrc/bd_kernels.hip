@@ -1279,7 +1279,7 @@ __device__ inline void bitonic_sort_lds(uint64_t* keys, int n) {
 // max_candidates (reference k_order_candidates :642-654, with the in-bucket
 // order fixed to (area desc, root slot asc)).  Lists longer than the LDS
 // buffer first find the key of rank max_candidates by an 8-bit radix select.
-__global__ __launch_bounds__(1024) void k_select(Plan plan) {
+__global__ __launch_bounds__(256) void k_select(Plan plan) {
     const FrameDesc* d = plan.frames + blockIdx.x;
     if (!d->active) return;
     __shared__ uint64_t keys[kSortCap];
@@ -2549,7 +2549,7 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
         case 3: hipLaunchKernelGGL(k_merge, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
         case 4: hipLaunchKernelGGL(k_stats, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
         case 5: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
-        case 6: hipLaunchKernelGGL(k_select, dim3(nf), dim3(1024), 0, stream, plan); break;
+        case 6: hipLaunchKernelGGL(k_select, dim3(nf), dim3(256), 0, stream, plan); break;
         case 7: hipLaunchKernelGGL(k_fit, dim3(min(plan.work_cap, plan.tune.fit_grid)), dim3(256), 0, stream, plan); break;
         case 8: {
             const int w = plan.tune.decode_waves;

@@ -130,6 +130,22 @@ def test_candidate_cap_small():
     assert not bad, bad
 
 
+def test_candidate_radix_select():
+    # 5035 tag-shaped rings (12x12, 3-px border, fill 0.75): more candidates
+    # than the LDS sort holds (2048) and than the 768 kept at 1920x1080, so
+    # k_select takes its radix-select path and the cap decides the set.
+    W, H = 1920, 1080
+    img = np.full((H, W), 200, np.uint8)
+    for y in range(10, H - 20, 20):
+        for x in range(10, W - 20, 20):
+            img[y:y + 12, x:x + 12] = 30
+            img[y + 3:y + 9, x + 3:x + 9] = 200
+    det, dets, g, o = run_both(img, 0)
+    assert int(o["counts"][0]) > 2048
+    bad = compare_frame(g, o, dets, det.stats())
+    assert not bad, bad
+
+
 def test_strided_host_frame():
     img, tags = scenes.scene(0, 1280, 800, 9)
     padded = np.zeros((800, 1344), dtype=np.uint8)
