@@ -1383,6 +1383,29 @@ typedef const uint8_t __attribute__((address_space(1))) gbyte;
 
 __device__ inline const gbyte* frame_bytes(const uint8_t* gray) { return (const gbyte*)gray; }
 
+typedef const uint64_t __attribute__((address_space(1))) gword64;
+
+// Bytes of a frame from its base to the last pixel, (height - 1) * stride +
+// width.
+__device__ inline uint32_t frame_span(int width, int height, int stride) {
+    return static_cast<uint32_t>(height - 1) * static_cast<uint32_t>(stride) + static_cast<uint32_t>(width);
+}
+
+// Pixels o and o + 1 of a frame (low byte first): one 8-byte load from the
+// 4-byte-aligned address at or below pixel o (one request for the pair
+// instead of two), or two byte loads where that load would start before the
+// frame's first byte or end past its last (`span`).
+__device__ inline uint32_t pixel_pair(const gbyte* image, uint32_t o, uint32_t span) {
+    const uint32_t mis = static_cast<uint32_t>((uintptr_t)image) & 3u;
+    const uint32_t a = (o + mis) & ~3u;  // aligned position, counted from base - mis
+    if (a >= mis && a - mis + 8u <= span) {
+        const uint64_t w = *reinterpret_cast<const gword64*>(image + (a - mis));
+        return __builtin_amdgcn_alignbyte(static_cast<uint32_t>(w >> 32), static_cast<uint32_t>(w), (o + mis) & 3u) &
+               0xffffu;
+    }
+    return static_cast<uint32_t>(image[o]) | (static_cast<uint32_t>(image[o + 1]) << 8);
+}
+
 __device__ inline float sample_bilinear(const gbyte* image, int width, int height, int stride,
                                         float x, float y) {
     if (!(x >= 0.0f && y >= 0.0f && x < static_cast<float>(width - 1) &&
@@ -1394,7 +1417,10 @@ __device__ inline float sample_bilinear(const gbyte* image, int width, int heigh
     const float fx = x - static_cast<float>(ix);
     const float fy = y - static_cast<float>(iy);
     const uint32_t o = static_cast<uint32_t>(iy) * static_cast<uint32_t>(stride) + static_cast<uint32_t>(ix);
-    const float a = image[o], b = image[o + 1], c = image[o + stride], e = image[o + stride + 1];
+    const uint32_t span = frame_span(width, height, stride);
+    const uint32_t top = pixel_pair(image, o, span), bottom = pixel_pair(image, o + stride, span);
+    const float a = static_cast<float>(top & 0xffu), b = static_cast<float>(top >> 8);
+    const float c = static_cast<float>(bottom & 0xffu), e = static_cast<float>(bottom >> 8);
     return (a * (1.0f - fx) + b * fx) * (1.0f - fy) + (c * (1.0f - fx) + e * fx) * fy;
 }
 
@@ -1920,6 +1946,7 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
     if (d->refine) {
         const gbyte* gray = frame_bytes(d->gray);
         const int W = d->width, H = d->height, stride = d->stride;
+        const uint32_t span = frame_span(W, H, stride);
         for (int iteration = 0; iteration < kFitIterations; ++iteration) {
             const bool first_pass = iteration == 0;
             const EdgeGeom g = edge_geometry(fit_quad, wave, d->dim, first_pass);
@@ -1949,18 +1976,28 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
                         ta[u].ok = ta[u].ok && live;
                         tb[u].ok = tb[u].ok && live;
                     }
-                    float va[3][4], vb[3][4];
+                    // Each tap's 2x2 neighbourhood as two horizontal pixel
+                    // pairs (one request each).
+                    uint32_t pa[3][2], pb[3][2];
 #pragma unroll
                     for (int u = 0; u < 3; ++u) {
                         const uint32_t p = ta[u].off, q = tb[u].off;
-                        va[u][0] = ta[u].ok ? static_cast<float>(gray[p]) : 0.0f;
-                        va[u][1] = ta[u].ok ? static_cast<float>(gray[p + 1]) : 0.0f;
-                        va[u][2] = ta[u].ok ? static_cast<float>(gray[p + stride]) : 0.0f;
-                        va[u][3] = ta[u].ok ? static_cast<float>(gray[p + stride + 1]) : 0.0f;
-                        vb[u][0] = tb[u].ok ? static_cast<float>(gray[q]) : 0.0f;
-                        vb[u][1] = tb[u].ok ? static_cast<float>(gray[q + 1]) : 0.0f;
-                        vb[u][2] = tb[u].ok ? static_cast<float>(gray[q + stride]) : 0.0f;
-                        vb[u][3] = tb[u].ok ? static_cast<float>(gray[q + stride + 1]) : 0.0f;
+                        pa[u][0] = ta[u].ok ? pixel_pair(gray, p, span) : 0u;
+                        pa[u][1] = ta[u].ok ? pixel_pair(gray, p + stride, span) : 0u;
+                        pb[u][0] = tb[u].ok ? pixel_pair(gray, q, span) : 0u;
+                        pb[u][1] = tb[u].ok ? pixel_pair(gray, q + stride, span) : 0u;
+                    }
+                    float va[3][4], vb[3][4];
+#pragma unroll
+                    for (int u = 0; u < 3; ++u) {
+                        va[u][0] = static_cast<float>(pa[u][0] & 0xffu);
+                        va[u][1] = static_cast<float>(pa[u][0] >> 8);
+                        va[u][2] = static_cast<float>(pa[u][1] & 0xffu);
+                        va[u][3] = static_cast<float>(pa[u][1] >> 8);
+                        vb[u][0] = static_cast<float>(pb[u][0] & 0xffu);
+                        vb[u][1] = static_cast<float>(pb[u][0] >> 8);
+                        vb[u][2] = static_cast<float>(pb[u][1] & 0xffu);
+                        vb[u][3] = static_cast<float>(pb[u][1] >> 8);
                     }
 #pragma unroll
                     for (int u = 0; u < 3; ++u) {

@@ -180,6 +180,30 @@ def test_device_pointer_path():
         det.detect_device(buf.ptr, 9_000_000)
 
 
+def test_misaligned_device_frames():
+    # Device frames at 1-3 bytes past a 4-byte boundary with an odd row
+    # stride: the pixel-pair loads fold the misalignment in, and the last
+    # bytes of the frame are read one at a time.  Results equal the host
+    # path's bit for bit.
+    import devmem
+    img, tags = scenes.scene(0, 1280, 800, 5)
+    det = bdcudatag.Detector(0, 1280, 800)
+    ref = det.detect(img)
+    assert len(ref) >= 6
+    stride = 1283
+    for off in (1, 2, 3):
+        raw = np.full(off + 800 * stride, 111, dtype=np.uint8)
+        view = raw[off:].reshape(-1)[:800 * stride].reshape(800, stride)
+        view[:, :1280] = img
+        buf = devmem.DeviceBuffer(raw)
+        got = det.detect_device(buf.ptr + off, stride)
+        assert [d["id"] for d in got] == [d["id"] for d in ref]
+        for p, q in zip(got, ref):
+            assert np.array_equal(p["corners"], q["corners"])
+            assert np.array_equal(p["homography"], q["homography"])
+            assert np.float32(p["decision_margin"]) == np.float32(q["decision_margin"])
+
+
 def test_stats_and_thread_safety():
     # detector/tests/test_detector.cpp:316-340: 4 threads x 5 frames, one handle
     import threading
