@@ -105,7 +105,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     // blocks (contiguous so one copy returns them for all frames).
     struct Off {
         size_t tmin, tmax, tmean, tthr, trlo, trhi, roi, hist, bits, rowc, rdir, runs, label, area, bbox,
-            keys, cands, quads, hquads, dec;
+            keys, roots, nroots, cands, quads, hquads, dec;
     };
     std::vector<Off> off(nf);
     std::vector<int> maxc(nf);
@@ -143,6 +143,8 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         o.area = take(slots * 4);
         o.bbox = take(slots * 16);
         o.keys = take(slots * 8);
+        o.roots = take(slots * 4);
+        o.nroots = take(static_cast<size_t>(th) * 4);
         o.cands = take(static_cast<size_t>(maxc[f]) * 4);
         o.quads = take(static_cast<size_t>(maxc[f]) * 32);
         o.hquads = take(static_cast<size_t>(maxc[f]) * 32);
@@ -259,13 +261,14 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         d.area = reinterpret_cast<uint32_t*>(base + o.area);
         d.bbox = reinterpret_cast<uint32_t*>(base + o.bbox);
         d.cand_keys = reinterpret_cast<uint64_t*>(base + o.keys);
+        d.roots = reinterpret_cast<uint32_t*>(base + o.roots);
+        d.band_nroots = reinterpret_cast<uint32_t*>(base + o.nroots);
         d.candidates = reinterpret_cast<uint32_t*>(base + o.cands);
         d.quads = reinterpret_cast<float*>(base + o.quads);
         d.hull_quads = reinterpret_cast<float*>(base + o.hquads);
         d.dec = reinterpret_cast<bd_detection*>(base + o.dec);
         d.out = d_out_ + static_cast<size_t>(f) * max_det_;
         prefix[SH_TILE_ROW * (nf + 1) + f + 1] = prefix[SH_TILE_ROW * (nf + 1) + f] + d.tile_h;
-        prefix[SH_ROW4 * (nf + 1) + f + 1] = prefix[SH_ROW4 * (nf + 1) + f] + (H + 3) / 4;
         prefix[SH_FRAME * (nf + 1) + f + 1] = f + 1;
         const int pairs = (d.tile_w + 1) / 2;
         const int rows_per_wg = pairs >= 256 ? 1 : 256 / pairs;

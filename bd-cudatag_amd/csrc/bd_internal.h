@@ -99,6 +99,8 @@ struct FrameDesc {
     uint32_t* area;       // height * row_cap
     uint32_t* bbox;       // height * row_cap * 4
     uint64_t* cand_keys;  // height * row_cap
+    uint32_t* roots;      // height * row_cap: band b's component roots from b * 8 * row_cap
+    uint32_t* band_nroots;  // tile_h: roots listed per band
     uint32_t* candidates; // max_candidates
     float* quads;         // max_candidates * 8
     float* hull_quads;    // max_candidates * 8
@@ -108,7 +110,7 @@ struct FrameDesc {
 
 // Prefix tables over frames: prefix[k][f] = first block of frame f for
 // kernel shape k; prefix[k][nframes] = total blocks.
-enum Shape : int { SH_TILE_ROW = 0, SH_ROW4 = 1, SH_FRAME = 2, SH_TSTAT = 3, SH_COUNT = 4 };
+enum Shape : int { SH_TILE_ROW = 0, SH_FRAME = 1, SH_TSTAT = 2, SH_COUNT = 3 };
 
 // Batch candidate list: a small header, then one entry per selected
 // candidate of any frame (frame << 10 | candidate index), appended by

@@ -1032,7 +1032,10 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
     if (!d->active) return;
     const int32_t* roi = d->row_roi + 4 * ty;
-    if (!roi[0]) return;
+    if (!roi[0]) {
+        if (threadIdx.x == 0) d->band_nroots[ty] = 0;  // an inactive band has no runs
+        return;
+    }
     const int H = d->height;
     const int y0 = ty * kTile;
     const int nrows = min(kTile, H - y0);
@@ -1041,7 +1044,7 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     __shared__ uint32_t hkey[kStatSlots], harea[kStatSlots], hx0[kStatSlots], hx1[kStatSlots], hy1[kStatSlots];
     __shared__ uint32_t s_off[kTile + 1];
     __shared__ uint32_t s_flush[kStatSlots];
-    __shared__ uint32_t s_nflush;
+    __shared__ uint32_t s_nflush, s_nroots;
     for (int e = t; e < kStatSlots; e += blockDim.x) {
         hkey[e] = kEmpty;
         harea[e] = 0;
@@ -1055,7 +1058,10 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         uint32_t total;
         const uint32_t ex = wave_excl_scan(c, t, &total);
         if (t <= kTile) s_off[t] = ex;
-        if (t == 0) s_nflush = 0;
+        if (t == 0) {
+            s_nflush = 0;
+            s_nroots = 0;
+        }
     }
     __syncthreads();
     uint32_t so[kTile + 1];
@@ -1151,7 +1157,12 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         const uint32_t root = labu;
         const int y = y0 + rowu;
         label[slotu] = root;
-        if (root == slotu) d->bbox[4 * static_cast<size_t>(root) + 1] = static_cast<uint32_t>(y);
+        if (root == slotu) {
+            // A component root: its top row, and an entry in the band's root
+            // list (the roots k_collect filters).
+            d->bbox[4 * static_cast<size_t>(root) + 1] = static_cast<uint32_t>(y);
+            d->roots[band_base + atomicAdd(&s_nroots, 1u)] = root;
+        }
         const uint32_t x0 = static_cast<uint32_t>(run_x0(runu)), x1 = static_cast<uint32_t>(run_x1(runu));
         int found = -1;
         const uint32_t h = (root * 2654435761u) >> 23;
@@ -1202,6 +1213,7 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         account(row[3], slot[3], run[3], lab[3]);
     }
     __syncthreads();
+    if (t == 0) d->band_nroots[ty] = s_nroots;
     // Occupied table entries are compacted first so every thread flushes
     // about the same number of roots.
     for (int e = t; e < kStatSlots; e += blockDim.x) {
@@ -1223,22 +1235,22 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
 // (c) candidates
 // ---------------------------------------------------------------------------
 
-// Roots that pass the tag-border filter append the sort key
+// One workgroup per band over the band's component roots (listed by
+// k_stats): roots that pass the tag-border filter append the sort key
 // (~area << 32 | root slot): ascending keys are largest first, ties by
-// raster order (reference k_collect_candidates :591-638).
+// raster order (reference k_collect_candidates :591-638, which visits every
+// run and keeps the roots).
 __global__ __launch_bounds__(256) void k_collect(Plan plan) {
-    int blk;
-    const FrameDesc* d = plan.frames + frame_of(plan, SH_ROW4, blockIdx.x, &blk);
+    int ty;
+    const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
     if (!d->active) return;
-    const int y = blk * 4 + static_cast<int>(threadIdx.x) / kWave;
-    const int lane = threadIdx.x % kWave;
-    if (y >= d->height) return;
     const int W = d->width, H = d->height;
-    const int n = static_cast<int>(d->row_count[y]);
-    const uint32_t base = static_cast<uint32_t>(y) * static_cast<uint32_t>(d->row_cap);
-    for (int i = lane; i < n; i += kWave) {
-        const uint32_t idx = base + static_cast<uint32_t>(i);
-        if (d->label[idx] != idx) continue;
+    const uint32_t row_cap = static_cast<uint32_t>(d->row_cap);
+    const uint32_t base = static_cast<uint32_t>(ty * kTile) * row_cap;
+    const uint32_t n = d->band_nroots[ty];
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint32_t idx = d->roots[base + i];
+        const int y = static_cast<int>(idx / row_cap);
         const int a = static_cast<int>(d->area[idx]);
         if (a < d->min_cluster) continue;
         const uint4 bb = *reinterpret_cast<const uint4*>(d->bbox + 4 * static_cast<size_t>(idx));
@@ -2585,7 +2597,7 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
             break;
         case 3: hipLaunchKernelGGL(k_merge, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
         case 4: hipLaunchKernelGGL(k_stats, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
-        case 5: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_ROW4]), dim3(256), 0, stream, plan); break;
+        case 5: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
         case 6: hipLaunchKernelGGL(k_select, dim3(nf), dim3(256), 0, stream, plan); break;
         case 7: hipLaunchKernelGGL(k_fit, dim3(min(plan.work_cap, plan.tune.fit_grid)), dim3(256), 0, stream, plan); break;
         case 8: {
