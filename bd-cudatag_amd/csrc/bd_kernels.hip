@@ -848,8 +848,15 @@ __device__ inline uint32_t uf_root(const uint32_t* label, uint32_t x) {
 // run in raster order (the reference's uf_union, gpu_detector.cu:408-421).
 __device__ inline void uf_union(uint32_t* label, uint32_t a, uint32_t b) {
     for (;;) {
-        a = uf_root(label, a);
-        b = uf_root(label, b);
+        // Both roots by plain reads, the two walks in lockstep so that their
+        // loads overlap (a finished walk re-reads its root).
+        uint32_t pa = label[a], pb = label[b];
+        while (pa != a || pb != b) {
+            a = pa;
+            b = pb;
+            pa = label[a];
+            pb = label[b];
+        }
         if (a == b) return;
         if (a > b) {
             const uint32_t tmp = a;
