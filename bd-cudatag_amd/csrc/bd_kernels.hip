@@ -1147,6 +1147,14 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     locate(t, row0, slot0);
     load(row0, slot0, run0, lab0);
     walk(slot0, run0, lab0);
+    // Component roots of the first runs' band roots, by compact index.  A
+    // band root is the smallest run of its band-local component, so every
+    // run among the first kItems * 256 finds its band root here.
+    __shared__ uint32_t s_broot[kItems * 256];
+#pragma unroll
+    for (int u = 0; u < kItems; ++u) {
+        if (row0[u] >= 0 && (run0[u] & kRootFlag)) s_broot[t + u * blockDim.x] = lab0[u];
+    }
     for (uint32_t i0 = t + kItems * blockDim.x; i0 < nb; i0 += kItems * blockDim.x) {
         int row[kItems];
         uint32_t slot[kItems], run[kItems], lab[kItems];
@@ -1202,7 +1210,19 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
             if (row[u] >= 0 && !(run[u] & kRootFlag)) lab[u] = label[lab[u]];
         }
     };
-    resolve(row0, run0, lab0);
+    // The first runs take their band root's component root from LDS: the
+    // band root's compact index is its row's offset plus its place in the
+    // row.
+#pragma unroll
+    for (int u = 0; u < kItems; ++u) {
+        if (row0[u] >= 0 && !(run0[u] & kRootFlag)) {
+            const uint32_t rel = lab0[u] - band_base;
+            int r = 0;
+#pragma unroll
+            for (int q = 1; q < kTile; ++q) r += rel >= static_cast<uint32_t>(q) * row_cap ? 1 : 0;
+            lab0[u] = s_broot[s_off[r] + (rel - static_cast<uint32_t>(r) * row_cap)];
+        }
+    }
     account(row0[0], slot0[0], run0[0], lab0[0]);
     account(row0[1], slot0[1], run0[1], lab0[1]);
     account(row0[2], slot0[2], run0[2], lab0[2]);
