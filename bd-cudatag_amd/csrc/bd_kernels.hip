@@ -508,6 +508,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 
     __shared__ uint8_t thr[1024];
     __shared__ uint32_t s_off[kTile + 1];
+    __shared__ uint32_t s_cnt[kTile];  // runs per band row
     // Band arrays in dynamic LDS: plan.band_runs runs each.
     extern __shared__ __align__(16) uint32_t band_lds[];
     const int band_cap = plan.band_runs;
@@ -564,6 +565,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         for (int i = t; i < nrows * ds; i += blockDim.x) d->run_dir[static_cast<size_t>(y0) * ds + i] = 0;
         return;
     }
+    if (t < kTile) s_cnt[t] = 0;
     __syncthreads();
 
     const uint8_t* gray = d->gray;
@@ -572,6 +574,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     // width launch parameter: 64 or 128 px).
     const int wchunks = (w64 + kChunkWords - 1) / kChunkWords;
     const int items = nrows * wchunks;
+    uint32_t frag_acc = 0;
     for (int it = t; it < items; it += blockDim.x) {
         const int r = it / wchunks;
         const int y = y0 + r;
@@ -615,6 +618,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             }
             masks[r][w] = m;
             d->bits[static_cast<size_t>(y) * w64 + w] = m;
+            // The word's run starts, counted here rather than in a pass over
+            // the masks: the bit before the word is the pixel left of it,
+            // thresholded by the same rule (white outside the row's ROI
+            // groups); in image row 0 a fragment also starts at bit 0 and
+            // bit 32 of every word (reference k_find_runs :454).  And the
+            // reference's fragment count, per 32-px half (:454-468).
+            uint64_t prev = 1ull;
+            if (y == 0) {
+                prev = 1ull | (1ull << 32);
+            } else if (w > 0) {
+                const int xp = 64 * w - 1;
+                const int tile0p = (2 * w - 1) * 4;
+                if (tile0p <= re && tile0p + 3 >= rs)
+                    prev = gray[static_cast<size_t>(y) * stride + xp] > thr[xp >> 3] ? 1ull : 0ull;
+            }
+            atomicAdd(&s_cnt[r], static_cast<uint32_t>(__popcll(~m & ((m << 1) | prev))));
+            for (int h = 0; h < 2; ++h) {
+                if ((2 * w + h) * 32 >= W) break;
+                const uint32_t m32 = static_cast<uint32_t>(m >> (32 * h));
+                frag_acc += __popc(~m32 & ((m32 << 1) | 1u));
+            }
         }
     }
     __syncthreads();
@@ -637,34 +661,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         *starts = ~m & ((m << 1) | prev);
     };
 
-    // Pass A: runs per row, and the reference's fragment count (:454-468).
-    uint32_t block_frags = 0;
-    for (int r = wave; r < nrows; r += nwaves) {
-        uint32_t n = 0, frags = 0;
-        for (int w = lane; w < w64; w += kWave) {
-            uint64_t st, en;
-            edges(r, w, &st, &en);
-            n += __popcll(st);
-            const uint64_t m = masks[r][w];
-            for (int h = 0; h < 2; ++h) {
-                if ((2 * w + h) * 32 >= W) break;
-                const uint32_t m32 = static_cast<uint32_t>(m >> (32 * h));
-                frags += __popc(~m32 & ((m32 << 1) | 1u));
-            }
-        }
-        n = wave_sum(n);
-        frags = wave_sum(frags);
-        if (lane == 0) {
-            s_off[r + 1] = n;
-            d->row_count[y0 + r] = n;
-        }
-        block_frags += frags;
+    // Row offsets of the band from the counted starts (a wave scan).
+    if (t < kWave) {
+        const uint32_t c = t < nrows ? s_cnt[t] : 0u;
+        uint32_t total;
+        const uint32_t ex = wave_excl_scan(c, t, &total);
+        if (t <= kTile) s_off[t] = ex;
+        if (t < nrows) d->row_count[y0 + t] = c;
     }
-    if (t == 0) s_off[0] = 0;
-    __syncthreads();
-    if (t == 0) {
-        for (int r = 0; r < nrows; ++r) s_off[r + 1] += s_off[r];
-    }
+    const uint32_t block_frags = wave_sum(frag_acc);
     __syncthreads();
     const uint32_t nb = s_off[nrows];
     const bool local = nb <= static_cast<uint32_t>(band_cap);
