@@ -1732,8 +1732,8 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
         const uint32_t per_row = 2u * width;
         const int shift = 31 - __clz(static_cast<int>(per_row));
         const uint32_t items = nlist << shift;
-        for (uint32_t j0 = static_cast<uint32_t>(t); j0 < items; j0 += 4u * blockDim.x) {
-            constexpr int kItems = 4;
+        for (uint32_t j0 = static_cast<uint32_t>(t); j0 < items; j0 += 2u * blockDim.x) {
+            constexpr int kItems = 2;
             uint32_t slot[kItems], ends[kItems];
             int sr[kItems];
 #pragma unroll
