@@ -1995,15 +1995,16 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
             const bool first_pass = iteration == 0;
             const EdgeGeom g = edge_geometry(fit_quad, wave, d->dim, first_pass);
             if (g.ok) {
-                // Every (sample, probe) rise of this edge, three per lane per
-                // round with all 24 byte loads issued before the arithmetic
-                // (three keeps the kernel at 128 VGPRs, four waves per SIMD).
+                // Every (sample, probe) rise of this edge, one per lane per
+                // round (its four pixel-pair loads issued together); more per
+                // round measured slower: the larger code and registers cost
+                // more than the extra loads in flight gain.
                 const int total = g.samples * g.probes;
-                for (int it0 = lane; it0 < total; it0 += 3 * kWave) {
-                    Tap ta[3], tb[3];
-                    int slot[3];
+                for (int it0 = lane; it0 < total; it0 += 1 * kWave) {
+                    Tap ta[1], tb[1];
+                    int slot[1];
 #pragma unroll
-                    for (int u = 0; u < 3; ++u) {
+                    for (int u = 0; u < 1; ++u) {
                         const int it = it0 + u * kWave;
                         const bool live = it < total;
                         const int s = live ? it / g.probes : 0;
@@ -2022,18 +2023,18 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
                     }
                     // Each tap's 2x2 neighbourhood as two horizontal pixel
                     // pairs (one request each).
-                    uint32_t pa[3][2], pb[3][2];
+                    uint32_t pa[1][2], pb[1][2];
 #pragma unroll
-                    for (int u = 0; u < 3; ++u) {
+                    for (int u = 0; u < 1; ++u) {
                         const uint32_t p = ta[u].off, q = tb[u].off;
                         pa[u][0] = ta[u].ok ? pixel_pair(gray, p, span) : 0u;
                         pa[u][1] = ta[u].ok ? pixel_pair(gray, p + stride, span) : 0u;
                         pb[u][0] = tb[u].ok ? pixel_pair(gray, q, span) : 0u;
                         pb[u][1] = tb[u].ok ? pixel_pair(gray, q + stride, span) : 0u;
                     }
-                    float va[3][4], vb[3][4];
+                    float va[1][4], vb[1][4];
 #pragma unroll
-                    for (int u = 0; u < 3; ++u) {
+                    for (int u = 0; u < 1; ++u) {
                         va[u][0] = static_cast<float>(pa[u][0] & 0xffu);
                         va[u][1] = static_cast<float>(pa[u][0] >> 8);
                         va[u][2] = static_cast<float>(pa[u][1] & 0xffu);
@@ -2044,7 +2045,7 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
                         vb[u][3] = static_cast<float>(pb[u][1] >> 8);
                     }
 #pragma unroll
-                    for (int u = 0; u < 3; ++u) {
+                    for (int u = 0; u < 1; ++u) {
                         if (it0 + u * kWave >= total) continue;
                         const float ahead = tap_value(ta[u], va[u][0], va[u][1], va[u][2], va[u][3]);
                         const float behind = tap_value(tb[u], vb[u][0], vb[u][1], vb[u][2], vb[u][3]);
