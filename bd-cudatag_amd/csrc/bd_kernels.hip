@@ -1092,11 +1092,11 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     // Band roots walk to their root, splitting the path on the way: each
     // visited label is lowered to its grandparent with atomicMin, which can
     // only move it towards the root, so concurrent walkers never undo each
-    // other.  A thread advances four walks in lockstep so their loads
+    // other.  A thread advances its walks in lockstep so their loads
     // overlap.  Walks only visit band roots (a band root's label always
     // names another band root), so the labels of the other runs are final
     // from the start.
-    constexpr int kItems = 4;
+    constexpr int kItems = 2;
     const auto locate = [&](uint32_t i0, int* row, uint32_t* slot) {
 #pragma unroll
         for (int u = 0; u < kItems; ++u) {
@@ -1230,8 +1230,6 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     }
     account(row0[0], slot0[0], run0[0], lab0[0]);
     account(row0[1], slot0[1], run0[1], lab0[1]);
-    account(row0[2], slot0[2], run0[2], lab0[2]);
-    account(row0[3], slot0[3], run0[3], lab0[3]);
     for (uint32_t i0 = t + kItems * blockDim.x; i0 < nb; i0 += kItems * blockDim.x) {
         int row[kItems];
         uint32_t slot[kItems], run[kItems], lab[kItems];
@@ -1241,8 +1239,6 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         resolve(row, run, lab);
         account(row[0], slot[0], run[0], lab[0]);
         account(row[1], slot[1], run[1], lab[1]);
-        account(row[2], slot[2], run[2], lab[2]);
-        account(row[3], slot[3], run[3], lab[3]);
     }
     __syncthreads();
     if (t == 0) d->band_nroots[ty] = s_nroots;
