@@ -1056,7 +1056,7 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     __shared__ uint32_t hkey[kStatSlots], harea[kStatSlots], hx0[kStatSlots], hx1[kStatSlots], hy1[kStatSlots];
     __shared__ uint32_t s_off[kTile + 1];
     __shared__ uint32_t s_flush[kStatSlots];
-    __shared__ uint32_t s_nflush, s_nroots;
+    __shared__ uint32_t s_nflush, s_nroots, s_nwalk;
     for (int e = t; e < kStatSlots; e += blockDim.x) {
         hkey[e] = kEmpty;
         harea[e] = 0;
@@ -1073,6 +1073,7 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         if (t == 0) {
             s_nflush = 0;
             s_nroots = 0;
+            s_nwalk = 0;
         }
     }
     __syncthreads();
@@ -1145,27 +1146,78 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
             }
         }
     };
-    // The band's first kItems * blockDim.x runs stay in registers from the
-    // walk to the accumulation; a longer band reloads the rest.
+    // The band roots (a few dozen among the band's hundreds of runs) are
+    // gathered into an LDS list and walk in one batch, so a band pays the
+    // walk's load latency once however many runs it has.  Band roots beyond
+    // the list's capacity (a band joined in global memory has every run a
+    // band root) walk where they are found.
+    constexpr uint32_t kWalkCap = kItems * 256;
+    __shared__ uint32_t s_wslot[kWalkCap], s_wlab[kWalkCap];
+    // Component roots of the first kItems * 256 runs' band roots, by compact
+    // index.  A band root is the smallest run of its band-local component,
+    // so every one of these runs finds its band root here.
+    __shared__ uint32_t s_broot[kItems * 256];
+    const auto gather = [&](const int* row, const uint32_t* slot, const uint32_t* run, uint32_t* lab, bool first) {
+        uint32_t rest[kItems];
+        bool any = false;
+#pragma unroll
+        for (int u = 0; u < kItems; ++u) {
+            rest[u] = 0;
+            if (row[u] >= 0 && (run[u] & kRootFlag)) {
+                const uint32_t k = atomicAdd(&s_nwalk, 1u);
+                if (k < kWalkCap) {
+                    s_wslot[k] = slot[u];
+                    s_wlab[k] = lab[u];
+                } else {
+                    rest[u] = kRootFlag;
+                    any = true;
+                }
+            }
+        }
+        if (__any(any)) {
+            walk(slot, rest, lab);
+#pragma unroll
+            for (int u = 0; u < kItems; ++u) {
+                if (first && rest[u]) s_broot[t + u * blockDim.x] = lab[u];
+            }
+        }
+    };
+    // The band's first kItems * blockDim.x runs stay in registers from here
+    // to the accumulation; a longer band reloads the rest.
     int row0[kItems];
     uint32_t slot0[kItems], run0[kItems], lab0[kItems];
     locate(t, row0, slot0);
     load(row0, slot0, run0, lab0);
-    walk(slot0, run0, lab0);
-    // Component roots of the first runs' band roots, by compact index.  A
-    // band root is the smallest run of its band-local component, so every
-    // run among the first kItems * 256 finds its band root here.
-    __shared__ uint32_t s_broot[kItems * 256];
-#pragma unroll
-    for (int u = 0; u < kItems; ++u) {
-        if (row0[u] >= 0 && (run0[u] & kRootFlag)) s_broot[t + u * blockDim.x] = lab0[u];
-    }
+    gather(row0, slot0, run0, lab0, true);
     for (uint32_t i0 = t + kItems * blockDim.x; i0 < nb; i0 += kItems * blockDim.x) {
         int row[kItems];
         uint32_t slot[kItems], run[kItems], lab[kItems];
         locate(i0, row, slot);
         load(row, slot, run, lab);
+        gather(row, slot, run, lab, false);
+    }
+    __syncthreads();
+    const uint32_t nwalk = min(s_nwalk, kWalkCap);
+    if (static_cast<uint32_t>(t) < nwalk) {
+        uint32_t slot[kItems], run[kItems], lab[kItems];
+#pragma unroll
+        for (int u = 0; u < kItems; ++u) {
+            const uint32_t k = t + u * blockDim.x;
+            run[u] = k < nwalk ? kRootFlag : 0u;
+            slot[u] = k < nwalk ? s_wslot[k] : 0u;
+            lab[u] = k < nwalk ? s_wlab[k] : 0u;
+        }
         walk(slot, run, lab);
+#pragma unroll
+        for (int u = 0; u < kItems; ++u) {
+            if (!run[u]) continue;
+            const uint32_t rel = slot[u] - band_base;
+            int r = 0;
+#pragma unroll
+            for (int q = 1; q < kTile; ++q) r += rel >= static_cast<uint32_t>(q) * row_cap ? 1 : 0;
+            const uint32_t i = s_off[r] + (rel - static_cast<uint32_t>(r) * row_cap);
+            if (i < kItems * 256) s_broot[i] = lab[u];
+        }
     }
     __syncthreads();
 
@@ -1217,16 +1269,19 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     };
     // The first runs take their band root's component root from LDS: the
     // band root's compact index is its row's offset plus its place in the
-    // row.
+    // row, and a band root's own entry holds its component root.
 #pragma unroll
     for (int u = 0; u < kItems; ++u) {
-        if (row0[u] >= 0 && !(run0[u] & kRootFlag)) {
+        if (row0[u] < 0) continue;
+        uint32_t idx = t + u * blockDim.x;
+        if (!(run0[u] & kRootFlag)) {
             const uint32_t rel = lab0[u] - band_base;
             int r = 0;
 #pragma unroll
             for (int q = 1; q < kTile; ++q) r += rel >= static_cast<uint32_t>(q) * row_cap ? 1 : 0;
-            lab0[u] = s_broot[s_off[r] + (rel - static_cast<uint32_t>(r) * row_cap)];
+            idx = s_off[r] + (rel - static_cast<uint32_t>(r) * row_cap);
         }
+        lab0[u] = s_broot[idx];
     }
     account(row0[0], slot0[0], run0[0], lab0[0]);
     account(row0[1], slot0[1], run0[1], lab0[1]);
