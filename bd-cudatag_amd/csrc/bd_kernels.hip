@@ -1223,19 +1223,13 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
 
     // Every run takes its root and is accounted.  One call per item with
     // scalar arguments (a loop over the item arrays with the probe loop
-    // inside would not unroll and would put the arrays in scratch memory).
-    const auto account = [&](int rowu, uint32_t slotu, uint32_t runu, uint32_t labu) {
-        if (rowu < 0) return;
-        const uint32_t root = labu;
-        const int y = y0 + rowu;
-        label[slotu] = root;
-        if (root == slotu) {
-            // A component root: its top row, and an entry in the band's root
-            // list (the roots k_collect filters).
-            d->bbox[4 * static_cast<size_t>(root) + 1] = static_cast<uint32_t>(y);
-            d->roots[band_base + atomicAdd(&s_nroots, 1u)] = root;
-        }
-        const uint32_t x0 = static_cast<uint32_t>(run_x0(runu)), x1 = static_cast<uint32_t>(run_x1(runu));
+    // inside would not unroll and would put the arrays in scratch memory),
+    // made by the whole wave: a wave's 64 runs are consecutive runs of the
+    // band and mostly belong to one or two components, so the runs sharing
+    // the first lane's root are reduced across the wave (DPP) and folded
+    // into the table once; the other runs fold one by one.
+    const int lane = t & (kWave - 1);
+    const auto fold = [&](uint32_t root, uint32_t area, uint32_t x0, uint32_t x1, uint32_t y1) {
         int found = -1;
         const uint32_t h = (root * 2654435761u) >> 23;
         for (int probe = 0; probe < 32; ++probe) {
@@ -1248,16 +1242,48 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
             }
         }
         if (found >= 0) {
-            atomicAdd(&harea[found], x1 - x0 + 1);
+            atomicAdd(&harea[found], area);
             atomicMin(&hx0[found], x0);
             atomicMax(&hx1[found], x1);
-            atomicMax(&hy1[found], static_cast<uint32_t>(y));
+            atomicMax(&hy1[found], y1);
         } else {
-            atomicAdd(&d->area[root], x1 - x0 + 1);
+            atomicAdd(&d->area[root], area);
             atomicMin(&d->bbox[4 * static_cast<size_t>(root) + 0], x0);
             atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 2], x1);
-            atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 3], static_cast<uint32_t>(y));
+            atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 3], y1);
         }
+    };
+    constexpr int kWaveFoldMin = 4;  // fewer shared runs fold one by one
+    const auto account = [&](int rowu, uint32_t slotu, uint32_t runu, uint32_t labu) {
+        const bool valid = rowu >= 0;
+        const uint32_t root = labu;
+        const uint32_t y = static_cast<uint32_t>(y0 + rowu);
+        const uint32_t x0 = static_cast<uint32_t>(run_x0(runu)), x1 = static_cast<uint32_t>(run_x1(runu));
+        if (valid) {
+            label[slotu] = root;
+            if (root == slotu) {
+                // A component root: its top row, and an entry in the band's
+                // root list (the roots k_collect filters).
+                d->bbox[4 * static_cast<size_t>(root) + 1] = y;
+                d->roots[band_base + atomicAdd(&s_nroots, 1u)] = root;
+            }
+        }
+        bool one_by_one = valid;
+        const uint64_t vm = __ballot(valid);
+        if (vm) {
+            const int leader = __ffsll(static_cast<long long>(vm)) - 1;
+            const uint32_t lead_root = wave_lane(root, leader);
+            const bool same = valid && root == lead_root;
+            if (__popcll(__ballot(same)) >= kWaveFoldMin) {
+                const uint32_t area = wave_sum(same ? x1 - x0 + 1 : 0u);
+                const uint32_t mn = wave_min(same ? x0 : 0xffffffffu);
+                const uint32_t mx = wave_max(same ? x1 : 0u);
+                const uint32_t my = wave_max(same ? y : 0u);
+                if (lane == leader) fold(lead_root, area, mn, mx, my);
+                one_by_one = valid && !same;
+            }
+        }
+        if (one_by_one) fold(root, x1 - x0 + 1, x0, x1, y);
     };
     // A run that is not a band root points at its band root, whose label
     // is now the component root.
@@ -1285,10 +1311,10 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     }
     account(row0[0], slot0[0], run0[0], lab0[0]);
     account(row0[1], slot0[1], run0[1], lab0[1]);
-    for (uint32_t i0 = t + kItems * blockDim.x; i0 < nb; i0 += kItems * blockDim.x) {
+    for (uint32_t b0 = kItems * blockDim.x; b0 < nb; b0 += kItems * blockDim.x) {
         int row[kItems];
         uint32_t slot[kItems], run[kItems], lab[kItems];
-        locate(i0, row, slot);
+        locate(b0 + t, row, slot);
         load(row, slot, run, lab);
         // A band root's label is its root (written by the walk).
         resolve(row, run, lab);
