@@ -91,7 +91,11 @@ class ServeReport(ctypes.Structure):
 class Tune(ctypes.Structure):
     _fields_ = [("fit_grid", ctypes.c_int32), ("decode_grid", ctypes.c_int32),
                 ("head_ranks", ctypes.c_int32), ("streams", ctypes.c_int32),
-                ("decode_waves", ctypes.c_int32), ("chunk_px", ctypes.c_int32)]
+                ("decode_waves", ctypes.c_int32), ("chunk_px", ctypes.c_int32),
+                ("head_area", ctypes.c_int32)]
+
+
+TUNE_KEYS = tuple(name for name, _ in Tune._fields_)
 
 
 _P = ctypes.c_void_p
@@ -312,12 +316,11 @@ class Batch:
         t = Tune()
         if lib().bd_batch_get_tune(self.handle, ctypes.byref(t)) != BD_OK:
             raise RuntimeError(f"bd_batch_get_tune failed: {last_error()}")
-        return {"fit_grid": t.fit_grid, "decode_grid": t.decode_grid, "head_ranks": t.head_ranks,
-                "streams": t.streams, "decode_waves": t.decode_waves, "chunk_px": t.chunk_px}
+        return {k: getattr(t, k) for k in TUNE_KEYS}
 
     def set_tune(self, fit_grid: int, decode_grid: int, head_ranks: int, streams: int = 2,
-                 decode_waves: int = 8, chunk_px: int = 64) -> None:
-        t = Tune(fit_grid, decode_grid, head_ranks, streams, decode_waves, chunk_px)
+                 decode_waves: int = 8, chunk_px: int = 64, head_area: int = 2000) -> None:
+        t = Tune(fit_grid, decode_grid, head_ranks, streams, decode_waves, chunk_px, head_area)
         rc = lib().bd_batch_set_tune(self.handle, ctypes.byref(t))
         if rc != BD_OK:
             raise (ValueError if rc == BD_ERROR_INVALID_ARGUMENT else RuntimeError)(
@@ -342,16 +345,15 @@ class Batch:
             if key in table:
                 t = table[key]
                 self.set_tune(t["fit_grid"], t["decode_grid"], t["head_ranks"], t.get("streams", 2),
-                              t.get("decode_waves", 8), t.get("chunk_px", 64))
+                              t.get("decode_waves", 8), t.get("chunk_px", 64), t.get("head_area", 2000))
                 return dict(t, cached=True)
         best = Tune()
         ms = ctypes.c_float()
         rc = lib().bd_batch_autotune(self.handle, reps, ctypes.byref(best), ctypes.byref(ms))
         if rc != BD_OK:
             raise RuntimeError(f"bd_batch_autotune failed: {last_error()}")
-        out = {"fit_grid": best.fit_grid, "decode_grid": best.decode_grid, "head_ranks": best.head_ranks,
-               "streams": best.streams, "decode_waves": best.decode_waves, "chunk_px": best.chunk_px,
-               "pipeline_ms": round(float(ms.value), 4)}
+        out = {k: getattr(best, k) for k in TUNE_KEYS}
+        out["pipeline_ms"] = round(float(ms.value), 4)
         if cache:
             table[key] = out
             os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)

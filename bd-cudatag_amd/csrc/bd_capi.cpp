@@ -375,7 +375,7 @@ int32_t bd_batch_get_tune(bd_batch* batch, bd_tune* out) {
         return BD_ERROR_INVALID_ARGUMENT;
     }
     const bd::Tune t = batch->engine->tune();
-    *out = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, t.streams, t.decode_waves, t.chunk_px};
+    *out = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, t.streams, t.decode_waves, t.chunk_px, t.head_area};
     return BD_OK;
 }
 
@@ -391,6 +391,7 @@ int32_t bd_batch_set_tune(bd_batch* batch, const bd_tune* tune) {
     t.streams = tune->streams == 0 ? bd::Tune{}.streams : tune->streams;
     t.decode_waves = tune->decode_waves == 0 ? bd::Tune{}.decode_waves : tune->decode_waves;
     t.chunk_px = tune->chunk_px == 0 ? bd::Tune{}.chunk_px : tune->chunk_px;
+    t.head_area = tune->head_area == 0 ? bd::Tune{}.head_area : tune->head_area;
     std::string error;
     if (!batch->engine->set_tune(t, &error)) {
         set_error(error);
@@ -411,7 +412,7 @@ int32_t bd_batch_autotune(bd_batch* batch, int32_t reps, bd_tune* best, float* b
         set_error(error);
         return BD_ERROR_CUDA;
     }
-    if (best) *best = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, t.streams, t.decode_waves, t.chunk_px};
+    if (best) *best = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, t.streams, t.decode_waves, t.chunk_px, t.head_area};
     if (best_ms) *best_ms = ms;
     return BD_OK;
 }

@@ -154,7 +154,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     }
     size_t work_cap = 0;
     for (int f = 0; f < nf; ++f) work_cap += static_cast<size_t>(maxc[f]);
-    const size_t work_words = kWorkItems + static_cast<size_t>(kHeadRanksMax) * nf + work_cap;
+    const size_t work_words = kWorkItems + static_cast<size_t>(kHeadSlots) * nf + work_cap;
     const size_t o_work = take(work_words * 4);
     // Lanes: frames [0, split) and [split, nf), split where the two pixel
     // counts are closest.
@@ -181,7 +181,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     size_t lane_cap[2] = {0, 0}, o_lane_work[2] = {0, 0}, o_lane_prefix[2] = {0, 0};
     for (int l = 0; l < 2 && lanes_ == 2; ++l) {
         for (int f = lane_first[l]; f < lane_first[l] + lane_count[l]; ++f) lane_cap[l] += static_cast<size_t>(maxc[f]);
-        o_lane_work[l] = take((kWorkItems + static_cast<size_t>(kHeadRanksMax) * lane_count[l] + lane_cap[l]) * 4);
+        o_lane_work[l] = take((kWorkItems + static_cast<size_t>(kHeadSlots) * lane_count[l] + lane_cap[l]) * 4);
         o_lane_prefix[l] = take(static_cast<size_t>(SH_COUNT) * (lane_count[l] + 1) * 4);
     }
     const size_t o_scal = take(static_cast<size_t>(nf) * S_COUNT * 4);
@@ -295,7 +295,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         st = hipMemcpy(base + o_lane_prefix[l], lane_prefix[l].data(), lane_prefix[l].size() * 4, hipMemcpyHostToDevice);
         if (st == hipSuccess)
             st = hipMemset(base + o_lane_work[l], 0,
-                           (kWorkItems + static_cast<size_t>(kHeadRanksMax) * lane_count[l] + lane_cap[l]) * 4);
+                           (kWorkItems + static_cast<size_t>(kHeadSlots) * lane_count[l] + lane_cap[l]) * 4);
     }
     if (st == hipSuccess) st = hipMemcpy(d_codes_, codes, static_cast<size_t>(code_count) * 8, hipMemcpyHostToDevice);
     // Accumulators start cleared; afterwards each consumer kernel clears them.
@@ -717,7 +717,7 @@ bool Engine::set_tune_locked(const Tune& t, std::string* error) {
     if (t.fit_grid < 64 || t.fit_grid > 16384 || t.decode_grid < 64 || t.decode_grid > 16384 ||
         t.head_ranks < 0 || t.head_ranks > kHeadRanksMax || t.streams < 1 || t.streams > 2 ||
         (t.decode_waves != 4 && t.decode_waves != 8 && t.decode_waves != 16) ||
-        (t.chunk_px != 64 && t.chunk_px != 128)) {
+        (t.chunk_px != 64 && t.chunk_px != 128) || t.head_area < 1) {
         if (error) *error = "launch parameters out of range";
         return false;
     }
@@ -786,6 +786,12 @@ bool Engine::autotune(int reps, Tune* best, float* best_ms, std::string* error) 
     for (int32_t h = 0; h <= kHeadRanksMax; ++h) {
         Tune t = cur;
         t.head_ranks = h;
+        if (!consider(t)) return false;
+    }
+    for (const int32_t a : {1000, 2000, 4000, 8000, INT32_MAX}) {
+        if (a == cur.head_area) continue;
+        Tune t = cur;
+        t.head_area = a;
         if (!consider(t)) return false;
     }
     if (lanes_ == 2) {

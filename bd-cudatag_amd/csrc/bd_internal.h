@@ -115,15 +115,18 @@ enum Shape : int { SH_TILE_ROW = 0, SH_FRAME = 1, SH_TSTAT = 2, SH_COUNT = 3 };
 // Batch candidate list: a small header, then one entry per selected
 // candidate of any frame (frame << 10 | candidate index), appended by
 // k_select and consumed by the persistent k_fit and k_decode grids.  The
-// first Plan::head_ranks candidates of every frame (its largest) sit in a
-// head region of kHeadRanksMax * nframes entries that is handed out first;
-// the rest follow in a tail region.
+// first Tune::head_ranks candidates of every frame (its largest), and every
+// further candidate of at least Tune::head_area pixels (up to kHeadSlots per
+// frame), sit in a head region of kHeadSlots * nframes entries that is
+// handed out first; the rest follow in a tail region in frame order.
 enum WorkSlot : int { kWorkCount = 0, kWorkFitCursor = 1, kWorkHeadCount = 3, kWorkItems = 4 };  // slot 2 unused
 constexpr int kHeadRanksMax = 4;
+constexpr int kHeadSlots = 32;
 
 // Launch parameters that change speed only, never results (autotuned per
 // batch shape): persistent grid sizes of k_fit and k_decode, the number of
-// each frame's largest candidates scheduled first, and whether a batch of
+// each frame's largest candidates scheduled first and the area above which
+// a candidate is scheduled first whatever its rank, and whether a batch of
 // two or more frames runs as two concurrent lanes (streams = 2) or as one.
 struct Tune {
     int32_t fit_grid = 1280;     // 5 resident 256-thread workgroups per CU
@@ -132,12 +135,13 @@ struct Tune {
     int32_t streams = 2;
     int32_t decode_waves = 8;    // 4, 8 or 16 waves per decode workgroup
     int32_t chunk_px = 64;       // pixels packed per k_binarize_runs item: 64 or 128
+    int32_t head_area = 2000;    // pixels; larger candidates join the head region
 };
 
 struct Plan {
     const FrameDesc* frames;
     const int32_t* prefix;  // SH_COUNT * (nframes + 1)
-    uint32_t* work;         // kWorkItems + kHeadRanksMax * nframes + work_cap entries
+    uint32_t* work;         // kWorkItems + kHeadSlots * nframes + work_cap entries
     int32_t nframes;
     int32_t work_cap;       // sum of max_candidates over frames
     int32_t band_runs;      // LDS band capacity of k_binarize_runs (runs)

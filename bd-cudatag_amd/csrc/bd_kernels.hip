@@ -1464,16 +1464,25 @@ __global__ __launch_bounds__(256) void k_select(Plan plan) {
     __syncthreads();
     if (m > 1) bitonic_sort_lds(keys, p2);
     // The frame's largest candidates go to the head of the batch list, so
-    // that the longest fits of the batch start first.
+    // that the longest fits of the batch start first: its first head_ranks
+    // and every other one of at least head_area pixels (a prefix of the
+    // sorted keys, whose high word is the complemented area).
     __shared__ uint32_t s_head, s_tail;
-    const uint32_t nhead = min(nsel, static_cast<uint32_t>(plan.tune.head_ranks));
+    uint32_t nbig = 0;
+    for (uint32_t i0 = 0; i0 < nsel; i0 += blockDim.x) {
+        const uint32_t i = i0 + t;
+        const bool big = i < nsel && ~static_cast<uint32_t>(keys[i] >> 32) >= static_cast<uint32_t>(plan.tune.head_area);
+        nbig += static_cast<uint32_t>(__syncthreads_count(big));
+    }
+    const uint32_t nhead = min(min(nsel, static_cast<uint32_t>(kHeadSlots)),
+                               max(nbig, static_cast<uint32_t>(plan.tune.head_ranks)));
     if (t == 0) {
         s_head = atomicAdd(&plan.work[kWorkHeadCount], nhead);
         s_tail = atomicAdd(&plan.work[kWorkCount], nsel - nhead);
     }
     __syncthreads();
     uint32_t* head = plan.work + kWorkItems;
-    uint32_t* tail = head + kHeadRanksMax * plan.nframes;
+    uint32_t* tail = head + kHeadSlots * plan.nframes;
     for (uint32_t i = t; i < nsel; i += blockDim.x) {
         d->candidates[i] = static_cast<uint32_t>(keys[i]);
         const uint32_t code = (blockIdx.x << 10) | i;
@@ -2281,7 +2290,7 @@ __device__ inline uint32_t next_item(uint32_t* cursor) {
 // largest candidates) first, then the tail region.
 __device__ inline uint32_t work_item(const Plan& plan, uint32_t item, uint32_t nhead) {
     const uint32_t* head = plan.work + kWorkItems;
-    return item < nhead ? head[item] : head[kHeadRanksMax * plan.nframes + (item - nhead)];
+    return item < nhead ? head[item] : head[kHeadSlots * plan.nframes + (item - nhead)];
 }
 
 // Persistent workgroups over the batch's candidate list (k_select appends

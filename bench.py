@@ -148,7 +148,7 @@ def uneven_batch(bdcudatag, devmem, scenes, steps):
     return {"frames": len(all_sizes), "mpix_per_step": round(px / 1e6, 2),
             "mpix_s": round(px * steps / t / 1e6, 1), "ms_per_step": round(t / steps * 1e3, 3),
             "device_ms_per_step": round(dev_ms, 3), "device_ms_default_launch": round(default_ms, 3),
-            "tune": {k: tune[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams", "decode_waves", "chunk_px")}}
+            "tune": {k: tune[k] for k in bdcudatag.TUNE_KEYS}}
 
 
 def precision(bdcudatag, devmem, scenes, frames_total):
@@ -260,9 +260,7 @@ def main():
     # --- batched path: B frames per step, device-resident inputs -----------
     batch = bdcudatag.Batch(0, [(W, H)] * B)
     if args.streams:
-        t0_ = batch.get_tune()
-        batch.set_tune(t0_["fit_grid"], t0_["decode_grid"], t0_["head_ranks"], args.streams,
-                       t0_["decode_waves"], t0_["chunk_px"])
+        batch.set_tune(**dict(batch.get_tune(), streams=args.streams))
     batch.detect(list(frames))  # stage frames for the per-kernel timing below
     stages = batch.time_stages(reps=5)
     res = batch.detect_device(ptrs)
@@ -270,8 +268,7 @@ def main():
     # Launch parameters for this batch shape.
     tune = batch.get_tune() if args.no_autotune else batch.autotune(reps=5)
     if args.streams:
-        batch.set_tune(tune["fit_grid"], tune["decode_grid"], tune["head_ranks"], args.streams,
-                       tune["decode_waves"], tune["chunk_px"])
+        batch.set_tune(**dict({k: tune[k] for k in bdcudatag.TUNE_KEYS}, streams=args.streams))
         tune = batch.get_tune()
     batch.prepare_device(ptrs)
     for _ in range(WU):
@@ -360,7 +357,7 @@ def main():
                            "note": "host-path graph replays: H2D of the frames from pinned staging "
                                    "inside the graph, kernels, D2H"},
         "detections_per_step": found,
-        "tune": {k: tune[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams", "decode_waves", "chunk_px")},
+        "tune": {k: tune[k] for k in bdcudatag.TUNE_KEYS},
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": {
             "bound": "hbm",

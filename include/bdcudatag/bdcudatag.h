@@ -250,6 +250,9 @@ typedef struct bd_tune {
                              bd_batch_set_tune selects the default */
     int32_t chunk_px;     /* pixels a binarization thread packs per item: 64 or 128 (default 64;
                              0 in bd_batch_set_tune selects the default) */
+    int32_t head_area;    /* candidates of at least this many pixels are also scheduled first
+                             (at most 32 per frame), >= 1; INT32_MAX turns this off (default
+                             2000; 0 in bd_batch_set_tune selects the default) */
 } bd_tune;
 
 BD_API int32_t bd_batch_get_tune(bd_batch* batch, bd_tune* out);

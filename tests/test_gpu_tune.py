@@ -25,27 +25,29 @@ def test_launch_parameters_keep_results(tmp_path):
     imgs = [scenes.scene(2, w, h, 40 + i)[0] for i, (w, h) in enumerate(sizes)]
     batch = bdcudatag.Batch(0, sizes)
     assert batch.get_tune() == {"fit_grid": 1280, "decode_grid": 1024, "head_ranks": 2, "streams": 2,
-                                "decode_waves": 8, "chunk_px": 64}
+                                "decode_waves": 8, "chunk_px": 64, "head_area": 2000}
     ref = batch.detect(imgs)
     assert sum(len(r) for r in ref) > 0
-    for fg, dg, hr, ns, dw, cp in [(64, 64, 0, 1, 4, 128), (256, 2048, 4, 2, 16, 64), (4096, 512, 1, 1, 8, 128),
-                                   (1280, 1024, 2, 2, 4, 64), (1280, 100, 2, 1, 16, 128)]:
-        batch.set_tune(fg, dg, hr, ns, dw, cp)
+    # head_area 1 puts every candidate (up to 32 per frame) in the head
+    # region, 2**31 - 1 none beyond the head ranks.
+    for fg, dg, hr, ns, dw, cp, ha in [(64, 64, 0, 1, 4, 128, 1), (256, 2048, 4, 2, 16, 64, 2**31 - 1),
+                                       (4096, 512, 1, 1, 8, 128, 500), (1280, 1024, 2, 2, 4, 64, 1),
+                                       (1280, 100, 2, 1, 16, 128, 8000)]:
+        batch.set_tune(fg, dg, hr, ns, dw, cp, ha)
         assert batch.get_tune() == {"fit_grid": fg, "decode_grid": dg, "head_ranks": hr, "streams": ns,
-                                    "decode_waves": dw, "chunk_px": cp}
+                                    "decode_waves": dw, "chunk_px": cp, "head_area": ha}
         same(batch.detect(imgs), ref)
     for bad in [(10, 1024, 2), (1024, 1 << 20, 2), (1024, 1024, 5), (1024, 1024, 2, 3), (1024, 1024, 2, 2, 12),
-                (1024, 1024, 2, 2, 8, 96)]:
+                (1024, 1024, 2, 2, 8, 96), (1024, 1024, 2, 2, 8, 64, -5)]:
         with pytest.raises(ValueError):
             batch.set_tune(*bad)
     cache = str(tmp_path / "tune.json")
     first = batch.autotune(reps=2, cache=cache)
     assert not first["cached"] and first["pipeline_ms"] > 0
-    assert batch.get_tune() == {k: first[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams", "decode_waves", "chunk_px")}
+    assert batch.get_tune() == {k: first[k] for k in bdcudatag.TUNE_KEYS}
     same(batch.detect(imgs), ref)
     again = batch.autotune(reps=2, cache=cache)
-    assert again["cached"] and {k: again[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams", "decode_waves", "chunk_px")} == \
-        {k: first[k] for k in ("fit_grid", "decode_grid", "head_ranks", "streams", "decode_waves", "chunk_px")}
+    assert again["cached"] and {k: again[k] for k in bdcudatag.TUNE_KEYS} == {k: first[k] for k in bdcudatag.TUNE_KEYS}
 
 
 def test_one_frame_batch_runs_one_lane():
