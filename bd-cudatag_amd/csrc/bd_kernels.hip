@@ -1183,18 +1183,30 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         }
     };
     // The band's first kItems * blockDim.x runs stay in registers from here
-    // to the accumulation; a longer band reloads the rest.
+    // to the accumulation; a longer band reloads the rest.  (One loop for
+    // both keeps a single copy of the code.)
     int row0[kItems];
     uint32_t slot0[kItems], run0[kItems], lab0[kItems];
-    locate(t, row0, slot0);
-    load(row0, slot0, run0, lab0);
-    gather(row0, slot0, run0, lab0, true);
-    for (uint32_t i0 = t + kItems * blockDim.x; i0 < nb; i0 += kItems * blockDim.x) {
+#pragma unroll
+    for (int u = 0; u < kItems; ++u) {
+        row0[u] = -1;
+        slot0[u] = run0[u] = lab0[u] = 0;
+    }
+    for (uint32_t b0 = 0; b0 < nb; b0 += kItems * blockDim.x) {
         int row[kItems];
         uint32_t slot[kItems], run[kItems], lab[kItems];
-        locate(i0, row, slot);
+        locate(b0 + t, row, slot);
         load(row, slot, run, lab);
-        gather(row, slot, run, lab, false);
+        if (b0 == 0) {
+#pragma unroll
+            for (int u = 0; u < kItems; ++u) {
+                row0[u] = row[u];
+                slot0[u] = slot[u];
+                run0[u] = run[u];
+                lab0[u] = lab[u];
+            }
+        }
+        gather(row, slot, run, lab, b0 == 0);
     }
     __syncthreads();
     const uint32_t nwalk = min(s_nwalk, kWalkCap);
@@ -1268,22 +1280,29 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
                 d->roots[band_base + atomicAdd(&s_nroots, 1u)] = root;
             }
         }
-        bool one_by_one = valid;
+        // What this lane folds: its own run, or the leader's wave total.
+        uint32_t f_root = root, f_area = x1 - x0 + 1, f_x0 = x0, f_x1 = x1, f_y1 = y;
+        bool folds = valid;
         const uint64_t vm = __ballot(valid);
         if (vm) {
             const int leader = __ffsll(static_cast<long long>(vm)) - 1;
             const uint32_t lead_root = wave_lane(root, leader);
             const bool same = valid && root == lead_root;
             if (__popcll(__ballot(same)) >= kWaveFoldMin) {
-                const uint32_t area = wave_sum(same ? x1 - x0 + 1 : 0u);
+                const uint32_t area = wave_sum(same ? f_area : 0u);
                 const uint32_t mn = wave_min(same ? x0 : 0xffffffffu);
                 const uint32_t mx = wave_max(same ? x1 : 0u);
                 const uint32_t my = wave_max(same ? y : 0u);
-                if (lane == leader) fold(lead_root, area, mn, mx, my);
-                one_by_one = valid && !same;
+                folds = valid && (!same || lane == leader);
+                if (lane == leader) {
+                    f_area = area;
+                    f_x0 = mn;
+                    f_x1 = mx;
+                    f_y1 = my;
+                }
             }
         }
-        if (one_by_one) fold(root, x1 - x0 + 1, x0, x1, y);
+        if (folds) fold(f_root, f_area, f_x0, f_x1, f_y1);
     };
     // A run that is not a band root points at its band root, whose label
     // is now the component root.
@@ -1309,15 +1328,23 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         }
         lab0[u] = s_broot[idx];
     }
-    account(row0[0], slot0[0], run0[0], lab0[0]);
-    account(row0[1], slot0[1], run0[1], lab0[1]);
-    for (uint32_t b0 = kItems * blockDim.x; b0 < nb; b0 += kItems * blockDim.x) {
+    for (uint32_t b0 = 0; b0 < nb; b0 += kItems * blockDim.x) {
         int row[kItems];
         uint32_t slot[kItems], run[kItems], lab[kItems];
-        locate(b0 + t, row, slot);
-        load(row, slot, run, lab);
-        // A band root's label is its root (written by the walk).
-        resolve(row, run, lab);
+        if (b0 == 0) {
+#pragma unroll
+            for (int u = 0; u < kItems; ++u) {
+                row[u] = row0[u];
+                slot[u] = slot0[u];
+                run[u] = run0[u];
+                lab[u] = lab0[u];
+            }
+        } else {
+            locate(b0 + t, row, slot);
+            load(row, slot, run, lab);
+            // A band root's label is its root (written by the walk).
+            resolve(row, run, lab);
+        }
         account(row[0], slot[0], run[0], lab[0]);
         account(row[1], slot[1], run[1], lab[1]);
     }
