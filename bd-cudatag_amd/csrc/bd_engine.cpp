@@ -105,7 +105,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     // blocks (contiguous so one copy returns them for all frames).
     struct Off {
         size_t tmin, tmax, tmean, tthr, trlo, trhi, roi, hist, bits, rowc, rdir, runs, label, area, bbox,
-            keys, roots, nroots, cands, quads, hquads, dec;
+            keys, roots, nroots, rext, rxfull, cands, quads, hquads, dec;
     };
     std::vector<Off> off(nf);
     std::vector<int> maxc(nf);
@@ -145,6 +145,8 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         o.keys = take(slots * 8);
         o.roots = take(slots * 4);
         o.nroots = take(static_cast<size_t>(th) * 4);
+        o.rext = take(static_cast<size_t>(H) * kRowExtSlots * 8);
+        o.rxfull = take(static_cast<size_t>(H) * 4);
         o.cands = take(static_cast<size_t>(maxc[f]) * 4);
         o.quads = take(static_cast<size_t>(maxc[f]) * 32);
         o.hquads = take(static_cast<size_t>(maxc[f]) * 32);
@@ -263,6 +265,8 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         d.cand_keys = reinterpret_cast<uint64_t*>(base + o.keys);
         d.roots = reinterpret_cast<uint32_t*>(base + o.roots);
         d.band_nroots = reinterpret_cast<uint32_t*>(base + o.nroots);
+        d.row_ext = reinterpret_cast<uint64_t*>(base + o.rext);
+        d.row_ext_full = reinterpret_cast<uint32_t*>(base + o.rxfull);
         d.candidates = reinterpret_cast<uint32_t*>(base + o.cands);
         d.quads = reinterpret_cast<float*>(base + o.quads);
         d.hull_quads = reinterpret_cast<float*>(base + o.hquads);

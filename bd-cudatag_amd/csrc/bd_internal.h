@@ -24,7 +24,7 @@ constexpr int kFitIterations = 2;
 constexpr int kFitProbeHalf = 24;
 constexpr int kMaxProbes = 2 * kFitProbeHalf + 1;
 constexpr int kMaxSamples = 16;
-constexpr uint32_t kFitWindow = 8;   // runs checked at each end of a row range first
+constexpr int kRowExtSlots = 64;     // component slots per row of the row-extreme tables
 constexpr int kMaxCodes = 587;       // largest codebook (tag36h11)
 
 // Device frames are addressed by 32-bit byte offsets from their base in the
@@ -101,6 +101,10 @@ struct FrameDesc {
     uint64_t* cand_keys;  // height * row_cap
     uint32_t* roots;      // height * row_cap: band b's component roots from b * 8 * row_cap
     uint32_t* band_nroots;  // tile_h: roots listed per band
+    uint64_t* row_ext;    // height * kRowExtSlots: per row, a hash table of the
+                          // components in the row: root << 32 | x0 << 16 | x1
+                          // (leftmost start, rightmost end), ~0 when empty
+    uint32_t* row_ext_full;  // height: 1 when the row's table overflowed
     uint32_t* candidates; // max_candidates
     float* quads;         // max_candidates * 8
     float* hull_quads;    // max_candidates * 8

@@ -837,16 +837,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 // (b) union-find over runs
 // ---------------------------------------------------------------------------
 
-// Root of x by plain reads only.  A stale read returns an older, larger
-// label that is still an ancestor, so a walk may be longer but never wrong.
-__device__ inline uint32_t uf_root(const uint32_t* label, uint32_t x) {
-    uint32_t p = label[x];
-    while (p != x) {
-        x = p;
-        p = label[x];
-    }
-    return x;
-}
 
 // Hooks the larger root under the smaller one with atomicMin; labels only
 // decrease, so the root of a component is its smallest slot, i.e. its first
@@ -1024,7 +1014,7 @@ __global__ __launch_bounds__(256) void k_merge(Plan plan) {
     }
 }
 
-constexpr int kStatSlots = 512;
+constexpr int kStatSlots = 256;
 constexpr uint32_t kEmpty = 0xffffffffu;
 
 // One workgroup per band of 8 rows, after all joins.
@@ -1057,6 +1047,16 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     __shared__ uint32_t s_off[kTile + 1];
     __shared__ uint32_t s_flush[kStatSlots];
     __shared__ uint32_t s_nflush, s_nroots, s_nwalk;
+    // Per band row, the components in the row with their leftmost start and
+    // rightmost end (k_fit's row extremes).
+    __shared__ uint32_t s_rkey[kTile][kRowExtSlots], s_rx0[kTile][kRowExtSlots], s_rx1[kTile][kRowExtSlots];
+    __shared__ uint32_t s_rfull[kTile];
+    for (int e = t; e < kTile * kRowExtSlots; e += blockDim.x) {
+        (&s_rkey[0][0])[e] = kEmpty;
+        (&s_rx0[0][0])[e] = 0xffffffffu;
+        (&s_rx1[0][0])[e] = 0u;
+    }
+    if (t < kTile) s_rfull[t] = 0;
     for (int e = t; e < kStatSlots; e += blockDim.x) {
         hkey[e] = kEmpty;
         harea[e] = 0;
@@ -1243,7 +1243,7 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     const int lane = t & (kWave - 1);
     const auto fold = [&](uint32_t root, uint32_t area, uint32_t x0, uint32_t x1, uint32_t y1) {
         int found = -1;
-        const uint32_t h = (root * 2654435761u) >> 23;
+        const uint32_t h = (root * 2654435761u) >> 24;
         for (int probe = 0; probe < 32; ++probe) {
             const int e = static_cast<int>((h + static_cast<uint32_t>(probe)) & (kStatSlots - 1));
             uint32_t k = hkey[e];
@@ -1264,6 +1264,20 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
             atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 2], x1);
             atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 3], y1);
         }
+    };
+    const auto fold_row = [&](uint32_t root, int r, uint32_t x0, uint32_t x1) {
+        const uint32_t h = (root * 2654435761u) >> 26;
+        for (int probe = 0; probe < kRowExtSlots; ++probe) {
+            const int e = static_cast<int>((h + static_cast<uint32_t>(probe)) & (kRowExtSlots - 1));
+            uint32_t k = s_rkey[r][e];
+            if (k == kEmpty) k = atomicCAS(&s_rkey[r][e], kEmpty, root);
+            if (k == kEmpty || k == root) {
+                atomicMin(&s_rx0[r][e], x0);
+                atomicMax(&s_rx1[r][e], x1);
+                return;
+            }
+        }
+        s_rfull[r] = 1;
     };
     constexpr int kWaveFoldMin = 4;  // fewer shared runs fold one by one
     const auto account = [&](int rowu, uint32_t slotu, uint32_t runu, uint32_t labu) {
@@ -1287,7 +1301,8 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         if (vm) {
             const int leader = __ffsll(static_cast<long long>(vm)) - 1;
             const uint32_t lead_root = wave_lane(root, leader);
-            const bool same = valid && root == lead_root;
+            const int lead_row = static_cast<int>(wave_lane(static_cast<uint32_t>(rowu), leader));
+            const bool same = valid && root == lead_root && rowu == lead_row;
             if (__popcll(__ballot(same)) >= kWaveFoldMin) {
                 const uint32_t area = wave_sum(same ? f_area : 0u);
                 const uint32_t mn = wave_min(same ? x0 : 0xffffffffu);
@@ -1302,7 +1317,10 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
                 }
             }
         }
-        if (folds) fold(f_root, f_area, f_x0, f_x1, f_y1);
+        if (folds) {
+            fold(f_root, f_area, f_x0, f_x1, f_y1);
+            fold_row(f_root, rowu, f_x0, f_x1);
+        }
     };
     // A run that is not a band root points at its band root, whose label
     // is now the component root.
@@ -1350,6 +1368,13 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     }
     __syncthreads();
     if (t == 0) d->band_nroots[ty] = s_nroots;
+    for (int e = t; e < nrows * kRowExtSlots; e += blockDim.x) {
+        const uint32_t key = (&s_rkey[0][0])[e];
+        d->row_ext[static_cast<size_t>(y0) * kRowExtSlots + e] =
+            key == kEmpty ? ~0ull
+                          : (static_cast<uint64_t>(key) << 32) | ((&s_rx0[0][0])[e] << 16) | (&s_rx1[0][0])[e];
+    }
+    if (t < nrows) d->row_ext_full[y0 + t] = s_rfull[t];
     // Occupied table entries are compacted first so every thread flushes
     // about the same number of roots.
     for (int e = t; e < kStatSlots; e += blockDim.x) {
@@ -1731,7 +1756,8 @@ __device__ inline int fit_row(int s, int kept, int rows, int by0) {
 
 // One workgroup (4 waves) per candidate.
 //  1. Row extremes of the component over (at most 1024 sampled) bbox rows,
-//     by binary search in each row's runs (collect_row_extremes :676-715).
+//     one lookup per row in the row's component table (collect_row_extremes
+//     :676-715 scans the row's runs from a binary search to bx0).
 //  2. The extreme boundary point in each of 32 directions: 8 slices of the
 //     boundary x 32 directions, first strictly larger projection wins, so the
 //     result is the first maximum in boundary order as in :1132-1170.
@@ -1755,14 +1781,11 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
     __shared__ uint32_t ppoint[8][kFitDirections];
     __shared__ uint32_t extreme[kFitDirections];
     __shared__ uint32_t hull_pts[kFitDirections];
-    // Probe magnitudes of the refinement; the row-extreme pass uses the same
-    // LDS before the refinement starts.
-    // Rows padded to a multiple of four floats for 16-byte LDS reads.
+    // Probe magnitudes of the refinement, rows padded to a multiple of four
+    // floats for 16-byte LDS reads.
     constexpr int kProbeStride = (kMaxProbes + 3) & ~3;
     __shared__ __align__(16) uint32_t work_lds[4 * kMaxSamples * kProbeStride];
-    static_assert(4 * kMaxSamples * kProbeStride >= 3 * kMaxBoundaryRows + 1, "row-extreme work arrays");
     float (*mags)[kMaxSamples][kProbeStride] = reinterpret_cast<float (*)[kMaxSamples][kProbeStride]>(work_lds);
-    __shared__ uint32_t wave_total[4];
     __shared__ float fit_quad[4][2];
     __shared__ float edge_lines[4][4];
     __shared__ int edge_have[4];
@@ -1771,156 +1794,47 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
 
     const uint32_t root = d->candidates[ci];
     const uint32_t row_cap = static_cast<uint32_t>(d->row_cap);
-    const uint4 bb = *reinterpret_cast<const uint4*>(d->bbox + 4 * static_cast<size_t>(root));
-    const int bx0 = static_cast<int>(bb.x), by0 = static_cast<int>(root / row_cap);
-    const int bx1 = static_cast<int>(bb.z), by1 = static_cast<int>(bb.w);
+    // The root is the component's first run, so its row is the top row.
+    const int by0 = static_cast<int>(root / row_cap);
+    const int by1 = static_cast<int>(d->bbox[4 * static_cast<size_t>(root) + 3]);
     const int rows = by1 - by0 + 1;
     const int kept = min(rows, kMaxBoundaryRows);
     const int points = 2 * kept;
 
-    // Row extremes: (1) per sampled row, a range of runs covering [bx0, bx1]
-    // from the run directory; (2) the component's first and last run in
-    // each range, found by progressive end windows.  Runs are x-ordered and
-    // the component lies inside its bbox, so these give the same extremes
-    // as a scan of the whole row; every row of the bbox holds a run of the
-    // (connected) component.  The component's matches fold into the row's
-    // left/right by LDS atomics.
-    uint32_t* row_slot = work_lds;                        // first run slot of the range
-    uint32_t* row_list = work_lds + kMaxBoundaryRows;     // rows of the current pass
-    uint32_t* row_len = work_lds + 2 * kMaxBoundaryRows + 1;  // runs in the range | window hit bits
-    constexpr uint32_t kLeftHit = 1u << 30, kRightHit = 1u << 31, kLenMask = kLeftHit - 1u;
-    {
-        // Range of each sampled row from the run directory: runs ending
-        // before bx0's half word to runs starting before the half word after
-        // bx1's.  The extra runs at either end lie outside the bbox and never
-        // carry the component's label.
-        const uint32_t dstride = static_cast<uint32_t>(2 * d->words64 + 1);
-        const uint32_t h0 = static_cast<uint32_t>(bx0) >> 5, h1 = (static_cast<uint32_t>(bx1) >> 5) + 1u;
-        for (int s = t; s < kept; s += blockDim.x) {
-            const int y = fit_row(s, kept, rows, by0);
-            const uint32_t* dir = d->run_dir + static_cast<size_t>(y) * dstride;
-            const uint32_t a_lo = dir[h0] & 0xffffu, b_lo = dir[h1] >> 16;
-            row_slot[s] = static_cast<uint32_t>(y) * row_cap + a_lo;
-            row_len[s] = b_lo - a_lo;
-            boundary[2 * s] = 0xffffffffu;  // min x0
-            boundary[2 * s + 1] = 0u;       // max x1 + 1
-        }
-    }
-    __syncthreads();
-    // Progressive end windows: pass p checks the next `width` runs at each
-    // end of every row range not yet resolved (depth runs already checked
-    // at each end; width = kFitWindow << p).  A row is resolved when both
-    // ends have found the component or its whole range has been checked.
-    // Rows of a pass are compacted into row_list and padded to 2 * width
-    // items, so an item's row is a shift away.
-    uint32_t depth = 0, width = kFitWindow;
-    for (int pass = 0;; ++pass) {
-        {
-            uint32_t need = 0, cnt = 0;
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj) {
-                const int s = 4 * t + jj;
-                if (s >= kept) continue;
-                const uint32_t len = row_len[s], n = len & kLenMask;
-                const bool open = pass == 0 || (len & (kLeftHit | kRightHit)) != (kLeftHit | kRightHit);
-                if (n > 2u * depth && open) {
-                    need |= 1u << jj;
-                    ++cnt;
-                }
-            }
-            uint32_t wtot;
-            uint32_t ex = wave_excl_scan(cnt, lane, &wtot);
-            __syncthreads();  // row_list of the previous pass is no longer read
-            if (lane == 0) wave_total[wave] = wtot;
-            __syncthreads();
-            for (int w = 0; w < wave; ++w) ex += wave_total[w];
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj) {
-                if (need & (1u << jj)) row_list[ex++] = static_cast<uint32_t>(4 * t + jj);
-            }
-        }
-        const uint32_t nlist = wave_total[0] + wave_total[1] + wave_total[2] + wave_total[3];
-        __syncthreads();
-        if (nlist == 0) break;
-        const uint32_t per_row = 2u * width;
-        const int shift = 31 - __clz(static_cast<int>(per_row));
-        const uint32_t items = nlist << shift;
-        for (uint32_t j0 = static_cast<uint32_t>(t); j0 < items; j0 += 2u * blockDim.x) {
-            constexpr int kItems = 2;
-            uint32_t slot[kItems], ends[kItems];
-            int sr[kItems];
-#pragma unroll
-            for (int u = 0; u < kItems; ++u) {
-                sr[u] = -1;
-                slot[u] = 0;
-                ends[u] = 0;
-                const uint32_t j = j0 + u * blockDim.x;
-                if (j < items) {
-                    const uint32_t srow = row_list[j >> shift];
-                    const uint32_t q = j & (per_row - 1u);
-                    const uint32_t n = row_len[srow] & kLenMask;
-                    const uint32_t rem = n - 2u * depth;
-                    bool valid = true;
-                    uint32_t k;
-                    if (rem <= per_row) {
-                        valid = q < rem;  // the rest of the range, checked whole
-                        k = depth + q;
-                    } else if (q < width) {
-                        k = depth + q;
-                        ends[u] = kLeftHit;
-                    } else {
-                        k = n - depth - per_row + q;
-                        ends[u] = kRightHit;
-                    }
-                    if (valid) {
-                        sr[u] = static_cast<int>(srow);
-                        slot[u] = row_slot[srow] + k;
-                    }
-                }
-            }
-            uint32_t lab[kItems], run[kItems];
-#pragma unroll
-            for (int u = 0; u < kItems; ++u) {
-                lab[u] = sr[u] >= 0 ? d->label[slot[u]] : ~root;
-                run[u] = sr[u] >= 0 ? d->runs[slot[u]] : 0u;
-            }
-            const uint64_t below = (1ull << lane) - 1ull;
-#pragma unroll
-            for (int u = 0; u < kItems; ++u) {
-                // Consecutive lanes hold consecutive items of a row (in run
-                // order), so a row's items in this wave are a contiguous
-                // lane segment: only the segment's first match (left) and
-                // last match (right) update LDS.  Segment starts: lanes whose
-                // row differs from the lower neighbour's.
-                const bool hit = lab[u] == root;
-                const uint64_t m = __ballot(hit);
-                const uint32_t lower_row = dpp_from_lower(0xfffffffeu, static_cast<uint32_t>(sr[u]));
-                const uint64_t starts = __ballot(static_cast<uint32_t>(sr[u]) != lower_row) | 1ull;
-                if (!hit) continue;
-                const uint64_t seg_lo = starts & (below | (1ull << lane));   // my segment start and below
-                const int s0 = 63 - __clzll(static_cast<long long>(seg_lo));
-                const uint64_t above = starts & ~(below | (1ull << lane));
-                const int s1 = above ? __ffsll(static_cast<long long>(above)) - 1 : 64;
-                const uint64_t seg = (s1 >= 64 ? ~0ull : ((1ull << s1) - 1ull)) & ~((1ull << s0) - 1ull);
-                const bool first = (m & seg & below) == 0;
-                const bool last = (m & seg & ~(below | (1ull << lane))) == 0;
-                if (first) atomicMin(&boundary[2 * sr[u]], static_cast<uint32_t>(run_x0(run[u])));
-                if (last) atomicMax(&boundary[2 * sr[u] + 1], static_cast<uint32_t>(run_x1(run[u]) + 1));
-                // A hit in the left window makes the segment's first hit a
-                // left-window item (items are in run order), and likewise
-                // for the right window and the last hit.
-                if (ends[u] && (first || last)) atomicOr(&row_len[sr[u]], ends[u]);
-            }
-        }
-        __syncthreads();
-        depth += width;
-        width *= 2u;
-    }
+    // Row extremes: the component's leftmost start and rightmost end in
+    // each sampled row, looked up in the row's component table (written by
+    // k_stats from every run of the row); a row whose table overflowed is
+    // scanned whole.  Every row of the bbox holds a run of the (connected)
+    // component.
     for (int s = t; s < kept; s += blockDim.x) {
-        const uint32_t y = static_cast<uint32_t>(fit_row(s, kept, rows, by0)) << 16;
-        const uint32_t l = boundary[2 * s], r = boundary[2 * s + 1];
-        boundary[2 * s] = l == 0xffffffffu ? 0xffffffffu : (l | y);
-        boundary[2 * s + 1] = r == 0u ? 0xffffffffu : ((r - 1u) | y);
+        const int y = fit_row(s, kept, rows, by0);
+        uint32_t lo = 0xffffffffu, hi = 0u;  // min x0, max x1 + 1 (0: none)
+        if (!d->row_ext_full[y]) {
+            const uint64_t* tab = d->row_ext + static_cast<size_t>(y) * kRowExtSlots;
+            const uint32_t h = (root * 2654435761u) >> 26;
+            for (int probe = 0; probe < kRowExtSlots; ++probe) {
+                const uint64_t e = tab[(h + static_cast<uint32_t>(probe)) & (kRowExtSlots - 1)];
+                if (static_cast<uint32_t>(e >> 32) == root) {
+                    lo = static_cast<uint32_t>(e >> 16) & 0xffffu;
+                    hi = (static_cast<uint32_t>(e) & 0xffffu) + 1u;
+                    break;
+                }
+                if (e == ~0ull) break;
+            }
+        } else {
+            const uint32_t base = static_cast<uint32_t>(y) * row_cap;
+            const uint32_t n = d->row_count[y];
+            for (uint32_t k = 0; k < n; ++k) {
+                if (d->label[base + k] != root) continue;
+                const uint32_t run = d->runs[base + k];
+                lo = min(lo, static_cast<uint32_t>(run_x0(run)));
+                hi = max(hi, static_cast<uint32_t>(run_x1(run)) + 1u);
+            }
+        }
+        // Boundary points x | y << 16, left then right; ~0 when absent.
+        const uint32_t yy = static_cast<uint32_t>(y) << 16;
+        boundary[2 * s] = lo == 0xffffffffu ? 0xffffffffu : (lo | yy);
+        boundary[2 * s + 1] = hi == 0u ? 0xffffffffu : ((hi - 1u) | yy);
     }
     __syncthreads();
 
