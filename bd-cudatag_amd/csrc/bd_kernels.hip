@@ -1294,21 +1294,37 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
                 d->roots[band_base + atomicAdd(&s_nroots, 1u)] = root;
             }
         }
-        // What this lane folds: its own run, or the leader's wave total.
-        uint32_t f_root = root, f_area = x1 - x0 + 1, f_x0 = x0, f_x1 = x1, f_y1 = y;
-        bool folds = valid;
+        // What this lane folds: its own run, or the leader's wave total for
+        // the root table; for the row tables, its own run, or the first of
+        // the root's runs in each row of the wave with the row's last one's
+        // end (a row's runs are x-ordered and its lanes contiguous).
+        uint32_t f_area = x1 - x0 + 1, f_x0 = x0, f_x1 = x1, f_y1 = y, row_x1 = x1;
+        bool folds = valid, folds_row = valid;
         const uint64_t vm = __ballot(valid);
         if (vm) {
             const int leader = __ffsll(static_cast<long long>(vm)) - 1;
             const uint32_t lead_root = wave_lane(root, leader);
-            const int lead_row = static_cast<int>(wave_lane(static_cast<uint32_t>(rowu), leader));
-            const bool same = valid && root == lead_root && rowu == lead_row;
-            if (__popcll(__ballot(same)) >= kWaveFoldMin) {
+            const bool same = valid && root == lead_root;
+            const uint64_t sm = __ballot(same);
+            if (__popcll(sm) >= kWaveFoldMin) {
                 const uint32_t area = wave_sum(same ? f_area : 0u);
                 const uint32_t mn = wave_min(same ? x0 : 0xffffffffu);
                 const uint32_t mx = wave_max(same ? x1 : 0u);
                 const uint32_t my = wave_max(same ? y : 0u);
+                const uint32_t lower_row = dpp_from_lower(0xfffffffeu, static_cast<uint32_t>(rowu));
+                const uint64_t starts = __ballot(static_cast<uint32_t>(rowu) != lower_row) | 1ull;
+                const uint64_t below = (1ull << lane) - 1ull;
+                const uint64_t seg_lo = starts & (below | (1ull << lane));
+                const int s0 = 63 - __clzll(static_cast<long long>(seg_lo));
+                const uint64_t above = starts & ~(below | (1ull << lane));
+                const int s1 = above ? __ffsll(static_cast<long long>(above)) - 1 : 64;
+                const uint64_t seg = (s1 >= 64 ? ~0ull : ((1ull << s1) - 1ull)) & ~((1ull << s0) - 1ull);
+                const uint64_t mine = sm & seg;
+                const int last = mine ? 63 - __clzll(static_cast<long long>(mine)) : lane;
+                const uint32_t last_x1 = static_cast<uint32_t>(__shfl(static_cast<int>(x1), last));
                 folds = valid && (!same || lane == leader);
+                folds_row = valid && (!same || (mine & below) == 0);
+                if (same) row_x1 = last_x1;
                 if (lane == leader) {
                     f_area = area;
                     f_x0 = mn;
@@ -1317,10 +1333,8 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
                 }
             }
         }
-        if (folds) {
-            fold(f_root, f_area, f_x0, f_x1, f_y1);
-            fold_row(f_root, rowu, f_x0, f_x1);
-        }
+        if (folds) fold(root, f_area, f_x0, f_x1, f_y1);
+        if (folds_row) fold_row(root, rowu, x0, row_x1);
     };
     // A run that is not a band root points at its band root, whose label
     // is now the component root.

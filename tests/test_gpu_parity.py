@@ -12,7 +12,7 @@ import pytest
 import bdcudatag
 import pyoracle
 import scenes
-from parity import compare_frame, gpu_stage_dump
+from parity import compare_frame, gpu_stage_dump, oracle_components
 
 pytestmark = pytest.mark.gpu
 
@@ -144,6 +144,21 @@ def test_candidate_radix_select():
     assert int(o["counts"][0]) > 2048
     bad = compare_frame(g, o, dets, det.stats())
     assert not bad, bad
+
+
+def test_row_table_overflow():
+    # Eighty 2-px vertical bars beside a tag put more components in each of
+    # the tag's rows than a row's component table holds (64), so the quad
+    # fit takes the whole-row scan for those rows.
+    img = scenes.render_tag(0, 3, 640, 480, scenes.make_pose(320, 240, 170, 0.0))
+    for x in list(range(4, 200, 5)) + list(range(444, 636, 5)):
+        img[100:380, x:x + 2] = 20
+    det, dets, g, o = run_both(img, 0, **KA)
+    comps = oracle_components(o)
+    assert min(len({c[2] for c in comps[y]}) for y in range(160, 320)) > 64
+    bad = compare_frame(g, o, dets, det.stats())
+    assert not bad, bad
+    assert [d["id"] for d in dets] == [3]
 
 
 def test_strided_host_frame():
