@@ -104,7 +104,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     // Arena layout: every frame's buffers, then the shared scalar and output
     // blocks (contiguous so one copy returns them for all frames).
     struct Off {
-        size_t tmin, tmax, tmean, tthr, trlo, trhi, roi, hist, bits, rowc, rdir, runs, label, area, bbox,
+        size_t tmin, tmax, tmean, tthr, trlo, trhi, roi, hist, bits, rowc, runs, label, area, bbox,
             keys, roots, nroots, rext, rxfull, cands, quads, hquads, dec;
     };
     std::vector<Off> off(nf);
@@ -137,7 +137,6 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         o.hist = take(320 * 4);
         o.bits = take(static_cast<size_t>(H) * w64 * 8);
         o.rowc = take(static_cast<size_t>(H) * 4);
-        o.rdir = take(static_cast<size_t>(H) * (2 * w64 + 1) * 4);
         o.runs = take(slots * 4);
         o.label = take(slots * 4);
         o.area = take(slots * 4);
@@ -257,7 +256,6 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         d.scal = d_scal_ + static_cast<size_t>(f) * S_COUNT;
         d.bits = reinterpret_cast<uint64_t*>(base + o.bits);
         d.row_count = reinterpret_cast<uint32_t*>(base + o.rowc);
-        d.run_dir = reinterpret_cast<uint32_t*>(base + o.rdir);
         d.runs = reinterpret_cast<uint32_t*>(base + o.runs);
         d.label = reinterpret_cast<uint32_t*>(base + o.label);
         d.area = reinterpret_cast<uint32_t*>(base + o.area);

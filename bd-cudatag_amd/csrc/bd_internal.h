@@ -92,8 +92,6 @@ struct FrameDesc {
     uint32_t* scal;       // S_COUNT scalars
     uint64_t* bits;       // height * words64
     uint32_t* row_count;  // height
-    uint32_t* run_dir;    // height * (2 * words64 + 1): per 32-px half word h,
-                          // runs ending before 32h | runs starting before 32h << 16
     uint32_t* runs;       // height * row_cap, x0 | x1 << 16
     uint32_t* label;      // height * row_cap
     uint32_t* area;       // height * row_cap

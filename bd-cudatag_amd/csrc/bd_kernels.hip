@@ -561,8 +561,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             d->bits[static_cast<size_t>(y0 + r) * w64 + (i - r * w64)] = ~0ull;
         }
         if (t < nrows) d->row_count[y0 + t] = 0;
-        const int ds = 2 * w64 + 1;
-        for (int i = t; i < nrows * ds; i += blockDim.x) d->run_dir[static_cast<size_t>(y0) * ds + i] = 0;
         return;
     }
     if (t < kTile) s_cnt[t] = 0;
@@ -701,10 +699,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             uint32_t tot_s, tot_e;
             uint32_t ks = carry_s + wave_excl_scan(__popc(starts), lane, &tot_s);
             uint32_t ke = carry_e + wave_excl_scan(__popc(ends), lane, &tot_e);
-            if (hw < nh) {
-                d->run_dir[static_cast<size_t>(y) * (nh + 1) + hw] = ke | (ks << 16);
-                end_before[r][hw] = static_cast<uint16_t>(ke);
-            }
+            if (hw < nh) end_before[r][hw] = static_cast<uint16_t>(ke);
             const int xb = hw * 32;
             while (starts | ends) {
                 if (starts) {
@@ -736,7 +731,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             carry_s += tot_s;
             carry_e += tot_e;
         }
-        if (lane == 0) d->run_dir[static_cast<size_t>(y) * (nh + 1) + nh] = carry_e | (carry_s << 16);
     }
     if (t == 0) roi[3] = local ? 1 : 0;
     if (!local) return;
