@@ -319,7 +319,7 @@ class Batch:
         return {k: getattr(t, k) for k in TUNE_KEYS}
 
     def set_tune(self, fit_grid: int, decode_grid: int, head_ranks: int, streams: int = 2,
-                 decode_waves: int = 8, chunk_px: int = 64, head_area: int = 2000) -> None:
+                 decode_waves: int = 8, chunk_px: int = 64, head_area: int = 1000) -> None:
         t = Tune(fit_grid, decode_grid, head_ranks, streams, decode_waves, chunk_px, head_area)
         rc = lib().bd_batch_set_tune(self.handle, ctypes.byref(t))
         if rc != BD_OK:
@@ -345,7 +345,7 @@ class Batch:
             if key in table:
                 t = table[key]
                 self.set_tune(t["fit_grid"], t["decode_grid"], t["head_ranks"], t.get("streams", 2),
-                              t.get("decode_waves", 8), t.get("chunk_px", 64), t.get("head_area", 2000))
+                              t.get("decode_waves", 8), t.get("chunk_px", 64), t.get("head_area", 1000))
                 return dict(t, cached=True)
         best = Tune()
         ms = ctypes.c_float()

@@ -790,7 +790,7 @@ bool Engine::autotune(int reps, Tune* best, float* best_ms, std::string* error) 
         t.head_ranks = h;
         if (!consider(t)) return false;
     }
-    for (const int32_t a : {1000, 2000, 4000, 8000, INT32_MAX}) {
+    for (const int32_t a : {500, 1000, 2000, 4000, INT32_MAX}) {
         if (a == cur.head_area) continue;
         Tune t = cur;
         t.head_area = a;

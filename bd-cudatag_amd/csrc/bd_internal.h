@@ -137,7 +137,7 @@ struct Tune {
     int32_t streams = 2;
     int32_t decode_waves = 8;    // 4, 8 or 16 waves per decode workgroup
     int32_t chunk_px = 64;       // pixels packed per k_binarize_runs item: 64 or 128
-    int32_t head_area = 2000;    // pixels; larger candidates join the head region
+    int32_t head_area = 1000;    // pixels; larger candidates join the head region
 };
 
 struct Plan {

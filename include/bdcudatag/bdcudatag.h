@@ -252,7 +252,7 @@ typedef struct bd_tune {
                              0 in bd_batch_set_tune selects the default) */
     int32_t head_area;    /* candidates of at least this many pixels are also scheduled first
                              (at most 32 per frame), >= 1; INT32_MAX turns this off (default
-                             2000; 0 in bd_batch_set_tune selects the default) */
+                             1000; 0 in bd_batch_set_tune selects the default) */
 } bd_tune;
 
 BD_API int32_t bd_batch_get_tune(bd_batch* batch, bd_tune* out);

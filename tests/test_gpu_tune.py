@@ -25,7 +25,7 @@ def test_launch_parameters_keep_results(tmp_path):
     imgs = [scenes.scene(2, w, h, 40 + i)[0] for i, (w, h) in enumerate(sizes)]
     batch = bdcudatag.Batch(0, sizes)
     assert batch.get_tune() == {"fit_grid": 1280, "decode_grid": 1024, "head_ranks": 2, "streams": 2,
-                                "decode_waves": 8, "chunk_px": 64, "head_area": 2000}
+                                "decode_waves": 8, "chunk_px": 64, "head_area": 1000}
     ref = batch.detect(imgs)
     assert sum(len(r) for r in ref) > 0
     # head_area 1 puts every candidate (up to 32 per frame) in the head
