@@ -460,18 +460,20 @@ __device__ inline uint32_t lds_find(uint32_t* lab, uint32_t x) {
     return x;
 }
 
-__device__ inline void lds_union(uint32_t* lab, uint32_t a, uint32_t b) {
+// Joins the sets of a and b; returns the joined set's root at the time of
+// the join (an ancestor of both from then on).
+__device__ inline uint32_t lds_union(uint32_t* lab, uint32_t a, uint32_t b) {
     for (;;) {
         a = lds_find(lab, a);
         b = lds_find(lab, b);
-        if (a == b) return;
+        if (a == b) return a;
         if (a > b) {
             const uint32_t tmp = a;
             a = b;
             b = tmp;
         }
         const uint32_t old = atomicMin(&lab[b], a);
-        if (old == b) return;
+        if (old == b) return a;
         b = old;
     }
 }
@@ -701,30 +703,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             uint32_t ke = carry_e + wave_excl_scan(__popc(ends), lane, &tot_e);
             if (hw < nh) end_before[r][hw] = static_cast<uint16_t>(ke);
             const int xb = hw * 32;
-            while (starts | ends) {
-                if (starts) {
-                    const int b = __ffs(starts) - 1;
+            if (local) {
+                // A run's start and end fill the low and high halves of its
+                // LDS word.
+                while (starts | ends) {
+                    const bool has_s = starts != 0, has_e = ends != 0;
+                    const int bs = __ffs(starts) - 1, be = __ffs(ends) - 1;
                     starts &= starts - 1;
-                    const uint32_t k = ks++;
-                    const uint32_t slot = base + k;
-                    if (local) {
-                        lds16[2 * (lbase + k)] = static_cast<uint16_t>(xb + b);
-                    } else {
+                    ends &= ends - 1;
+                    if (has_s) lds16[2 * (lbase + ks)] = static_cast<uint16_t>(xb + bs);
+                    if (has_e) lds16[2 * (lbase + ke) + 1] = static_cast<uint16_t>(xb + be);
+                    ks += has_s ? 1u : 0u;
+                    ke += has_e ? 1u : 0u;
+                }
+            } else {
+                while (starts | ends) {
+                    if (starts) {
+                        const int b = __ffs(starts) - 1;
+                        starts &= starts - 1;
+                        const uint32_t slot = base + ks++;
                         runs16[2 * slot] = static_cast<uint16_t>(xb + b);
                         d->label[slot] = slot;
                         d->area[slot] = 0;
                         uint4* bb = reinterpret_cast<uint4*>(d->bbox + 4 * static_cast<size_t>(slot));
                         *bb = make_uint4(0xffffffffu, 0xffffffffu, 0u, 0u);
                     }
-                }
-                if (ends) {
-                    const int b = __ffs(ends) - 1;
-                    ends &= ends - 1;
-                    const uint32_t k = ke++;
-                    if (local) {
-                        lds16[2 * (lbase + k) + 1] = static_cast<uint16_t>(xb + b);
-                    } else {
-                        runs16[2 * (base + k) + 1] = static_cast<uint16_t>((xb + b) | 0x8000);
+                    if (ends) {
+                        const int b = __ffs(ends) - 1;
+                        ends &= ends - 1;
+                        runs16[2 * (base + ke++) + 1] = static_cast<uint16_t>((xb + b) | 0x8000);
                     }
                 }
             }
@@ -783,9 +790,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         const int r = row_of(i);
         const int beg = static_cast<int>(s_off[r]);
         const int x1 = run_x1(b_run[i]);
+        // anc: an ancestor of i, its set's root after the last join; a run
+        // above whose label is anc is already in the set.
+        uint32_t anc = i;
         for (int k = first; k < beg; ++k) {
             if (run_x0(b_run[k]) > x1 + 1) break;
-            lds_union(b_lab, i, static_cast<uint32_t>(k));
+            if (b_lab[k] == anc) continue;
+            anc = lds_union(b_lab, anc, static_cast<uint32_t>(k));
         }
     };
 #pragma unroll
