@@ -1246,28 +1246,35 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     // the first lane's root are reduced across the wave (DPP) and folded
     // into the table once; the other runs fold one by one.
     const int lane = t & (kWave - 1);
-    const auto fold = [&](uint32_t root, uint32_t area, uint32_t x0, uint32_t x1, uint32_t y1) {
-        int found = -1;
+    // Area per root here; a root's x extent and last row are folded from
+    // the row tables at the end (or straight into global memory for a run
+    // whose row table is full).  A root that finds no table slot goes to
+    // global memory too.
+    const auto root_slot = [&](uint32_t root, bool insert) {
         const uint32_t h = (root * 2654435761u) >> 24;
         for (int probe = 0; probe < 32; ++probe) {
             const int e = static_cast<int>((h + static_cast<uint32_t>(probe)) & (kStatSlots - 1));
             uint32_t k = hkey[e];
-            if (k == kEmpty) k = atomicCAS(&hkey[e], kEmpty, root);
-            if (k == kEmpty || k == root) {
-                found = e;
-                break;
-            }
+            if (k == kEmpty && insert) k = atomicCAS(&hkey[e], kEmpty, root);
+            if (k == root || (k == kEmpty && insert)) return e;
+            if (k == kEmpty) return -1;
         }
-        if (found >= 0) {
-            atomicAdd(&harea[found], area);
-            atomicMin(&hx0[found], x0);
-            atomicMax(&hx1[found], x1);
-            atomicMax(&hy1[found], y1);
+        return -1;
+    };
+    const auto fold = [&](uint32_t root, uint32_t area) {
+        const int e = root_slot(root, true);
+        if (e >= 0) atomicAdd(&harea[e], area); else atomicAdd(&d->area[root], area);
+    };
+    const auto fold_extent = [&](uint32_t root, uint32_t x0, uint32_t x1, uint32_t y) {
+        const int e = root_slot(root, false);
+        if (e >= 0) {
+            atomicMin(&hx0[e], x0);
+            atomicMax(&hx1[e], x1);
+            atomicMax(&hy1[e], y);
         } else {
-            atomicAdd(&d->area[root], area);
             atomicMin(&d->bbox[4 * static_cast<size_t>(root) + 0], x0);
             atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 2], x1);
-            atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 3], y1);
+            atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 3], y);
         }
     };
     const auto fold_row = [&](uint32_t root, int r, uint32_t x0, uint32_t x1) {
@@ -1283,6 +1290,9 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
             }
         }
         s_rfull[r] = 1;
+        atomicMin(&d->bbox[4 * static_cast<size_t>(root) + 0], x0);
+        atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 2], x1);
+        atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 3], static_cast<uint32_t>(y0 + r));
     };
     constexpr int kWaveFoldMin = 4;  // fewer shared runs fold one by one
     const auto account = [&](int rowu, uint32_t slotu, uint32_t runu, uint32_t labu) {
@@ -1303,7 +1313,7 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         // the root table; for the row tables, its own run, or the first of
         // the root's runs in each row of the wave with the row's last one's
         // end (a row's runs are x-ordered and its lanes contiguous).
-        uint32_t f_area = x1 - x0 + 1, f_x0 = x0, f_x1 = x1, f_y1 = y, row_x1 = x1;
+        uint32_t f_area = x1 - x0 + 1, row_x1 = x1;
         bool folds = valid, folds_row = valid;
         const uint64_t vm = __ballot(valid);
         if (vm) {
@@ -1313,9 +1323,6 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
             const uint64_t sm = __ballot(same);
             if (__popcll(sm) >= kWaveFoldMin) {
                 const uint32_t area = wave_sum(same ? f_area : 0u);
-                const uint32_t mn = wave_min(same ? x0 : 0xffffffffu);
-                const uint32_t mx = wave_max(same ? x1 : 0u);
-                const uint32_t my = wave_max(same ? y : 0u);
                 const uint32_t lower_row = dpp_from_lower(0xfffffffeu, static_cast<uint32_t>(rowu));
                 const uint64_t starts = __ballot(static_cast<uint32_t>(rowu) != lower_row) | 1ull;
                 const uint64_t below = (1ull << lane) - 1ull;
@@ -1330,15 +1337,10 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
                 folds = valid && (!same || lane == leader);
                 folds_row = valid && (!same || (mine & below) == 0);
                 if (same) row_x1 = last_x1;
-                if (lane == leader) {
-                    f_area = area;
-                    f_x0 = mn;
-                    f_x1 = mx;
-                    f_y1 = my;
-                }
+                if (lane == leader) f_area = area;
             }
         }
-        if (folds) fold(root, f_area, f_x0, f_x1, f_y1);
+        if (folds) fold(root, f_area);
         if (folds_row) fold_row(root, rowu, x0, row_x1);
     };
     // A run that is not a band root points at its band root, whose label
@@ -1389,11 +1391,13 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     if (t == 0) d->band_nroots[ty] = s_nroots;
     for (int e = t; e < nrows * kRowExtSlots; e += blockDim.x) {
         const uint32_t key = (&s_rkey[0][0])[e];
+        const uint32_t rx0 = (&s_rx0[0][0])[e], rx1 = (&s_rx1[0][0])[e];
         d->row_ext[static_cast<size_t>(y0) * kRowExtSlots + e] =
-            key == kEmpty ? ~0ull
-                          : (static_cast<uint64_t>(key) << 32) | ((&s_rx0[0][0])[e] << 16) | (&s_rx1[0][0])[e];
+            key == kEmpty ? ~0ull : (static_cast<uint64_t>(key) << 32) | (rx0 << 16) | rx1;
+        if (key != kEmpty) fold_extent(key, rx0, rx1, static_cast<uint32_t>(y0 + e / kRowExtSlots));
     }
     if (t < nrows) d->row_ext_full[y0 + t] = s_rfull[t];
+    __syncthreads();
     // Occupied table entries are compacted first so every thread flushes
     // about the same number of roots.
     for (int e = t; e < kStatSlots; e += blockDim.x) {
