@@ -117,6 +117,9 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     };
     input_off_.assign(nf, 0);
     input_bytes_ = 0;
+    int widest_frame = 0;
+    for (int f = 0; f < nf; ++f) widest_frame = std::max(widest_frame, w_[f]);
+    const int row_ext_log2 = row_ext_log2_for_width(widest_frame);
     for (int f = 0; f < nf; ++f) {
         const int W = w_[f], H = h_[f];
         const int tw = (W + kTile - 1) / kTile, th = (H + kTile - 1) / kTile;
@@ -144,7 +147,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         o.keys = take(slots * 8);
         o.roots = take(slots * 4);
         o.nroots = take(static_cast<size_t>(th) * 4);
-        o.rext = take(static_cast<size_t>(H) * kRowExtSlots * 8);
+        o.rext = take((static_cast<size_t>(H) << row_ext_log2) * 8);
         o.rxfull = take(static_cast<size_t>(H) * 4);
         o.cands = take(static_cast<size_t>(maxc[f]) * 4);
         o.quads = take(static_cast<size_t>(maxc[f]) * 32);
@@ -234,6 +237,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         d.tile_h = (H + kTile - 1) / kTile;
         d.words64 = (W + 63) / 64;
         d.row_cap = W / 2 + 2;
+        d.row_ext_log2 = row_ext_log2;
         d.max_candidates = maxc[f];
         d.max_detections = max_det_;
         d.dim = bits + 2;
@@ -328,6 +332,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         return band_runs_for_width(widest);
     };
     plan_host_.band_runs = band_runs_of(0, nf);
+    plan_host_.row_ext_log2 = row_ext_log2;
     for (int k = 0; k < SH_COUNT; ++k) plan_host_.blocks[k] = prefix[k * (nf + 1) + nf];
     plan_dev_ = plan_host_;
     plan_dev_.frames = d_desc_dev_;

@@ -24,7 +24,6 @@ constexpr int kFitIterations = 2;
 constexpr int kFitProbeHalf = 24;
 constexpr int kMaxProbes = 2 * kFitProbeHalf + 1;
 constexpr int kMaxSamples = 16;
-constexpr int kRowExtSlots = 64;     // component slots per row of the row-extreme tables
 constexpr int kMaxCodes = 587;       // largest codebook (tag36h11)
 
 // Device frames are addressed by 32-bit byte offsets from their base in the
@@ -55,6 +54,18 @@ inline int band_runs_for_width(int width) {
     const int want = (width * 8 / 5 + 511) / 512 * 512;
     return want < kBandRunsMin ? kBandRunsMin : want > kBandRunsMax ? kBandRunsMax : want;
 }
+// Component slots per row of the row-extreme tables (log2), one value per
+// batch from its widest frame: the least power of two of at least width / 20,
+// 64 to 256.  Rows of noisy eight-tag scenes hold 8.7 components on average
+// and 44 at most at 1280 px (64 slots), 28 and 48 at 1920 px (128), 68 and
+// 100 at 4096 px (256); a row with more components than slots is scanned
+// whole by k_fit.  k_stats is compiled once per size.
+constexpr int kRowExtLog2Min = 6, kRowExtLog2Max = 8;
+inline int row_ext_log2_for_width(int width) {
+    int lg = kRowExtLog2Min;
+    while (lg < kRowExtLog2Max && (20 << lg) < width) ++lg;
+    return lg;
+}
 
 // Per-frame scalar block (u32 each).  Accumulators are cleared by the kernel
 // that consumes them, so a replayed graph needs no memset nodes.
@@ -76,6 +87,7 @@ struct FrameDesc {
     int32_t active;  // 0: this position carries no frame in the current launch
     int32_t width, height, stride;
     int32_t tile_w, tile_h, words64, row_cap;
+    int32_t row_ext_log2;  // log2 of the component slots per row of row_ext
     int32_t max_candidates, max_detections;
     int32_t dim, code_count, max_hamming;
     int32_t min_cluster, min_tag, refine;
@@ -99,7 +111,7 @@ struct FrameDesc {
     uint64_t* cand_keys;  // height * row_cap
     uint32_t* roots;      // height * row_cap: band b's component roots from b * 8 * row_cap
     uint32_t* band_nroots;  // tile_h: roots listed per band
-    uint64_t* row_ext;    // height * kRowExtSlots: per row, a hash table of the
+    uint64_t* row_ext;    // height << row_ext_log2: per row, a hash table of the
                           // components in the row: root << 32 | x0 << 16 | x1
                           // (leftmost start, rightmost end), ~0 when empty
     uint32_t* row_ext_full;  // height: 1 when the row's table overflowed
@@ -147,6 +159,7 @@ struct Plan {
     int32_t nframes;
     int32_t work_cap;       // sum of max_candidates over frames
     int32_t band_runs;      // LDS band capacity of k_binarize_runs (runs)
+    int32_t row_ext_log2;   // the batch's component slots per row (log2)
     Tune tune;
     int32_t blocks[SH_COUNT];
 };

@@ -1034,7 +1034,9 @@ constexpr uint32_t kEmpty = 0xffffffffu;
 //     root's counters (reference k_region_stats :562-586 does one atomic per
 //     run).  The root is the component's first run in raster order, so its
 //     row is the component's top row.
+template <int kRowLog2>
 __global__ __launch_bounds__(256) void k_stats(Plan plan) {
+    constexpr int kRowSlots = 1 << kRowLog2;
     int ty;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
     if (!d->active) return;
@@ -1054,9 +1056,9 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     __shared__ uint32_t s_nflush, s_nroots, s_nwalk;
     // Per band row, the components in the row with their leftmost start and
     // rightmost end (k_fit's row extremes).
-    __shared__ uint32_t s_rkey[kTile][kRowExtSlots], s_rx0[kTile][kRowExtSlots], s_rx1[kTile][kRowExtSlots];
+    __shared__ uint32_t s_rkey[kTile][kRowSlots], s_rx0[kTile][kRowSlots], s_rx1[kTile][kRowSlots];
     __shared__ uint32_t s_rfull[kTile];
-    for (int e = t; e < kTile * kRowExtSlots; e += blockDim.x) {
+    for (int e = t; e < kTile * kRowSlots; e += blockDim.x) {
         (&s_rkey[0][0])[e] = kEmpty;
         (&s_rx0[0][0])[e] = 0xffffffffu;
         (&s_rx1[0][0])[e] = 0u;
@@ -1278,9 +1280,9 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         }
     };
     const auto fold_row = [&](uint32_t root, int r, uint32_t x0, uint32_t x1) {
-        const uint32_t h = (root * 2654435761u) >> 26;
-        for (int probe = 0; probe < kRowExtSlots; ++probe) {
-            const int e = static_cast<int>((h + static_cast<uint32_t>(probe)) & (kRowExtSlots - 1));
+        const uint32_t h = (root * 2654435761u) >> (32 - kRowLog2);
+        for (int probe = 0; probe < kRowSlots; ++probe) {
+            const int e = static_cast<int>((h + static_cast<uint32_t>(probe)) & (kRowSlots - 1));
             uint32_t k = s_rkey[r][e];
             if (k == kEmpty) k = atomicCAS(&s_rkey[r][e], kEmpty, root);
             if (k == kEmpty || k == root) {
@@ -1389,12 +1391,12 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     }
     __syncthreads();
     if (t == 0) d->band_nroots[ty] = s_nroots;
-    for (int e = t; e < nrows * kRowExtSlots; e += blockDim.x) {
+    for (int e = t; e < nrows * kRowSlots; e += blockDim.x) {
         const uint32_t key = (&s_rkey[0][0])[e];
         const uint32_t rx0 = (&s_rx0[0][0])[e], rx1 = (&s_rx1[0][0])[e];
-        d->row_ext[static_cast<size_t>(y0) * kRowExtSlots + e] =
+        d->row_ext[static_cast<size_t>(y0) * kRowSlots + e] =
             key == kEmpty ? ~0ull : (static_cast<uint64_t>(key) << 32) | (rx0 << 16) | rx1;
-        if (key != kEmpty) fold_extent(key, rx0, rx1, static_cast<uint32_t>(y0 + e / kRowExtSlots));
+        if (key != kEmpty) fold_extent(key, rx0, rx1, static_cast<uint32_t>(y0 + e / kRowSlots));
     }
     if (t < nrows) d->row_ext_full[y0 + t] = s_rfull[t];
     __syncthreads();
@@ -1833,10 +1835,11 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
         const int y = fit_row(s, kept, rows, by0);
         uint32_t lo = 0xffffffffu, hi = 0u;  // min x0, max x1 + 1 (0: none)
         if (!d->row_ext_full[y]) {
-            const uint64_t* tab = d->row_ext + static_cast<size_t>(y) * kRowExtSlots;
-            const uint32_t h = (root * 2654435761u) >> 26;
-            for (int probe = 0; probe < kRowExtSlots; ++probe) {
-                const uint64_t e = tab[(h + static_cast<uint32_t>(probe)) & (kRowExtSlots - 1)];
+            const int rlg = d->row_ext_log2, rs = 1 << rlg;
+            const uint64_t* tab = d->row_ext + (static_cast<size_t>(y) << rlg);
+            const uint32_t h = (root * 2654435761u) >> (32 - rlg);
+            for (int probe = 0; probe < rs; ++probe) {
+                const uint64_t e = tab[(h + static_cast<uint32_t>(probe)) & (rs - 1)];
                 if (static_cast<uint32_t>(e >> 32) == root) {
                     lo = static_cast<uint32_t>(e >> 16) & 0xffffu;
                     hi = (static_cast<uint32_t>(e) & 0xffffu) + 1u;
@@ -2679,7 +2682,17 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
                                    static_cast<size_t>(plan.band_runs) * 8, stream, plan);
             break;
         case 3: hipLaunchKernelGGL(k_merge, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
-        case 4: hipLaunchKernelGGL(k_stats, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
+        case 4: {
+            const dim3 grid(plan.blocks[SH_TILE_ROW]);
+            if (plan.row_ext_log2 <= 6) {
+                hipLaunchKernelGGL(k_stats<6>, grid, dim3(256), 0, stream, plan);
+            } else if (plan.row_ext_log2 == 7) {
+                hipLaunchKernelGGL(k_stats<7>, grid, dim3(256), 0, stream, plan);
+            } else {
+                hipLaunchKernelGGL(k_stats<8>, grid, dim3(256), 0, stream, plan);
+            }
+            break;
+        }
         case 5: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
         case 6: hipLaunchKernelGGL(k_select, dim3(nf), dim3(256), 0, stream, plan); break;
         case 7: hipLaunchKernelGGL(k_fit, dim3(min(plan.work_cap, plan.tune.fit_grid)), dim3(256), 0, stream, plan); break;
