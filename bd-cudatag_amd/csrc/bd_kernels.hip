@@ -7,10 +7,12 @@
 //                     row, band-local union-find in LDS                    (b)
 //   k_merge           8-connected union across band boundaries            (b)
 //   k_stats           band roots resolved to component roots, root labels,
-//                     component area and bounding box                     (b)
+//                     component area and bounding box, per-row component
+//                     tables (leftmost start, rightmost end)               (b)
 //   k_collect         tag-shaped component filter                         (c)
 //   k_select          largest-first candidate order (LDS bitonic sort)    (c)
-//   k_fit             row extremes, hull probe, quad, edge refinement     (d)
+//   k_fit             row extremes (table lookups), hull probe, quad,
+//                     edge refinement                                     (d)
 //   k_decode          4 rotations x {refined, hull} decode                (e)
 //   k_finalize        de-duplication, stable id sort, FP64 homography     (e)
 //
@@ -1027,13 +1029,16 @@ constexpr uint32_t kEmpty = 0xffffffffu;
 //     a band joined in global memory) walk to their component root, which
 //     is written to their label.
 //  2. Every run takes its root (its own label if it is a band root, else its
-//     band root's label) and writes it to its label; the run's length and
-//     extent go into an LDS
-//     table keyed by root, flushed with one set of global atomics per (band,
-//     root), so a component with thousands of runs does not serialise on its
-//     root's counters (reference k_region_stats :562-586 does one atomic per
-//     run).  The root is the component's first run in raster order, so its
-//     row is the component's top row.
+//     band root's label) and writes it to its label; the run's length goes
+//     into an LDS table keyed by root and its extent into a per-row LDS table
+//     keyed by root (the row's components with their leftmost start and
+//     rightmost end).  The root table takes each root's x extent and last row
+//     from the row tables and is flushed with one set of global atomics per
+//     (band, root), so a component with thousands of runs does not serialise
+//     on its root's counters (reference k_region_stats :562-586 does one
+//     atomic per run); the row tables are written out for k_fit's row
+//     extremes (1 << kRowLog2 slots per row).  The root is the component's
+//     first run in raster order, so its row is the component's top row.
 template <int kRowLog2>
 __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     constexpr int kRowSlots = 1 << kRowLog2;
