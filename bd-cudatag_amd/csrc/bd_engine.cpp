@@ -377,8 +377,7 @@ hipError_t Engine::enqueue_lane(bool device_path, int lane) {
     hipStream_t s = lane == 1 ? stream2_ : stream_;
     hipError_t st;
     if (device_path) {
-        st = hipMemcpyAsync(d_desc_dev_ + first, h_desc_dev_ + first, static_cast<size_t>(n) * sizeof(FrameDesc),
-                            hipMemcpyHostToDevice, s);
+        st = hipSuccess;  // descriptors: upload_descriptors() before the launch
     } else {
         const size_t begin = input_off_[first];
         const size_t end = first + n < nf ? input_off_[first + n] : input_bytes_;
@@ -394,7 +393,39 @@ hipError_t Engine::enqueue_lane(bool device_path, int lane) {
     return st;
 }
 
+// The device path's descriptors change only in frame pointer, stride and
+// active flag; they are copied (whole, before the lanes fork) only when one
+// of these differs from the last upload, so repeated calls on the same
+// buffers launch the graphs alone.  The pinned staging is not written again
+// before the launch completes (callers wait for it).
+hipError_t Engine::upload_descriptors() {
+    const int nf = frame_count();
+    bool changed = !up_valid_;
+    for (int f = 0; f < nf && !changed; ++f) {
+        const FrameDesc& d = h_desc_dev_[f];
+        changed = d.gray != up_gray_[f] || d.stride != up_stride_[f] || d.active != up_active_[f];
+    }
+    if (!changed) return hipSuccess;
+    const hipError_t st = hipMemcpyAsync(d_desc_dev_, h_desc_dev_, static_cast<size_t>(nf) * sizeof(FrameDesc),
+                                         hipMemcpyHostToDevice, stream_);
+    if (st != hipSuccess) return st;
+    up_gray_.resize(nf);
+    up_stride_.resize(nf);
+    up_active_.resize(nf);
+    for (int f = 0; f < nf; ++f) {
+        up_gray_[f] = h_desc_dev_[f].gray;
+        up_stride_[f] = h_desc_dev_[f].stride;
+        up_active_[f] = h_desc_dev_[f].active;
+    }
+    up_valid_ = true;
+    return hipSuccess;
+}
+
 hipError_t Engine::launch(bool device_path) {
+    if (device_path) {
+        const hipError_t st = upload_descriptors();
+        if (st != hipSuccess) return st;
+    }
     if (!two_lanes()) {
         hipGraphExec_t g = device_path ? graph_dev_ : graph_host_;
         return g ? hipGraphLaunch(g, stream_) : enqueue_lane(device_path, -1);

@@ -100,6 +100,7 @@ private:
     // One pipeline run on the staged frames, by graph where one was
     // captured; everything later queued on stream_ runs after it.
     hipError_t launch(bool device_path);
+    hipError_t upload_descriptors();
     bool set_tune_locked(const Tune& t, std::string* error);
     bool replay_locked(int reps, bool device_path, float* ms, std::string* error);
     void release();
@@ -126,6 +127,11 @@ private:
     uint32_t* h_scal_ = nullptr;
     bd_detection* h_out_ = nullptr;
     FrameDesc* h_desc_dev_ = nullptr;   // pinned staging of the device-path descriptors
+    // Frame pointer, stride and active flag of each device-path descriptor as
+    // last uploaded; a launch uploads the descriptors only when one changed.
+    std::vector<const uint8_t*> up_gray_;
+    std::vector<int32_t> up_stride_, up_active_;
+    bool up_valid_ = false;
 
     Plan plan_host_{};
     Plan plan_dev_{};

@@ -195,6 +195,35 @@ def test_device_pointer_path():
         det.detect_device(buf.ptr, 9_000_000)
 
 
+def test_device_batch_pointer_changes():
+    # Frame descriptors are uploaded only when a pointer, stride or active
+    # flag changes: alternate two sets of device frames, then change one
+    # frame only, and check every call against a fresh batch.
+    import devmem
+    W, H, n = 640, 480, 4
+    a = [scenes.scene(0, W, H, 200 + i)[0] for i in range(n)]
+    b = [scenes.scene(0, W, H, 300 + i)[0] for i in range(n)]
+    abuf, bbuf = devmem.DeviceBuffer(np.stack(a)), devmem.DeviceBuffer(np.stack(b))
+    pa = [(abuf.ptr + i * W * H, W) for i in range(n)]
+    pb = [(bbuf.ptr + i * W * H, W) for i in range(n)]
+
+    def fresh(ptrs):
+        return bdcudatag.Batch(0, [(W, H)] * n).detect_device(ptrs)
+
+    def same(x, y):
+        assert [[d["id"] for d in f] for f in x] == [[d["id"] for d in f] for f in y]
+        for fx, fy in zip(x, y):
+            for dx, dy in zip(fx, fy):
+                assert np.array_equal(dx["corners"], dy["corners"])
+
+    ra, rb = fresh(pa), fresh(pb)
+    assert sum(len(f) for f in ra) > 0 and sum(len(f) for f in rb) > 0
+    batch = bdcudatag.Batch(0, [(W, H)] * n)
+    mixed = [pa[0], pb[1], pa[2], pa[3]]
+    for ptrs, want in [(pa, ra), (pa, ra), (pb, rb), (pa, ra), (mixed, [ra[0], rb[1], ra[2], ra[3]])]:
+        same(batch.detect_device(ptrs), want)
+
+
 def test_misaligned_device_frames():
     # Device frames at 1-3 bytes past a 4-byte boundary with an odd row
     # stride: the pixel-pair loads fold the misalignment in, and the last
