@@ -189,7 +189,6 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         o_lane_prefix[l] = take(static_cast<size_t>(SH_COUNT) * (lane_count[l] + 1) * 4);
     }
     const size_t o_scal = take(static_cast<size_t>(nf) * S_COUNT * 4);
-    const size_t o_out = take(static_cast<size_t>(nf) * max_det_ * sizeof(bd_detection));
     const size_t o_desc_h = take(static_cast<size_t>(nf) * sizeof(FrameDesc));
     const size_t o_desc_d = take(static_cast<size_t>(nf) * sizeof(FrameDesc));
     const size_t o_prefix = take(static_cast<size_t>(SH_COUNT) * (nf + 1) * 4);
@@ -206,7 +205,6 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     uint8_t* base = static_cast<uint8_t*>(arena_);
     d_scal_ = reinterpret_cast<uint32_t*>(base + o_scal);
     d_work_ = reinterpret_cast<uint32_t*>(base + o_work);
-    d_out_ = reinterpret_cast<bd_detection*>(base + o_out);
     d_desc_host_ = reinterpret_cast<FrameDesc*>(base + o_desc_h);
     d_desc_dev_ = reinterpret_cast<FrameDesc*>(base + o_desc_d);
     d_prefix_ = reinterpret_cast<int32_t*>(base + o_prefix);
@@ -217,6 +215,11 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     if (st == hipSuccess) st = hipHostMalloc(reinterpret_cast<void**>(&h_scal_), static_cast<size_t>(nf) * S_COUNT * 4, hipHostMallocPortable);
     if (st == hipSuccess) st = hipHostMalloc(reinterpret_cast<void**>(&h_out_), static_cast<size_t>(nf) * max_det_ * sizeof(bd_detection), hipHostMallocPortable);
     if (st == hipSuccess) st = hipHostMalloc(reinterpret_cast<void**>(&h_desc_dev_), static_cast<size_t>(nf) * sizeof(FrameDesc), hipHostMallocPortable);
+    // k_finalize writes each frame's detection records straight into the
+    // pinned host array (only the records found cross PCIe; a copy node of
+    // the whole array moved nframes x max_detections x 84 bytes per call).
+    bd_detection* out_dev = nullptr;
+    if (st == hipSuccess) st = hipHostGetDevicePointer(reinterpret_cast<void**>(&out_dev), h_out_, 0);
     if (st != hipSuccess) {
         if (error) *error = hip_error("hipHostMalloc", st);
         return false;
@@ -273,7 +276,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         d.quads = reinterpret_cast<float*>(base + o.quads);
         d.hull_quads = reinterpret_cast<float*>(base + o.hquads);
         d.dec = reinterpret_cast<bd_detection*>(base + o.dec);
-        d.out = d_out_ + static_cast<size_t>(f) * max_det_;
+        d.out = out_dev + static_cast<size_t>(f) * max_det_;
         prefix[SH_TILE_ROW * (nf + 1) + f + 1] = prefix[SH_TILE_ROW * (nf + 1) + f] + d.tile_h;
         prefix[SH_FRAME * (nf + 1) + f + 1] = f + 1;
         const int pairs = (d.tile_w + 1) / 2;
@@ -387,9 +390,6 @@ hipError_t Engine::enqueue_lane(bool device_path, int lane) {
     if (st == hipSuccess)
         st = hipMemcpyAsync(h_scal_ + static_cast<size_t>(first) * S_COUNT, d_scal_ + static_cast<size_t>(first) * S_COUNT,
                             static_cast<size_t>(n) * S_COUNT * 4, hipMemcpyDeviceToHost, s);
-    if (st == hipSuccess)
-        st = hipMemcpyAsync(h_out_ + static_cast<size_t>(first) * max_det_, d_out_ + static_cast<size_t>(first) * max_det_,
-                            static_cast<size_t>(n) * max_det_ * sizeof(bd_detection), hipMemcpyDeviceToHost, s);
     return st;
 }
 

@@ -121,7 +121,6 @@ private:
     uint64_t* d_codes_ = nullptr;
     uint32_t* d_scal_ = nullptr;        // nframes * S_COUNT
     uint32_t* d_work_ = nullptr;        // batch candidate list (header, head and tail regions)
-    bd_detection* d_out_ = nullptr;     // nframes * max_det
 
     uint8_t* h_input_ = nullptr;        // pinned staging
     uint32_t* h_scal_ = nullptr;

@@ -119,7 +119,7 @@ struct FrameDesc {
     float* quads;         // max_candidates * 8
     float* hull_quads;    // max_candidates * 8
     bd_detection* dec;    // max_candidates, per-candidate decode result
-    bd_detection* out;    // max_detections, final detections
+    bd_detection* out;    // max_detections, final detections (pinned host memory)
 };
 
 // Prefix tables over frames: prefix[k][f] = first block of frame f for
