@@ -480,6 +480,19 @@ __device__ inline uint32_t lds_union(uint32_t* lab, uint32_t a, uint32_t b) {
     }
 }
 
+// Bit b (0-3) set when byte b of x is above t (unsigned), four bytes at
+// once: x > t is x >= t + 1, decided per byte by its top bit where those
+// differ and else by the low seven bits, whose difference borrows into the
+// top bit (bytes cannot borrow from each other: each top bit is set before
+// the subtraction).  The four flags are gathered by one multiply.
+__device__ inline uint32_t bytes_above(uint32_t x, uint32_t t) {
+    if (t >= 255u) return 0u;
+    const uint32_t t1 = (t + 1u) * 0x01010101u;
+    const uint32_t low = (x | 0x80808080u) - (t1 & 0x7f7f7f7fu);
+    const uint32_t ge = ((x ^ t1) & x) | (~(x ^ t1) & low);
+    return ((ge & 0x80808080u) * 0x00204081u) >> 28;
+}
+
 // One workgroup per tile row (8 pixel rows).
 //  1. Pooled 3x3 thresholds of the row's tiles in LDS (k_tile_threshold,
 //     reference gpu_detector.cu:237-276).
@@ -599,10 +612,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                         const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
                         for (int j = 0; j < 4; ++j) {
                             const uint32_t tv = thr[(xs >> 3) + 2 * q + (j >> 1)];
-                            for (int b = 0; b < 4; ++b) {
-                                const uint32_t px = (wd[j] >> (8 * b)) & 0xffu;
-                                m32 |= static_cast<uint32_t>(px > tv) << (16 * q + 4 * j + b);
-                            }
+                            m32 |= bytes_above(wd[j], tv) << (16 * q + 4 * j);
                         }
                     }
                 } else {
