@@ -1,11 +1,16 @@
-// C ABI and C++ wrapper (reference detector/src/detector.cpp:1-382): the
-// same validation order, status codes, thread-local last error and config
-// defaults, on top of the HIP engine.
+// C ABI and C++ API on top of the HIP engine.  The C ABI keeps the
+// reference's observable behaviour (detector/src/detector.cpp:1-225): the
+// same validation order, status codes, error strings, thread-local last
+// error and config defaults.  The C++ Detector (detector.cpp:227-380 for the
+// behaviour: exceptions, clamps, messages) owns its engine directly; both
+// APIs share one argument check and call path per frame.
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "bd_engine.h"
@@ -22,6 +27,14 @@ bool valid_family(int family) { return family == 0 || family == 1; }
 
 bool valid_dimensions(int width, int height) {
     return width >= 16 && width <= BD_MAX_WIDTH && height >= 16 && height <= BD_MAX_HEIGHT;
+}
+
+// The argument checks of detector creation, in the reference's order
+// (detector.cpp:112-129); nullptr when the family and size are acceptable.
+const char* creation_problem(int family, int width, int height) {
+    if (!valid_family(family)) return "unknown tag family";
+    if (!valid_dimensions(width, height)) return "image dimensions out of range (16..8192 x 16..16384)";
+    return nullptr;
 }
 
 // detector.cpp:39-57 (config_to_gpu)
@@ -43,12 +56,34 @@ bd::Params to_params(int family, const bd_config* config) {
     return p;
 }
 
-bool check_output(const bd_detection* out, int32_t capacity) {
-    if (capacity < 0 || (capacity > 0 && !out)) {
-        set_error("invalid output buffer");
-        return false;
+// One frame through a single-frame engine: the reference's per-call checks
+// (detector.cpp:164-218, in its order), then the detection.  Returns the
+// number of tags found (records written: min(found, capacity)), or a
+// negative bd_status with *why set.
+int32_t run_single(bd::Engine& engine, const uint8_t* gray, int32_t stride, bd_detection* out,
+                   int32_t capacity, bool device, std::string* why) {
+    const int width = engine.width(0), height = engine.height(0);
+    const char* bad = nullptr;
+    if (!gray) {
+        bad = device ? "null device image pointer" : "null image pointer";
+    } else if (stride < width) {
+        bad = "stride is smaller than the image width";
+    } else if (device && !bd::frame_span_fits(width, height, stride)) {
+        bad = "device frame spans 4 GiB or more";
+    } else if (capacity < 0 || (capacity > 0 && !out)) {
+        bad = "invalid output buffer";
     }
-    return true;
+    if (bad) {
+        *why = bad;
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    const bd_frame frame{gray, width, height, stride, 0};
+    int32_t found = 0;
+    if (engine.detect(&frame, device, out, capacity, &found, why) < 0) {
+        if (why->empty()) *why = "HIP detection failed";
+        return BD_ERROR_CUDA;
+    }
+    return found;
 }
 
 }  // namespace
@@ -111,13 +146,8 @@ void bd_config_init(bd_config* config) {
 bd_detector* bd_detector_create(int32_t family, int32_t width, int32_t height,
                                 const bd_config* config, int32_t* status_out) {
     if (status_out) *status_out = BD_OK;
-    if (!valid_family(family)) {
-        set_error("unknown tag family");
-        if (status_out) *status_out = BD_ERROR_INVALID_ARGUMENT;
-        return nullptr;
-    }
-    if (!valid_dimensions(width, height)) {
-        set_error("image dimensions out of range (16..8192 x 16..16384)");
+    if (const char* problem = creation_problem(family, width, height)) {
+        set_error(problem);
         if (status_out) *status_out = BD_ERROR_INVALID_ARGUMENT;
         return nullptr;
     }
@@ -159,27 +189,9 @@ static int32_t detect_one(bd_detector* detector, const uint8_t* gray, int32_t st
         set_error("null detector");
         return BD_ERROR_INVALID_ARGUMENT;
     }
-    if (!gray) {
-        set_error(device ? "null device image pointer" : "null image pointer");
-        return BD_ERROR_INVALID_ARGUMENT;
-    }
-    if (stride < detector->width) {
-        set_error("stride is smaller than the image width");
-        return BD_ERROR_INVALID_ARGUMENT;
-    }
-    if (device && !bd::frame_span_fits(detector->width, detector->height, stride)) {
-        set_error("device frame spans 4 GiB or more");
-        return BD_ERROR_INVALID_ARGUMENT;
-    }
-    if (!check_output(out, out_capacity)) return BD_ERROR_INVALID_ARGUMENT;
-    bd_frame frame{gray, detector->width, detector->height, stride, 0};
-    int32_t found = 0;
-    std::string error;
-    const int written = detector->batch.engine->detect(&frame, device, out, out_capacity, &found, &error);
-    if (written < 0) {
-        set_error(error.empty() ? "HIP detection failed" : error);
-        return BD_ERROR_CUDA;
-    }
+    std::string why;
+    const int32_t found = run_single(*detector->batch.engine, gray, stride, out, out_capacity, device, &why);
+    if (found < 0) set_error(why);
     return found;
 }
 
@@ -420,132 +432,133 @@ int32_t bd_batch_autotune(bd_batch* batch, int32_t reps, bd_tune* best, float* b
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// C++ API (reference detector.cpp:227-380)
+// C++ API
 // ---------------------------------------------------------------------------
 
 namespace bdcudatag {
+namespace {
 
+Detection to_cpp(const bd_detection& r) {
+    Detection d;
+    d.id = r.id;
+    d.hamming = r.hamming;
+    d.decision_margin = r.decision_margin;
+    for (size_t k = 0; k < d.corners.size(); ++k) d.corners[k] = {r.corners[2 * k], r.corners[2 * k + 1]};
+    std::copy(r.homography, r.homography + 9, d.homography.begin());
+    return d;
+}
+
+}  // namespace
+
+// The C++ detector owns its engine outright, with no C handle in between.
+// `records` is the engine's output for the current call, reused across
+// calls; `lock` serialises the calls that share it (the engine serialises
+// its own state too), so one Detector may be shared between threads.
 struct Detector::Impl {
-    bd_detector* handle = nullptr;
-    int max_detections = 0;
-    std::vector<bd_detection> scratch;
-    std::mutex mutex;
+    std::unique_ptr<bd::Engine> engine;
+    std::vector<bd_detection> records;
+    std::mutex lock;
+
+    static Impl& live(Impl* impl) {
+        if (!impl) throw std::runtime_error("detector has been moved from");
+        return *impl;
+    }
+
+    // Host or device input: the same checks, errors and record conversion.
+    // Writes at most min(capacity, max_detections) detections.
+    int run(const uint8_t* gray, int stride, Detection* out, int capacity, bool device) {
+        if (capacity < 0 || (capacity > 0 && !out)) throw std::invalid_argument("invalid detection output buffer");
+        capacity = std::min(capacity, static_cast<int>(records.size()));
+        const std::lock_guard<std::mutex> hold(lock);
+        std::string why;
+        const int32_t found = run_single(*engine, gray, stride, records.data(), capacity, device, &why);
+        if (found < 0) {
+            set_error(why);
+            throw std::runtime_error(why);
+        }
+        const int written = std::min(static_cast<int>(found), capacity);
+        std::transform(records.begin(), records.begin() + written, out, to_cpp);
+        return written;
+    }
 };
 
 Detector::Detector(Family family, int width, int height, const DetectorConfig& config)
     : family_(family), width_(width), height_(height) {
-    bd_config native;
-    bd_config_init(&native);
-    native.max_hamming = config.max_hamming;
-    native.min_cluster_pixels = config.min_cluster_pixels;
-    native.min_tag_pixels = config.min_tag_pixels;
-    native.max_detections = config.max_detections;
-    native.max_candidates = config.max_candidates;
-    native.refine_edges = config.refine_edges ? 1 : 0;
-    native.use_cuda_graph = config.use_cuda_graph ? 1 : 0;
-    native.min_decision_margin = config.min_decision_margin;
-    native.decode_sharpening = config.decode_sharpening;
-    int32_t status = BD_OK;
-    impl_ = new Impl();
-    impl_->handle = bd_detector_create(static_cast<int32_t>(family), width, height, &native, &status);
-    if (!impl_->handle) {
-        const std::string message = bd_last_error();
-        delete impl_;
-        impl_ = nullptr;
-        throw std::runtime_error(message.empty() ? "failed to create bdcudatag detector" : message);
+    const int fam = static_cast<int>(family);
+    if (const char* problem = creation_problem(fam, width, height)) {
+        set_error(problem);
+        throw std::runtime_error(problem);
     }
-    impl_->max_detections = std::min(std::max(native.max_detections, 1), 256);
-    impl_->scratch.resize(static_cast<size_t>(impl_->max_detections));
+    bd::Params p;
+    p.family = fam;
+    p.max_hamming = config.max_hamming;
+    p.min_cluster_pixels = config.min_cluster_pixels;
+    p.min_tag_pixels = config.min_tag_pixels;
+    p.max_detections = config.max_detections;
+    p.max_candidates = config.max_candidates;
+    p.refine_edges = config.refine_edges;
+    p.use_graph = config.use_cuda_graph;
+    p.min_decision_margin = config.min_decision_margin;
+    p.decode_sharpening = config.decode_sharpening;
+    std::string why;
+    std::unique_ptr<bd::Engine> engine(bd::Engine::create(p, {width}, {height}, &why));
+    if (!engine) {
+        if (why.empty()) why = "failed to create bdcudatag detector";
+        set_error(why);
+        throw std::runtime_error(why);
+    }
+    std::unique_ptr<Impl> impl(new Impl());
+    impl->records.resize(static_cast<size_t>(engine->max_detections()));
+    impl->engine = std::move(engine);
+    impl_ = impl.release();
 }
 
-Detector::~Detector() {
-    if (!impl_) return;
-    if (impl_->handle) bd_detector_destroy(impl_->handle);
-    delete impl_;
-}
+Detector::~Detector() { delete impl_; }
 
 Detector::Detector(Detector&& other) noexcept
-    : impl_(other.impl_), family_(other.family_), width_(other.width_), height_(other.height_) {
-    other.impl_ = nullptr;
-}
+    : impl_(std::exchange(other.impl_, nullptr)), family_(other.family_), width_(other.width_),
+      height_(other.height_) {}
 
 Detector& Detector::operator=(Detector&& other) noexcept {
-    if (this == &other) return *this;
-    if (impl_) {
-        if (impl_->handle) bd_detector_destroy(impl_->handle);
-        delete impl_;
+    if (this != &other) {
+        delete std::exchange(impl_, std::exchange(other.impl_, nullptr));
+        family_ = other.family_;
+        width_ = other.width_;
+        height_ = other.height_;
     }
-    impl_ = other.impl_;
-    family_ = other.family_;
-    width_ = other.width_;
-    height_ = other.height_;
-    other.impl_ = nullptr;
     return *this;
 }
 
-static void copy_out(const bd_detection& src, Detection& dst) {
-    dst.id = src.id;
-    dst.hamming = src.hamming;
-    dst.decision_margin = src.decision_margin;
-    for (int k = 0; k < 4; ++k) {
-        dst.corners[static_cast<size_t>(k)][0] = src.corners[k * 2];
-        dst.corners[static_cast<size_t>(k)][1] = src.corners[k * 2 + 1];
-    }
-    for (int k = 0; k < 9; ++k) dst.homography[static_cast<size_t>(k)] = src.homography[k];
-}
-
 int Detector::detect(const uint8_t* gray, int stride, Detection* out, int out_capacity) {
-    if (!impl_) throw std::runtime_error("detector has been moved from");
-    if (out_capacity < 0 || (out_capacity > 0 && !out)) {
-        throw std::invalid_argument("invalid detection output buffer");
-    }
-    if (out_capacity > impl_->max_detections) out_capacity = impl_->max_detections;
-    const std::lock_guard<std::mutex> lock(impl_->mutex);
-    const int count = bd_detector_detect_host(impl_->handle, gray, stride, impl_->scratch.data(), out_capacity);
-    if (count < 0) throw std::runtime_error(bd_last_error());
-    const int written = std::min(count, out_capacity);
-    for (int i = 0; i < written; ++i) copy_out(impl_->scratch[static_cast<size_t>(i)], out[i]);
-    return written;
-}
-
-std::vector<Detection> Detector::detect(const uint8_t* gray, int stride) {
-    if (!impl_) throw std::runtime_error("detector has been moved from");
-    std::vector<Detection> result(static_cast<size_t>(impl_->max_detections));
-    const int count = detect(gray, stride, result.data(), impl_->max_detections);
-    result.resize(static_cast<size_t>(count));
-    return result;
+    return Impl::live(impl_).run(gray, stride, out, out_capacity, false);
 }
 
 int Detector::detectDevice(const uint8_t* gray_device, int stride, Detection* out, int out_capacity) {
-    if (!impl_) throw std::runtime_error("detector has been moved from");
-    if (out_capacity < 0 || (out_capacity > 0 && !out)) {
-        throw std::invalid_argument("invalid detection output buffer");
-    }
-    if (out_capacity > impl_->max_detections) out_capacity = impl_->max_detections;
-    const std::lock_guard<std::mutex> lock(impl_->mutex);
-    const int count = bd_detector_detect_device(impl_->handle, gray_device, stride, impl_->scratch.data(), out_capacity);
-    if (count < 0) throw std::runtime_error(bd_last_error());
-    const int written = std::min(count, out_capacity);
-    for (int i = 0; i < written; ++i) copy_out(impl_->scratch[static_cast<size_t>(i)], out[i]);
-    return written;
+    return Impl::live(impl_).run(gray_device, stride, out, out_capacity, true);
+}
+
+std::vector<Detection> Detector::detect(const uint8_t* gray, int stride) {
+    Impl& impl = Impl::live(impl_);
+    std::vector<Detection> found(impl.records.size());
+    found.resize(static_cast<size_t>(impl.run(gray, stride, found.data(), static_cast<int>(found.size()), false)));
+    return found;
 }
 
 Stats Detector::stats() const {
-    if (!impl_) return {};
-    const std::lock_guard<std::mutex> lock(impl_->mutex);
-    bd_stats native{};
-    bd_detector_get_stats(impl_->handle, &native);
-    Stats out;
-    out.frames = native.frames;
-    out.last_gpu_ms = native.last_gpu_ms;
-    out.average_gpu_ms = native.average_gpu_ms;
-    out.last_total_ms = native.last_total_ms;
-    out.runs = native.runs;
-    out.candidates = native.candidates;
-    out.detections = native.detections;
-    out.quads = native.quads;
-    out.roi_rows = native.roi_rows;
-    return out;
+    Stats s;
+    if (!impl_) return s;
+    bd_stats n{};
+    impl_->engine->stats(0, &n);
+    s.frames = n.frames;
+    s.last_gpu_ms = n.last_gpu_ms;
+    s.average_gpu_ms = n.average_gpu_ms;
+    s.last_total_ms = n.last_total_ms;
+    s.runs = n.runs;
+    s.candidates = n.candidates;
+    s.detections = n.detections;
+    s.quads = n.quads;
+    s.roi_rows = n.roi_rows;
+    return s;
 }
 
 }  // namespace bdcudatag

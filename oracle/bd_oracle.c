@@ -19,6 +19,13 @@
  *   - edge-line direction: the major-axis normal of the 2x2 moment matrix by
  *     the half-angle identities instead of atan2f/sinf/cosf;
  *   - hypotf(x, y) is sqrtf(x * x + y * y).
+ * Built with -DOR_LITERAL (FP32 only), these three places and the edge-fit
+ * moments follow the reference literally instead: libm cosf/sinf for the
+ * hull directions, raw moments about the image origin with
+ * 0.5f * atan2f(2 cxy, cxx - cyy) and -sinf/cosf for the edge line
+ * (gpu_detector.cu:981-1015), and hypotf (:1035, :1061, :1557).  That build
+ * bounds how far the closed forms above move the results; it is not the
+ * parity target of the HIP kernels.
  */
 #include "bd_oracle.h"
 
@@ -70,9 +77,17 @@ static uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
 static int popc32(uint32_t v) { return __builtin_popcount(v); }
 static int popc64(uint64_t v) { return __builtin_popcountll(v); }
 
-static real or_hypot(real x, real y) { return R_SQRT(x * x + y * y); }
+#if defined(OR_LITERAL) && defined(OR_FP64)
+#error "OR_LITERAL restates the FP32 reference"
+#endif
 
-#ifndef OR_FP64
+#ifdef OR_LITERAL
+static real or_hypot(real x, real y) { return hypotf(x, y); }
+#else
+static real or_hypot(real x, real y) { return R_SQRT(x * x + y * y); }
+#endif
+
+#if !defined(OR_FP64) && !defined(OR_LITERAL)
 /* cos/sin of fl32(fl32(6.28318530718f * d) / 32), correctly rounded to FP32
  * (the reference computes cosf/sinf of that angle at gpu_detector.cu:1139-1142). */
 static const float kDirCos[32] = {
@@ -98,7 +113,11 @@ static const float kDirSin[32] = {
 #endif
 
 static void hull_direction(int d, real* ux, real* uy) {
-#ifdef OR_FP64
+#if defined(OR_LITERAL)
+    const float angle = 6.28318530718f * (float)d / (float)kFitDirections;
+    *ux = cosf(angle);
+    *uy = sinf(angle);
+#elif defined(OR_FP64)
     const double a = 6.283185307179586 * (double)d / 32.0;
     *ux = cos(a);
     *uy = sin(a);
@@ -198,6 +217,14 @@ const uint64_t* or_family_codes(int32_t family, int32_t* count, int32_t* bits_pe
 }
 
 int32_t or_real_bits(void) { return R_BITS; }
+
+int32_t or_literal(void) {
+#ifdef OR_LITERAL
+    return 1;
+#else
+    return 0;
+#endif
+}
 
 or_frame* or_create(const or_config* c) {
     if (!c || c->family < 0 || c->family > 1 || c->width < 16 || c->height < 16 ||
@@ -850,7 +877,11 @@ static int fit_edge_line(const real quad[4][2], int edge, const uint8_t* image, 
      * (the reference sums raw x, x^2 at gpu_detector.cu:983-987): the same
      * line, without the cancellation that raw second moments of coordinates
      * near 1000 suffer in FP32. */
+#ifdef OR_LITERAL
+    const real rx = RL(0.0), ry = RL(0.0);
+#else
     const real rx = ax + RL(0.5) * ex, ry = ay + RL(0.5) * ey;
+#endif
 
     real lm[32][5];
     int lused[32];
@@ -938,7 +969,13 @@ static int fit_edge_line(const real quad[4][2], int edge, const uint8_t* image, 
     const real cxy = lm[0][3] * inv - cx * cy;
     const real cyy = lm[0][4] * inv - cy * cy;
     real lnx, lny;
+#ifdef OR_LITERAL
+    const float theta = 0.5f * atan2f(2.0f * cxy, cxx - cyy);
+    lnx = -sinf(theta);
+    lny = cosf(theta);
+#else
     axis_normal(cxx, cxy, cyy, &lnx, &lny);
+#endif
     line[0] = rx + cx;
     line[1] = ry + cy;
     line[2] = lnx;
@@ -1508,4 +1545,51 @@ int32_t or_match_code(int32_t family, uint64_t code, int32_t max_hamming, int32_
     const int id = match_code(codes, count, code, max_hamming, &h);
     if (hamming) *hamming = h;
     return id;
+}
+
+/* One edge fit of a frame created by or_create (its size and tag family),
+ * on `gray`: quad corners in pixels, edge e runs from corner e to e + 1. */
+int32_t or_fit_edge(const or_frame* f, const uint8_t* gray, int32_t stride, const double quad8[8],
+                    int32_t edge, int32_t first_pass, double line_out[4]) {
+    real quad[4][2];
+    for (int i = 0; i < 4; ++i) {
+        quad[i][0] = (real)quad8[2 * i];
+        quad[i][1] = (real)quad8[2 * i + 1];
+    }
+    real line[4];
+    const int ok = fit_edge_line((const real(*)[2])quad, edge & 3, gray, f->width, f->height, stride, f->dim,
+                                 first_pass != 0, line);
+    for (int k = 0; k < 4 && ok; ++k) line_out[k] = (double)line[k];
+    return ok;
+}
+
+/* decode_quad of one rotation with the frame's family and thresholds;
+ * returns 1 with id, hamming and margin when the quad decodes. */
+int32_t or_decode_rotation(const or_frame* f, const uint8_t* gray, int32_t stride, const double quad8[8],
+                           int32_t rot, int32_t* id, int32_t* hamming, double* margin) {
+    real quad[4][2], corners[4][2];
+    for (int i = 0; i < 4; ++i) {
+        quad[i][0] = (real)quad8[2 * i];
+        quad[i][1] = (real)quad8[2 * i + 1];
+    }
+    int i_id = -1, i_ham = 0;
+    real m = RL(0.0);
+    const int ok = decode_quad(f, gray, stride, (const real(*)[2])quad, rot & 3, &i_id, &i_ham, &m, corners);
+    if (ok) {
+        *id = i_id;
+        *hamming = i_ham;
+        *margin = (double)m;
+    }
+    return ok;
+}
+
+/* hull_to_quad on packed points x | y << 16. */
+int32_t or_hull_to_quad(const uint32_t* hull, int32_t count, double quad_out[8]) {
+    real quad[4][2];
+    const int ok = hull_to_quad(hull, count, quad);
+    for (int i = 0; i < 4 && ok; ++i) {
+        quad_out[2 * i] = (double)quad[i][0];
+        quad_out[2 * i + 1] = (double)quad[i][1];
+    }
+    return ok;
 }

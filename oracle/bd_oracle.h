@@ -99,6 +99,9 @@ typedef struct or_frame or_frame;
 
 /* Build precision: 32 or 64. */
 int32_t or_real_bits(void);
+/* 1 for the build that restates the reference's transcendental calls and
+ * raw edge moments literally (OR_LITERAL), else 0. */
+int32_t or_literal(void);
 
 or_frame* or_create(const or_config* config);
 void or_destroy(or_frame* frame);
@@ -113,6 +116,13 @@ int32_t or_hull_direction(int32_t dir, double* ux, double* uy);
 void or_axis_normal(double cxx, double cxy, double cyy, double* nx, double* ny);
 int32_t or_match_code(int32_t family, uint64_t code, int32_t max_hamming, int32_t* hamming);
 const uint64_t* or_family_codes(int32_t family, int32_t* count, int32_t* bits_per_side);
+/* One edge-line fit (line = point x, y and unit normal), one decode
+ * rotation, and the hull reduction, for closed-form tests. */
+int32_t or_fit_edge(const or_frame* frame, const uint8_t* gray, int32_t stride, const double quad[8],
+                    int32_t edge, int32_t first_pass, double line[4]);
+int32_t or_decode_rotation(const or_frame* frame, const uint8_t* gray, int32_t stride, const double quad[8],
+                           int32_t rot, int32_t* id, int32_t* hamming, double* margin);
+int32_t or_hull_to_quad(const uint32_t* hull, int32_t count, double quad[8]);
 
 #ifdef __cplusplus
 }

@@ -64,10 +64,14 @@ class View(ctypes.Structure):
 _libs = {}
 
 
-def lib(fp64=False):
-    key = 64 if fp64 else 32
+def lib(fp64=False, literal=False):
+    """liboracle32 (the parity build), liboracle64 (FP64 twin) or
+    liboracle32lit (the reference's transcendental calls and raw edge
+    moments restated literally)."""
+    key = 64 if fp64 else (3 if literal else 32)
     if key not in _libs:
-        path = os.path.join(BUILD, f"liboracle{key}.so")
+        name = {32: "liboracle32", 64: "liboracle64", 3: "liboracle32lit"}[key]
+        path = os.path.join(BUILD, f"{name}.so")
         if not os.path.exists(path):
             raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
         L = ctypes.CDLL(path)
@@ -78,6 +82,7 @@ def lib(fp64=False):
         L.or_run.restype = ctypes.c_int32
         L.or_get_view.argtypes = [ctypes.c_void_p, ctypes.POINTER(View)]
         L.or_real_bits.restype = ctypes.c_int32
+        L.or_literal.restype = ctypes.c_int32
         L.or_fill_homography.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _PD]
         L.or_hull_direction.argtypes = [ctypes.c_int32, _PD, _PD]
         L.or_hull_direction.restype = ctypes.c_int32
@@ -86,7 +91,16 @@ def lib(fp64=False):
         L.or_match_code.restype = ctypes.c_int32
         L.or_family_codes.argtypes = [ctypes.c_int32, _PI32, _PI32]
         L.or_family_codes.restype = ctypes.POINTER(ctypes.c_uint64)
-        assert L.or_real_bits() == key
+        L.or_fit_edge.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, _PD, ctypes.c_int32,
+                                  ctypes.c_int32, _PD]
+        L.or_fit_edge.restype = ctypes.c_int32
+        L.or_decode_rotation.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, _PD, ctypes.c_int32,
+                                         _PI32, _PI32, _PD]
+        L.or_decode_rotation.restype = ctypes.c_int32
+        L.or_hull_to_quad.argtypes = [_P32, ctypes.c_int32, _PD]
+        L.or_hull_to_quad.restype = ctypes.c_int32
+        assert L.or_real_bits() == (64 if fp64 else 32)
+        assert L.or_literal() == (1 if literal else 0)
         _libs[key] = L
     return _libs[key]
 

@@ -161,6 +161,16 @@ def test_row_table_overflow():
     assert [d["id"] for d in dets] == [3]
 
 
+def same_as_oracle(got, img, family=0, **cfg):
+    H, W = img.shape
+    want = pyoracle.Oracle(pyoracle.default_config(family, W, H, **cfg)).run(img)["detections"]
+    assert [d["id"] for d in got] == [d["id"] for d in want]
+    for x, y in zip(got, want):
+        assert x["hamming"] == y["hamming"]
+        assert np.array_equal(x["corners"], y["corners"])
+        assert np.array_equal(x["homography"], y["homography"])
+
+
 def test_strided_host_frame():
     img, tags = scenes.scene(0, 1280, 800, 9)
     padded = np.zeros((800, 1344), dtype=np.uint8)
@@ -171,6 +181,7 @@ def test_strided_host_frame():
     assert [d["id"] for d in a] == [d["id"] for d in b]
     for x, y in zip(a, b):
         assert np.array_equal(x["corners"], y["corners"])
+    same_as_oracle(a, img)
 
 
 def test_device_pointer_path():
@@ -182,6 +193,7 @@ def test_device_pointer_path():
     a = det.detect_device(buf.ptr, 640)
     b = det.detect(img)
     assert [d["id"] for d in a] == [11]
+    same_as_oracle(a, img, **KA)
     assert np.array_equal(a[0]["corners"], b[0]["corners"])
     assert np.array_equal(a[0]["homography"], b[0]["homography"])
     # padded device rows
@@ -241,6 +253,7 @@ def test_misaligned_device_frames():
         view[:, :1280] = img
         buf = devmem.DeviceBuffer(raw)
         got = det.detect_device(buf.ptr + off, stride)
+        same_as_oracle(got, img)
         assert [d["id"] for d in got] == [d["id"] for d in ref]
         for p, q in zip(got, ref):
             assert np.array_equal(p["corners"], q["corners"])

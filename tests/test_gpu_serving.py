@@ -2,12 +2,14 @@
 
 Each request's result must be the same alone (single-frame detector, device
 pointer path) and in the mix (admitted while other requests are on the GPU,
-batched with frames of other streams): bit-exact detections.
+batched with frames of other streams): bit-exact detections, and both equal
+the oracle's.
 """
 import numpy as np
 import pytest
 
 import bdcudatag
+import pyoracle
 import scenes
 
 pytestmark = pytest.mark.gpu
@@ -36,6 +38,12 @@ def test_same_alone_and_in_the_mix(frames):
         det = bdcudatag.Detector(0, c["width"], c["height"])
         alone.append([det.detect_device(p, c["width"]) for p in frames[ci][2]])
         det.close()
+        for img, got in zip(frames[ci][0], alone[ci]):
+            want = pyoracle.Oracle(pyoracle.default_config(0, c["width"], c["height"])).run(img)["detections"]
+            assert [d["id"] for d in got] == [d["id"] for d in want]
+            for a, b in zip(got, want):
+                assert np.array_equal(a["corners"], b["corners"])
+                assert np.array_equal(a["homography"], b["homography"])
     srv = bdcudatag.Server(0, CLASSES)
     order = []
     for rep in range(3):
