@@ -259,6 +259,49 @@ def score_scene(tags, detections):
     return found, len(truth), wrong, errors
 
 
+def protocol_score(tags, detections):
+    """The reference benchmark's accuracy rules (detector/bench/
+    bench_detector.cpp:52-85,153-182): a detection counts for a tag when it
+    has the tag's id and its centre lies within 0.75 x the tag's mean side
+    (of four) of the true centre; the tag's error is the least mean corner
+    distance over the detections that count, corners matched as an
+    unordered set; a wrong id is a detection whose id is in no tag of the
+    scene.  Returns (found, total, wrong_ids, errors)."""
+    import itertools
+    orders = [list(o) for o in itertools.permutations(range(4))]
+    ids = {t["id"] for t in tags}
+    wrong = sum(1 for d in detections if d["id"] not in ids)
+    found, errors = 0, []
+    for t in tags:
+        truth = np.asarray(t["corners"], dtype=np.float64)
+        side = np.mean([np.hypot(*(truth[(i + 1) % 4] - truth[i])) for i in range(4)])
+        best = None
+        for d in detections:
+            c = np.asarray(d["corners"], dtype=np.float64)
+            if d["id"] != t["id"] or np.hypot(*(c.mean(0) - truth.mean(0))) > 0.75 * side:
+                continue
+            e = min(float(np.hypot(*(c - truth[o]).T).mean()) for o in orders)
+            best = e if best is None else min(best, e)
+        if best is not None:
+            found += 1
+            errors.append(best)
+    return found, len(tags), wrong, errors
+
+
+def protocol_percentile(values, fraction):
+    """percentile() of bench_detector.cpp:43-48: the ceil(f * n)-th smallest."""
+    v = sorted(values)
+    if not v:
+        return 0.0
+    i = int(np.ceil(fraction * len(v)))
+    return v[min(len(v) - 1, max(i, 1) - 1)]
+
+
+# Published accuracy of the reference on its own scenes (README.md:140-155):
+# resolution -> median corner error in px (100 % found, 0 wrong ids each).
+REFERENCE_CORNER_MEDIAN = {(640, 480): 0.13, (1280, 720): 0.17, (1280, 800): 0.17, (1920, 1080): 0.23}
+
+
 if __name__ == "__main__":
     import sys
     # python scenes.py [output-dir] [scene-count]: the reference exporter's sets

@@ -1,0 +1,54 @@
+"""The pins of the quad fit and decode, through the HIP detector.
+
+  - The reference benchmark's 32 scenes at its four resolutions, through
+    bd_detector_detect_host with max_detections 32 (detector/bench/
+    bench_detector.cpp): every detection bit-identical with the oracle's,
+    100 % of tags found, no wrong id and the published median corner error
+    (README.md:140-155) within 0.005 px.
+  - The two halo frames of the nearest-peak rule, stage by stage against the
+    oracle, with the fit on the tag's border or on the ring's outer edge.
+"""
+import numpy as np
+import pytest
+
+import bdcudatag
+import pyoracle
+import scenes
+from parity import compare_frame, gpu_stage_dump
+from test_oracle_pins_fit import corner_distance, halo_frame
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("res", sorted(scenes.REFERENCE_CORNER_MEDIAN))
+def test_reference_protocol_accuracy_gpu(res):
+    W, H = res
+    det = bdcudatag.Detector(0, W, H, max_detections=32)
+    o = pyoracle.Oracle(pyoracle.default_config(0, W, H, max_detections=32))
+    found = total = wrong = 0
+    errors = []
+    for i in range(32):
+        img, tags = scenes.bench_scene(W, H, i)
+        got = det.detect(img)
+        want = o.run(img)["detections"]
+        assert [d["id"] for d in got] == [d["id"] for d in want]
+        for a, b in zip(got, want):
+            assert np.array_equal(a["corners"], b["corners"]) and np.array_equal(a["homography"], b["homography"])
+        f, n, w, e = scenes.protocol_score(tags, got)
+        found, total, wrong, errors = found + f, total + n, wrong + w, errors + e
+    assert (found, wrong) == (total, 0)
+    median = scenes.protocol_percentile(errors, 0.5)
+    assert abs(median - scenes.REFERENCE_CORNER_MEDIAN[res]) <= 0.005, median
+
+
+@pytest.mark.parametrize("level,inner", [(125, True), (105, False)])
+def test_nearest_peak_halo_gpu(level, inner):
+    img, tag, outer = halo_frame(level)
+    H, W = img.shape
+    det = bdcudatag.Detector(0, W, H)
+    dets = det.detect(img)
+    o = pyoracle.Oracle(pyoracle.default_config(0, W, H)).run(img)
+    bad = compare_frame(gpu_stage_dump(det.batch, 0), o, dets, det.stats())
+    assert not bad, bad
+    assert [d["id"] for d in dets] == [7]
+    assert corner_distance(dets[0]["corners"], tag if inner else outer) < (0.3 if inner else 0.1)
