@@ -140,6 +140,7 @@ SIGNATURES = {
     "bd_server_simulate": (_I, [_P, ctypes.POINTER(StreamSpec), _I, ctypes.c_double, ctypes.c_uint64,
                                 ctypes.POINTER(ServeReport)]),
     "bd_server_last_error": (ctypes.c_char_p, []),
+    "bd_detect_device_noalloc": (_I, [_I, _I, _I, ctypes.POINTER(Config), _P, _I, _P, _I, _P, _P]),
 }
 
 _lib = None
@@ -406,6 +407,20 @@ class Detector:
         s = Stats()
         lib().bd_detector_get_stats(self.handle, ctypes.byref(s))
         return _stats_dict(s)
+
+
+def detect_device_noalloc(family: int, width: int, height: int, gray_device: int, stride: int,
+                          out_device: int, out_capacity: int, count_device: int, stream: int | None = None,
+                          **config) -> int:
+    """bd_detect_device_noalloc: queues one device frame on `stream`; records
+    and count land in the given device buffers.  Returns BD_OK or raises."""
+    cfg = default_config(**config)
+    rc = lib().bd_detect_device_noalloc(family, width, height, ctypes.byref(cfg), gray_device, stride,
+                                        out_device, out_capacity, count_device, stream)
+    if rc != BD_OK:
+        raise (ValueError if rc == BD_ERROR_INVALID_ARGUMENT else RuntimeError)(
+            f"bd_detect_device_noalloc failed ({rc}): {last_error()}")
+    return rc
 
 
 class Server:

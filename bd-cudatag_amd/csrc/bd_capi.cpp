@@ -429,6 +429,29 @@ int32_t bd_batch_autotune(bd_batch* batch, int32_t reps, bd_tune* best, float* b
     return BD_OK;
 }
 
+int32_t bd_detect_device_noalloc(int32_t family, int32_t width, int32_t height, const bd_config* config,
+                                 const uint8_t* gray_device, int32_t stride, bd_detection* out_device,
+                                 int32_t out_capacity, int32_t* count_device, void* stream) {
+    const char* bad = creation_problem(family, width, height);
+    if (!bad && config && config->struct_size != 0 && config->struct_size < sizeof(bd_config))
+        bad = "bd_config.struct_size is too small for this library version";
+    if (!bad && !gray_device) bad = "null device image pointer";
+    if (!bad && stride < width) bad = "stride is smaller than the image width";
+    if (!bad && !bd::frame_span_fits(width, height, stride)) bad = "device frame spans 4 GiB or more";
+    if (!bad && (out_capacity < 0 || (out_capacity > 0 && !out_device))) bad = "invalid output buffer";
+    if (!bad && !count_device) bad = "null device count pointer";
+    if (bad) {
+        set_error(bad);
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    std::string why;
+    const int rc = bd::detect_noalloc(to_params(family, config), width, height, gray_device, stride, out_device,
+                                      out_capacity, count_device, static_cast<hipStream_t>(stream), &why);
+    if (rc == 0) return BD_OK;
+    set_error(why);
+    return rc == -1 ? BD_ERROR_INVALID_ARGUMENT : BD_ERROR_CUDA;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

@@ -35,6 +35,103 @@ size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 }  // namespace
 
+int max_candidates_for(const Params& p, int W, int H) {
+    return p.max_candidates > 0 ? std::min(std::max(p.max_candidates, 16), 1024)
+                                : std::min(std::max(next_power_of_two((W * H) / 4096) * 3 / 2, 96), 1024);
+}
+
+int max_detections_for(const Params& p) { return std::min(std::max(p.max_detections, 1), 256); }
+
+int code_count_for(int family) { return family == 0 ? 587 : 30; }
+
+const uint64_t* host_codes_for(int family) { return family == 0 ? bd_codes_tag36h11 : bd_codes_tag16h5; }
+
+FrameLayout layout_frame(int W, int H, int maxc, int row_ext_log2, size_t* cursor) {
+    auto take = [cursor](size_t bytes) {
+        const size_t at = *cursor;
+        *cursor = align_up(*cursor + std::max<size_t>(bytes, 1), 256);
+        return at;
+    };
+    const int tw = (W + kTile - 1) / kTile, th = (H + kTile - 1) / kTile;
+    const int w64 = (W + 63) / 64;
+    const size_t slots = static_cast<size_t>(H) * (W / 2 + 2);
+    const size_t tiles = static_cast<size_t>(tw) * th;
+    FrameLayout o;
+    o.tmin = take(tiles);
+    o.tmax = take(tiles);
+    o.tmean = take(tiles);
+    o.tthr = take(tiles);
+    o.trlo = take(th * 4);
+    o.trhi = take(th * 4);
+    o.roi = take(th * 16);
+    o.hist = take(320 * 4);
+    o.bits = take(static_cast<size_t>(H) * w64 * 8);
+    o.rowc = take(static_cast<size_t>(H) * 4);
+    o.runs = take(slots * 4);
+    o.label = take(slots * 4);
+    o.area = take(slots * 4);
+    o.bbox = take(slots * 16);
+    o.keys = take(slots * 8);
+    o.roots = take(slots * 4);
+    o.nroots = take(static_cast<size_t>(th) * 4);
+    o.rext = take((static_cast<size_t>(H) << row_ext_log2) * 8);
+    o.rxfull = take(static_cast<size_t>(H) * 4);
+    o.cands = take(static_cast<size_t>(maxc) * 4);
+    o.quads = take(static_cast<size_t>(maxc) * 32);
+    o.hquads = take(static_cast<size_t>(maxc) * 32);
+    o.dec = take(static_cast<size_t>(maxc) * sizeof(bd_detection));
+    return o;
+}
+
+FrameDesc bind_frame(const Params& p, int W, int H, int maxc, int max_det, int row_ext_log2, uint8_t* base,
+                     const FrameLayout& o, const uint64_t* codes) {
+    FrameDesc d{};
+    d.active = 1;
+    d.width = W;
+    d.height = H;
+    d.stride = W;
+    d.tile_w = (W + kTile - 1) / kTile;
+    d.tile_h = (H + kTile - 1) / kTile;
+    d.words64 = (W + 63) / 64;
+    d.row_cap = W / 2 + 2;
+    d.row_ext_log2 = row_ext_log2;
+    d.max_candidates = maxc;
+    d.max_detections = max_det;
+    d.dim = (p.family == 0 ? 6 : 4) + 2;
+    d.code_count = code_count_for(p.family);
+    d.max_hamming = p.max_hamming >= 0 ? std::min(p.max_hamming, 3) : (p.family == 0 ? 2 : 1);
+    d.min_cluster = std::max(p.min_cluster_pixels, 1);
+    d.min_tag = std::max(p.min_tag_pixels, 4);
+    d.refine = p.refine_edges ? 1 : 0;
+    d.min_margin = p.min_decision_margin;
+    d.sharpening = p.decode_sharpening;
+    d.codes = codes;
+    d.tile_min = base + o.tmin;
+    d.tile_max = base + o.tmax;
+    d.tile_mean = base + o.tmean;
+    d.tile_thresh = base + o.tthr;
+    d.trow_lo = reinterpret_cast<int32_t*>(base + o.trlo);
+    d.trow_hi = reinterpret_cast<int32_t*>(base + o.trhi);
+    d.row_roi = reinterpret_cast<int32_t*>(base + o.roi);
+    d.hist = reinterpret_cast<uint32_t*>(base + o.hist);
+    d.bits = reinterpret_cast<uint64_t*>(base + o.bits);
+    d.row_count = reinterpret_cast<uint32_t*>(base + o.rowc);
+    d.runs = reinterpret_cast<uint32_t*>(base + o.runs);
+    d.label = reinterpret_cast<uint32_t*>(base + o.label);
+    d.area = reinterpret_cast<uint32_t*>(base + o.area);
+    d.bbox = reinterpret_cast<uint32_t*>(base + o.bbox);
+    d.cand_keys = reinterpret_cast<uint64_t*>(base + o.keys);
+    d.roots = reinterpret_cast<uint32_t*>(base + o.roots);
+    d.band_nroots = reinterpret_cast<uint32_t*>(base + o.nroots);
+    d.row_ext = reinterpret_cast<uint64_t*>(base + o.rext);
+    d.row_ext_full = reinterpret_cast<uint32_t*>(base + o.rxfull);
+    d.candidates = reinterpret_cast<uint32_t*>(base + o.cands);
+    d.quads = reinterpret_cast<float*>(base + o.quads);
+    d.hull_quads = reinterpret_cast<float*>(base + o.hquads);
+    d.dec = reinterpret_cast<bd_detection*>(base + o.dec);
+    return d;
+}
+
 Engine* Engine::create(const Params& params, const std::vector<int>& widths,
                        const std::vector<int>& heights, std::string* error) {
     Engine* e = new (std::nothrow) Engine();
@@ -95,19 +192,13 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     h_ = heights;
     const int nf = static_cast<int>(w_.size());
     // Clamps of GpuDetector::Impl::initialize (gpu_detector.cu:1906-1943).
-    max_det_ = std::min(std::max(p_.max_detections, 1), 256);
-    const int hamming = p_.max_hamming >= 0 ? std::min(p_.max_hamming, 3) : (p_.family == 0 ? 2 : 1);
-    const uint64_t* codes = p_.family == 0 ? bd_codes_tag36h11 : bd_codes_tag16h5;
-    const int code_count = p_.family == 0 ? 587 : 30;
-    const int bits = p_.family == 0 ? 6 : 4;
+    max_det_ = max_detections_for(p_);
+    const uint64_t* codes = host_codes_for(p_.family);
+    const int code_count = code_count_for(p_.family);
 
     // Arena layout: every frame's buffers, then the shared scalar and output
     // blocks (contiguous so one copy returns them for all frames).
-    struct Off {
-        size_t tmin, tmax, tmean, tthr, trlo, trhi, roi, hist, bits, rowc, runs, label, area, bbox,
-            keys, roots, nroots, rext, rxfull, cands, quads, hquads, dec;
-    };
-    std::vector<Off> off(nf);
+    std::vector<FrameLayout> off(nf);
     std::vector<int> maxc(nf);
     size_t cur = 0;
     auto take = [&cur](size_t bytes) {
@@ -122,37 +213,8 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     const int row_ext_log2 = row_ext_log2_for_width(widest_frame);
     for (int f = 0; f < nf; ++f) {
         const int W = w_[f], H = h_[f];
-        const int tw = (W + kTile - 1) / kTile, th = (H + kTile - 1) / kTile;
-        const int w64 = (W + 63) / 64;
-        const size_t slots = static_cast<size_t>(H) * (W / 2 + 2);
-        maxc[f] = p_.max_candidates > 0
-                      ? std::min(std::max(p_.max_candidates, 16), 1024)
-                      : std::min(std::max(next_power_of_two((W * H) / 4096) * 3 / 2, 96), 1024);
-        const size_t tiles = static_cast<size_t>(tw) * th;
-        Off& o = off[f];
-        o.tmin = take(tiles);
-        o.tmax = take(tiles);
-        o.tmean = take(tiles);
-        o.tthr = take(tiles);
-        o.trlo = take(th * 4);
-        o.trhi = take(th * 4);
-        o.roi = take(th * 16);
-        o.hist = take(320 * 4);
-        o.bits = take(static_cast<size_t>(H) * w64 * 8);
-        o.rowc = take(static_cast<size_t>(H) * 4);
-        o.runs = take(slots * 4);
-        o.label = take(slots * 4);
-        o.area = take(slots * 4);
-        o.bbox = take(slots * 16);
-        o.keys = take(slots * 8);
-        o.roots = take(slots * 4);
-        o.nroots = take(static_cast<size_t>(th) * 4);
-        o.rext = take((static_cast<size_t>(H) << row_ext_log2) * 8);
-        o.rxfull = take(static_cast<size_t>(H) * 4);
-        o.cands = take(static_cast<size_t>(maxc[f]) * 4);
-        o.quads = take(static_cast<size_t>(maxc[f]) * 32);
-        o.hquads = take(static_cast<size_t>(maxc[f]) * 32);
-        o.dec = take(static_cast<size_t>(maxc[f]) * sizeof(bd_detection));
+        maxc[f] = max_candidates_for(p_, W, H);
+        off[f] = layout_frame(W, H, maxc[f], row_ext_log2, &cur);
         input_off_[f] = input_bytes_;
         input_bytes_ = align_up(input_bytes_ + static_cast<size_t>(W) * H, 256);
     }
@@ -228,54 +290,10 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     desc_.assign(nf, FrameDesc{});
     std::vector<int32_t> prefix(static_cast<size_t>(SH_COUNT) * (nf + 1), 0);
     for (int f = 0; f < nf; ++f) {
-        const int W = w_[f], H = h_[f];
         FrameDesc& d = desc_[f];
-        const Off& o = off[f];
+        d = bind_frame(p_, w_[f], h_[f], maxc[f], max_det_, row_ext_log2, base, off[f], d_codes_);
         d.gray = d_input_ + input_off_[f];
-        d.active = 1;
-        d.width = W;
-        d.height = H;
-        d.stride = W;
-        d.tile_w = (W + kTile - 1) / kTile;
-        d.tile_h = (H + kTile - 1) / kTile;
-        d.words64 = (W + 63) / 64;
-        d.row_cap = W / 2 + 2;
-        d.row_ext_log2 = row_ext_log2;
-        d.max_candidates = maxc[f];
-        d.max_detections = max_det_;
-        d.dim = bits + 2;
-        d.code_count = code_count;
-        d.max_hamming = hamming;
-        d.min_cluster = std::max(p_.min_cluster_pixels, 1);
-        d.min_tag = std::max(p_.min_tag_pixels, 4);
-        d.refine = p_.refine_edges ? 1 : 0;
-        d.min_margin = p_.min_decision_margin;
-        d.sharpening = p_.decode_sharpening;
-        d.codes = d_codes_;
-        d.tile_min = base + o.tmin;
-        d.tile_max = base + o.tmax;
-        d.tile_mean = base + o.tmean;
-        d.tile_thresh = base + o.tthr;
-        d.trow_lo = reinterpret_cast<int32_t*>(base + o.trlo);
-        d.trow_hi = reinterpret_cast<int32_t*>(base + o.trhi);
-        d.row_roi = reinterpret_cast<int32_t*>(base + o.roi);
-        d.hist = reinterpret_cast<uint32_t*>(base + o.hist);
         d.scal = d_scal_ + static_cast<size_t>(f) * S_COUNT;
-        d.bits = reinterpret_cast<uint64_t*>(base + o.bits);
-        d.row_count = reinterpret_cast<uint32_t*>(base + o.rowc);
-        d.runs = reinterpret_cast<uint32_t*>(base + o.runs);
-        d.label = reinterpret_cast<uint32_t*>(base + o.label);
-        d.area = reinterpret_cast<uint32_t*>(base + o.area);
-        d.bbox = reinterpret_cast<uint32_t*>(base + o.bbox);
-        d.cand_keys = reinterpret_cast<uint64_t*>(base + o.keys);
-        d.roots = reinterpret_cast<uint32_t*>(base + o.roots);
-        d.band_nroots = reinterpret_cast<uint32_t*>(base + o.nroots);
-        d.row_ext = reinterpret_cast<uint64_t*>(base + o.rext);
-        d.row_ext_full = reinterpret_cast<uint32_t*>(base + o.rxfull);
-        d.candidates = reinterpret_cast<uint32_t*>(base + o.cands);
-        d.quads = reinterpret_cast<float*>(base + o.quads);
-        d.hull_quads = reinterpret_cast<float*>(base + o.hquads);
-        d.dec = reinterpret_cast<bd_detection*>(base + o.dec);
         d.out = out_dev + static_cast<size_t>(f) * max_det_;
         prefix[SH_TILE_ROW * (nf + 1) + f + 1] = prefix[SH_TILE_ROW * (nf + 1) + f] + d.tile_h;
         prefix[SH_FRAME * (nf + 1) + f + 1] = f + 1;

@@ -28,6 +28,37 @@ struct Params {
     float decode_sharpening = 0.25f;
 };
 
+// One frame's device buffers, as byte offsets from the base of the memory
+// that holds them (an engine's arena or the no-allocation scratch array).
+struct FrameLayout {
+    size_t tmin, tmax, tmean, tthr, trlo, trhi, roi, hist, bits, rowc, runs, label, area, bbox, keys, roots,
+        nroots, rext, rxfull, cands, quads, hquads, dec;
+};
+
+// Candidates kept per frame: the configured count clamped to [16, 1024], or
+// clamp(next_pow2(w * h / 4096) * 3 / 2, 96, 1024) when it is 0 (reference
+// gpu_detector.cu:1918-1923).
+int max_candidates_for(const Params& p, int width, int height);
+// Places one frame's buffers at *cursor (each 256-byte aligned), advancing it.
+FrameLayout layout_frame(int width, int height, int max_candidates, int row_ext_log2, size_t* cursor);
+// The descriptor of a frame whose buffers start at `base` per `o`: sizes,
+// clamped parameters (gpu_detector.cu:1906-1943) and buffer pointers.  The
+// input pointer, stride, scalar block and output records are the caller's.
+FrameDesc bind_frame(const Params& p, int width, int height, int max_candidates, int max_detections,
+                     int row_ext_log2, uint8_t* base, const FrameLayout& o, const uint64_t* codes);
+// Resolved family constants.
+int max_detections_for(const Params& p);
+int code_count_for(int family);
+const uint64_t* host_codes_for(int family);
+
+// One frame through the pipeline with no allocation (bd_noalloc.hip): the
+// working memory is a static per-device array, the work goes on `stream`
+// and the records and count land in the caller's device buffers.  Returns 0
+// when queued, -1 when the frame needs more working memory than the array
+// holds, -2 on a HIP error (*error says which).
+int detect_noalloc(const Params& p, int width, int height, const uint8_t* gray, int stride, bd_detection* out,
+                   int capacity, int32_t* count, hipStream_t stream, std::string* error);
+
 struct FrameStats {
     uint64_t frames = 0;
     float last_gpu_ms = 0.0f;

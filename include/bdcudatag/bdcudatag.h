@@ -341,6 +341,27 @@ BD_API int32_t bd_server_simulate(bd_server* server, const bd_stream_spec* strea
                                   bd_serve_report* report);
 BD_API const char* bd_server_last_error(void);
 
+/*
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, serving loops that must not call
+ * hipMalloc, harnesses that forbid it.  One width x height frame at
+ * gray_device (a HIP device pointer) is detected with the kernels of
+ * bd_detector_detect_device, queued on `stream` (a hipStream_t; NULL is the
+ * null stream), and the call returns at once: BD_OK when queued, else a
+ * negative bd_status with bd_last_error() set.  When the stream reaches it,
+ * *count_device (device memory) holds min(found, max_detections) and
+ * out_device (device memory) its first min(count, out_capacity) records,
+ * homographies included.  Nothing is allocated: the working memory is one
+ * static 48 MiB device array per GPU, held until the process ends, which
+ * fits frames up to 1920x1200; larger frames are rejected with
+ * BD_ERROR_INVALID_ARGUMENT before any GPU work.  Calls that overlap on one
+ * GPU share that array and race: serialise them (one stream per GPU).
+ */
+BD_API int32_t bd_detect_device_noalloc(int32_t family, int32_t width, int32_t height,
+                                        const bd_config* config, const uint8_t* gray_device,
+                                        int32_t stride, bd_detection* out_device,
+                                        int32_t out_capacity, int32_t* count_device, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
