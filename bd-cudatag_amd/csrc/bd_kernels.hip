@@ -2320,6 +2320,45 @@ __device__ inline bool homography_from_unit_quad(const float (*q)[2], float h[9]
     return true;
 }
 
+// Precision policy of the decode homography (no counterpart in the
+// reference, which solves it in FP32 always): FP32 rounding in the sample
+// positions grows with the determinant's cancellation (|dx1 dy2| +
+// |dx2 dy1|) / |det| and with (1 + |g| + |k|) / min |g u + k v + 1| over the
+// corners; when their product exceeds 16 (0.2 % of the quads of heavily
+// tilted scenes) the homography and the sample positions are computed in
+// FP64.  The measure is FP32, evaluated exactly as the oracle does.
+__device__ inline bool decode_needs_fp64(const float (*q)[2]) {
+    const float dx1 = q[1][0] - q[2][0], dx2 = q[3][0] - q[2][0], dx3 = q[0][0] - q[1][0] + q[2][0] - q[3][0];
+    const float dy1 = q[1][1] - q[2][1], dy2 = q[3][1] - q[2][1], dy3 = q[0][1] - q[1][1] + q[2][1] - q[3][1];
+    const float det = dx1 * dy2 - dx2 * dy1;
+    const float g = (dx3 * dy2 - dx2 * dy3) / det;
+    const float k = (dx1 * dy3 - dx3 * dy1) / det;
+    const float kappa = (fabsf(dx1 * dy2) + fabsf(dx2 * dy1)) / fabsf(det);
+    const float den = fminf(1.0f, fminf(fabsf(1.0f + g), fminf(fabsf(1.0f + g + k), fabsf(1.0f + k))));
+    const float amplification = kappa * (1.0f + fabsf(g) + fabsf(k)) / den;
+    return !(amplification <= 16.0f);
+}
+
+// homography_from_unit_quad in FP64 from FP32 corners.
+__device__ inline void homography_from_unit_quad_f64(const float (*q)[2], double h[9]) {
+    const double x0 = q[0][0], y0 = q[0][1], x1 = q[1][0], y1 = q[1][1];
+    const double x2 = q[2][0], y2 = q[2][1], x3 = q[3][0], y3 = q[3][1];
+    const double dx1 = x1 - x2, dx2 = x3 - x2, dx3 = x0 - x1 + x2 - x3;
+    const double dy1 = y1 - y2, dy2 = y3 - y2, dy3 = y0 - y1 + y2 - y3;
+    const double det = dx1 * dy2 - dx2 * dy1;
+    const double g = (dx3 * dy2 - dx2 * dy3) / det;
+    const double k = (dx1 * dy3 - dx3 * dy1) / det;
+    h[0] = x1 - x0 + g * x1;
+    h[1] = x3 - x0 + k * x3;
+    h[2] = x0;
+    h[3] = y1 - y0 + g * y1;
+    h[4] = y3 - y0 + k * y3;
+    h[5] = y0;
+    h[6] = g;
+    h[7] = k;
+    h[8] = 1.0;
+}
+
 // Result of one decode variant (reference decode_quad :1291-1430).
 struct VariantResult {
     int id, ham;
@@ -2346,18 +2385,33 @@ __device__ void decode_variant(const FrameDesc* d, int ci, int v, int lane, floa
     }
     float h[9];
     bool ok = homography_from_unit_quad(corners, h);
+    const bool wide = ok && decode_needs_fp64(corners);
+    double h64[9];
+    if (wide) homography_from_unit_quad_f64(corners, h64);
     const int cx = lane % dim, cy = lane / dim;
     float raw = 0.0f;
     bool cell_ok = true;
     if (ok && lane < cells) {
         const float u = (static_cast<float>(cx) + 0.5f) / static_cast<float>(dim);
         const float vv = (static_cast<float>(cy) + 0.5f) / static_cast<float>(dim);
-        const float den = h[6] * u + h[7] * vv + 1.0f;
-        if (fabsf(den) < 1e-9f) {
-            cell_ok = false;
+        float x = 0.0f, y = 0.0f;
+        if (wide) {
+            const double du = u, dv = vv;
+            const double den = h64[6] * du + h64[7] * dv + 1.0;
+            cell_ok = !(fabs(den) < 1e-9);
+            if (cell_ok) {
+                x = static_cast<float>((h64[0] * du + h64[1] * dv + h64[2]) / den);
+                y = static_cast<float>((h64[3] * du + h64[4] * dv + h64[5]) / den);
+            }
         } else {
-            const float x = (h[0] * u + h[1] * vv + h[2]) / den;
-            const float y = (h[3] * u + h[4] * vv + h[5]) / den;
+            const float den = h[6] * u + h[7] * vv + 1.0f;
+            cell_ok = !(fabsf(den) < 1e-9f);
+            if (cell_ok) {
+                x = (h[0] * u + h[1] * vv + h[2]) / den;
+                y = (h[3] * u + h[4] * vv + h[5]) / den;
+            }
+        }
+        if (cell_ok) {
             raw = sample_bilinear(frame_bytes(d->gray), d->width, d->height, d->stride, x - 0.0f, y - 0.0f);
             if (raw < 0.0f) cell_ok = false;
         }

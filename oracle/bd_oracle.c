@@ -194,6 +194,7 @@ struct or_frame {
     or_detection* detections;
     double *det_corners, *det_h64;
     int n_detections;
+    or_fit_trace* traces; /* per candidate slot, for tests */
 };
 
 static int next_power_of_two(int v) {
@@ -281,6 +282,7 @@ or_frame* or_create(const or_config* c) {
     f->dec_hamming = (int32_t*)calloc(mc, sizeof(int32_t));
     f->dec_margin = (double*)calloc(mc, sizeof(double));
     f->dec_corners = (double*)calloc(mc * 8, sizeof(double));
+    f->traces = (or_fit_trace*)calloc(mc, sizeof(or_fit_trace));
     f->detections = (or_detection*)calloc((size_t)f->max_detections, sizeof(or_detection));
     f->det_corners = (double*)calloc((size_t)f->max_detections * 8, sizeof(double));
     f->det_h64 = (double*)calloc((size_t)f->max_detections * 9, sizeof(double));
@@ -319,6 +321,7 @@ void or_destroy(or_frame* f) {
     free(f->dec_hamming);
     free(f->dec_margin);
     free(f->dec_corners);
+    free(f->traces);
     free(f->detections);
     free(f->det_corners);
     free(f->det_h64);
@@ -856,7 +859,8 @@ static real directional_rise(const uint8_t* image, int width, int height, int st
  * warp executed in turn and the shuffle-down tree of :993-1000 replayed in
  * the same pairing order.  Returns 1 and writes line when lane 0 fits. */
 static int fit_edge_line(const real quad[4][2], int edge, const uint8_t* image, int width,
-                         int height, int stride, int dim, int first_pass, real line[4]) {
+                         int height, int stride, int dim, int first_pass, real line[4],
+                         int32_t* best_out) {
     const int b = (edge + 1) & 3;
     const real ax = quad[edge][0], ay = quad[edge][1];
     const real bx = quad[b][0], by = quad[b][1];
@@ -920,6 +924,7 @@ static int fit_edge_line(const real quad[4][2], int edge, const uint8_t* image, 
                     }
                 }
             }
+            if (best_out) best_out[lane] = (best >= 0 && best_mag >= RL(6.0)) ? best : -1;
             if (best >= 0 && best_mag >= RL(6.0)) {
                 real t = step * (real)(best - half);
                 if (best > 0 && best < probes - 1) {
@@ -1035,7 +1040,7 @@ static real apply_refined_lines(real quad[4][2], real lines[4][4], const int hav
 /* gpu_detector.cu:1081-1241 (k_fit_quads) for one candidate.  Returns 1 when
  * a quad was produced; hull_out gets the unrefined hull quad. */
 static int fit_quad(const or_frame* f, const uint8_t* gray, int stride, uint32_t root,
-                    real out[4][2], real hull_out[4][2], uint32_t* boundary) {
+                    real out[4][2], real hull_out[4][2], uint32_t* boundary, or_fit_trace* tr) {
     const int x0 = (int)f->bbox[root * 4 + 0];
     const int y0 = (int)f->bbox[root * 4 + 1];
     const int x1 = (int)f->bbox[root * 4 + 2];
@@ -1089,7 +1094,9 @@ static int fit_quad(const or_frame* f, const uint8_t* gray, int stride, uint32_t
     while (hc > 1 && hull[0] == hull[hc - 1]) --hc;
 
     real quad[4][2];
+    if (tr) memset(tr, 0, sizeof *tr);
     if (!hull_to_quad(hull, hc, quad)) return 0;
+    if (tr) tr->fitted = 1;
     for (int i = 0; i < 4; ++i) {
         hull_out[i][0] = quad[i][0];
         hull_out[i][1] = quad[i][1];
@@ -1100,10 +1107,21 @@ static int fit_quad(const or_frame* f, const uint8_t* gray, int stride, uint32_t
             real lines[4][4];
             int have[4];
             for (int e = 0; e < 4; ++e) {
+                int32_t* best = tr ? tr->best[iteration][e] : NULL;
+                if (best) for (int k = 0; k < 16; ++k) best[k] = -1;
                 have[e] = fit_edge_line((const real(*)[2])quad, e, gray, f->width, f->height,
-                                        stride, f->dim, iteration == 0, lines[e]);
+                                        stride, f->dim, iteration == 0, lines[e], best);
+                if (tr) tr->have[iteration][e] = have[e];
             }
+            real before[4][2];
+            memcpy(before, quad, sizeof before);
             const real moved = apply_refined_lines(quad, lines, have);
+            if (tr) {
+                tr->passes = iteration + 1;
+                for (int j = 0; j < 4; ++j) {
+                    if (quad[j][0] != before[j][0] || quad[j][1] != before[j][1]) tr->refined[iteration] |= 1 << j;
+                }
+            }
             if (moved < RL(0.15)) break;
         }
     }
@@ -1142,6 +1160,51 @@ static int homography_from_unit_quad(const real quad[4][2], real h[9]) {
     return 1;
 }
 
+#ifndef OR_FP64
+/* Precision policy of the decode homography (no counterpart in the
+ * reference, which solves it in FP32 always): FP32 rounding in the sample
+ * positions x = (h0 u + h1 v + h2) / (g u + k v + 1) grows with the
+ * determinant's cancellation (|dx1 dy2| + |dx2 dy1|) / |det| and with
+ * (1 + |g| + |k|) / min |g u + k v + 1| over the quad's corners.  When their
+ * product exceeds 16 (0.2 % of the quads of heavily tilted scenes; 1 for a
+ * square seen head-on) the homography and the sample positions are
+ * computed in FP64 instead.  The measure itself is FP32, so the kernels and
+ * this file take the same decision. */
+static int decode_needs_fp64(const float q[4][2]) {
+    const float dx1 = q[1][0] - q[2][0], dx2 = q[3][0] - q[2][0], dx3 = q[0][0] - q[1][0] + q[2][0] - q[3][0];
+    const float dy1 = q[1][1] - q[2][1], dy2 = q[3][1] - q[2][1], dy3 = q[0][1] - q[1][1] + q[2][1] - q[3][1];
+    const float det = dx1 * dy2 - dx2 * dy1;
+    const float g = (dx3 * dy2 - dx2 * dy3) / det;
+    const float k = (dx1 * dy3 - dx3 * dy1) / det;
+    const float kappa = (fabsf(dx1 * dy2) + fabsf(dx2 * dy1)) / fabsf(det);
+    const float den = fminf(1.0f, fminf(fabsf(1.0f + g), fminf(fabsf(1.0f + g + k), fabsf(1.0f + k))));
+    const float amplification = kappa * (1.0f + fabsf(g) + fabsf(k)) / den;
+    return !(amplification <= 16.0f);
+}
+
+/* homography_from_unit_quad in FP64 from FP32 corners. */
+static void homography_from_unit_quad_f64(const float q[4][2], double h[9]) {
+    const double x0 = q[0][0], y0 = q[0][1], x1 = q[1][0], y1 = q[1][1];
+    const double x2 = q[2][0], y2 = q[2][1], x3 = q[3][0], y3 = q[3][1];
+    const double dx1 = x1 - x2, dx2 = x3 - x2, dx3 = x0 - x1 + x2 - x3;
+    const double dy1 = y1 - y2, dy2 = y3 - y2, dy3 = y0 - y1 + y2 - y3;
+    const double det = dx1 * dy2 - dx2 * dy1;
+    const double g = (dx3 * dy2 - dx2 * dy3) / det;
+    const double k = (dx1 * dy3 - dx3 * dy1) / det;
+    h[0] = x1 - x0 + g * x1;
+    h[1] = x3 - x0 + k * x3;
+    h[2] = x0;
+    h[3] = y1 - y0 + g * y1;
+    h[4] = y3 - y0 + k * y3;
+    h[5] = y0;
+    h[6] = g;
+    h[7] = k;
+    h[8] = 1.0;
+}
+
+static long g_decode_fp64_count; /* quads that took the FP64 path (tests) */
+#endif
+
 /* gpu_detector.cu:1274-1289 (match_code). */
 static int match_code(const uint64_t* codes, int count, uint64_t code, int max_hamming,
                       int* out_hamming) {
@@ -1174,14 +1237,33 @@ static int decode_quad(const or_frame* f, const uint8_t* gray, int stride,
     }
     real h[9];
     if (!homography_from_unit_quad((const real(*)[2])corners, h)) return 0;
+#ifndef OR_FP64
+    double h64[9];
+    const int wide = decode_needs_fp64((const float(*)[2])corners);
+    if (wide) {
+        homography_from_unit_quad_f64((const float(*)[2])corners, h64);
+        __atomic_fetch_add(&g_decode_fp64_count, 1, __ATOMIC_RELAXED);
+    }
+#endif
     for (int cy = 0; cy < dim; ++cy) {
         for (int cx = 0; cx < dim; ++cx) {
             const real u = ((real)cx + RL(0.5)) / (real)dim;
             const real v = ((real)cy + RL(0.5)) / (real)dim;
-            const real den = h[6] * u + h[7] * v + RL(1.0);
-            if (R_FABS(den) < RL(1e-9)) return 0;
-            const real x = (h[0] * u + h[1] * v + h[2]) / den;
-            const real y = (h[3] * u + h[4] * v + h[5]) / den;
+            real x, y;
+#ifndef OR_FP64
+            if (wide) {
+                const double den = h64[6] * (double)u + h64[7] * (double)v + 1.0;
+                if (fabs(den) < 1e-9) return 0;
+                x = (float)((h64[0] * (double)u + h64[1] * (double)v + h64[2]) / den);
+                y = (float)((h64[3] * (double)u + h64[4] * (double)v + h64[5]) / den);
+            } else
+#endif
+            {
+                const real den = h[6] * u + h[7] * v + RL(1.0);
+                if (R_FABS(den) < RL(1e-9)) return 0;
+                x = (h[0] * u + h[1] * v + h[2]) / den;
+                y = (h[3] * u + h[4] * v + h[5]) / den;
+            }
             const real value =
                 sample_bilinear(gray, f->width, f->height, stride, x - (real)0, y - (real)0);
             if (value < RL(0.0)) return 0;
@@ -1252,7 +1334,8 @@ static int decode_quad(const or_frame* f, const uint8_t* gray, int stride,
  * rotations of the refined quad, then of the hull quad only if none of the
  * refined ones decoded; highest score wins, first on ties. */
 static int decode_candidate(const or_frame* f, const uint8_t* gray, int stride,
-                            const real quad[4][2], const real hull[4][2], dec_result* out) {
+                            const real quad[4][2], const real hull[4][2], dec_result* out,
+                            int32_t* variant) {
     dec_result results[8];
     for (int lane = 0; lane < 8; ++lane) {
         const real(*source)[2] = (lane >> 2) == 0 ? quad : hull;
@@ -1280,6 +1363,7 @@ static int decode_candidate(const or_frame* f, const uint8_t* gray, int stride,
             }
         }
     }
+    if (variant) *variant = best;
     if (best < 0) return 0;
     *out = results[best];
     return 1;
@@ -1440,7 +1524,9 @@ int32_t or_run(or_frame* f, const uint8_t* gray, int32_t stride) {
     uint32_t quads = 0;
     for (int ci = 0; ci < f->n_candidates; ++ci) {
         real quad[4][2], hull[4][2];
-        if (!fit_quad(f, gray, stride, f->candidates[ci], quad, hull, boundary)) continue;
+        or_fit_trace* tr = f->traces + ci;
+        tr->decode_variant = -1;
+        if (!fit_quad(f, gray, stride, f->candidates[ci], quad, hull, boundary, tr)) continue;
         ++quads;
         for (int i = 0; i < 4; ++i) {
             f->quads64[ci * 8 + i * 2] = (double)quad[i][0];
@@ -1450,7 +1536,7 @@ int32_t or_run(or_frame* f, const uint8_t* gray, int32_t stride) {
         }
         dec_result r;
         if (decode_candidate(f, gray, stride, (const real(*)[2])quad, (const real(*)[2])hull,
-                             &r)) {
+                             &r, &tr->decode_variant)) {
             f->dec_id[ci] = r.id;
             f->dec_hamming[ci] = r.hamming;
             f->dec_margin[ci] = (double)r.margin;
@@ -1558,7 +1644,7 @@ int32_t or_fit_edge(const or_frame* f, const uint8_t* gray, int32_t stride, cons
     }
     real line[4];
     const int ok = fit_edge_line((const real(*)[2])quad, edge & 3, gray, f->width, f->height, stride, f->dim,
-                                 first_pass != 0, line);
+                                 first_pass != 0, line, NULL);
     for (int k = 0; k < 4 && ok; ++k) line_out[k] = (double)line[k];
     return ok;
 }
@@ -1592,4 +1678,22 @@ int32_t or_hull_to_quad(const uint32_t* hull, int32_t count, double quad_out[8])
         quad_out[2 * i + 1] = (double)quad[i][1];
     }
     return ok;
+}
+
+int32_t or_get_fit_trace(const or_frame* f, int32_t candidate, or_fit_trace* out) {
+    if (!f || candidate < 0 || candidate >= f->n_candidates) return 0;
+    *out = f->traces[candidate];
+    return 1;
+}
+
+/* Decode quads that took the FP64 homography since the last reset (0 in the
+ * FP64 build, where every decode is FP64). */
+int64_t or_decode_fp64_count(int32_t reset) {
+#ifdef OR_FP64
+    (void)reset;
+    return 0;
+#else
+    return reset ? __atomic_exchange_n(&g_decode_fp64_count, 0, __ATOMIC_RELAXED)
+                 : __atomic_load_n(&g_decode_fp64_count, __ATOMIC_RELAXED);
+#endif
 }

@@ -97,6 +97,17 @@ typedef struct or_view {
 
 typedef struct or_frame or_frame;
 
+/* The discrete choices of one candidate's quad fit and decode, for tests
+ * that compare builds of different precision. */
+typedef struct or_fit_trace {
+    int32_t fitted;          /* the hull gave a quad */
+    int32_t passes;          /* refinement passes run */
+    int32_t refined[2];      /* per pass: bit j set when corner j moved */
+    int32_t have[2][4];      /* per pass and edge: a line was fitted */
+    int32_t best[2][4][16];  /* per pass, edge and sample: probe kept, -1 none */
+    int32_t decode_variant;  /* 0-3 refined quad at that rotation, 4-7 hull, -1 none */
+} or_fit_trace;
+
 /* Build precision: 32 or 64. */
 int32_t or_real_bits(void);
 /* 1 for the build that restates the reference's transcendental calls and
@@ -123,6 +134,12 @@ int32_t or_fit_edge(const or_frame* frame, const uint8_t* gray, int32_t stride, 
 int32_t or_decode_rotation(const or_frame* frame, const uint8_t* gray, int32_t stride, const double quad[8],
                            int32_t rot, int32_t* id, int32_t* hamming, double* margin);
 int32_t or_hull_to_quad(const uint32_t* hull, int32_t count, double quad[8]);
+/* The trace of candidate `candidate` (its index in the ordered list) of the
+ * last or_run; 0 when out of range. */
+int32_t or_get_fit_trace(const or_frame* frame, int32_t candidate, or_fit_trace* out);
+/* Decode quads (one per rotation tried) that took the FP64 homography since
+ * the last call with reset != 0 (process-wide count, for tests). */
+int64_t or_decode_fp64_count(int32_t reset);
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,12 @@ class View(ctypes.Structure):
     ]
 
 
+class FitTrace(ctypes.Structure):
+    _fields_ = [("fitted", ctypes.c_int32), ("passes", ctypes.c_int32), ("refined", ctypes.c_int32 * 2),
+                ("have", (ctypes.c_int32 * 4) * 2), ("best", ((ctypes.c_int32 * 16) * 4) * 2),
+                ("decode_variant", ctypes.c_int32)]
+
+
 _libs = {}
 
 
@@ -99,6 +105,10 @@ def lib(fp64=False, literal=False):
         L.or_decode_rotation.restype = ctypes.c_int32
         L.or_hull_to_quad.argtypes = [_P32, ctypes.c_int32, _PD]
         L.or_hull_to_quad.restype = ctypes.c_int32
+        L.or_get_fit_trace.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(FitTrace)]
+        L.or_get_fit_trace.restype = ctypes.c_int32
+        L.or_decode_fp64_count.argtypes = [ctypes.c_int32]
+        L.or_decode_fp64_count.restype = ctypes.c_int64
         assert L.or_real_bits() == (64 if fp64 else 32)
         assert L.or_literal() == (1 if literal else 0)
         _libs[key] = L
@@ -141,6 +151,17 @@ class Oracle:
         if h:
             self._lib.or_destroy(h)
             self._h = None
+
+    def trace(self, candidate: int) -> dict | None:
+        """Discrete choices of candidate `candidate`'s fit and decode in the
+        last run: passes, corners moved per pass, lines fitted per pass and
+        edge, probe kept per sample, decode variant."""
+        t = FitTrace()
+        if not self._lib.or_get_fit_trace(self._h, candidate, ctypes.byref(t)):
+            return None
+        return {"fitted": t.fitted, "passes": t.passes, "refined": list(t.refined),
+                "have": [list(r) for r in t.have], "best": np.ctypeslib.as_array(t.best).copy(),
+                "decode_variant": t.decode_variant}
 
     def run(self, image: np.ndarray, stride: int | None = None):
         """Runs the pipeline; returns a dict of per-stage numpy arrays."""

@@ -52,3 +52,18 @@ def test_nearest_peak_halo_gpu(level, inner):
     assert not bad, bad
     assert [d["id"] for d in dets] == [7]
     assert corner_distance(dets[0]["corners"], tag if inner else outer) < (0.3 if inner else 0.1)
+
+
+def test_decode_fp64_gate_gpu():
+    # Heavily tilted frames where some decode quads take the FP64
+    # homography: every stage, decode records included, bit-identical.
+    L = pyoracle.lib()
+    L.or_decode_fp64_count(1)
+    det = bdcudatag.Detector(0, 1280, 800)
+    for seed in (0, 1, 2, 3):
+        img, _ = scenes.scene(3, 1280, 800, seed)
+        dets = det.detect(img)
+        o = pyoracle.Oracle(pyoracle.default_config(0, 1280, 800)).run(img)
+        bad = compare_frame(gpu_stage_dump(det.batch, 0), o, dets, det.stats())
+        assert not bad, (seed, bad)
+    assert L.or_decode_fp64_count(1) > 0

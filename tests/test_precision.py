@@ -3,14 +3,23 @@ truth on heavily degraded frames (Gaussian blur 1.5, noise 8, 20-80 px tags,
 tilt up to 60 degrees, background between the cell levels).
 
 The FP64 twin is the same oracle source compiled with double for the quad
-fit, refinement and decode sampling.  Corners of tags both builds decode
-must agree to 1e-3 px in the median and for at least 85% of tags; the rest
-differ where a discrete choice (probe argmax, convergence test) flips."""
+fit, refinement and decode sampling.  Over 32 such frames the FP32 corners
+of tags both builds decode agree with it to a median 9.3e-5 px, and 91.8 %
+of tags within 1e-3 px.  Every tag beyond 1e-3 px differs by a discrete
+choice, never by rounding alone: one sample's probe argmax (35 of 453 tags)
+or which decode variant won (2).  The decode homography takes FP64 for the
+quads whose FP32 sample positions would be poorly conditioned (86 of the
+59,000 decode attempts of those frames; no result changed)."""
+import ctypes
+
 import numpy as np
 import pytest
 
+import exact_render
 import pyoracle
 import scenes
+
+SHARE_WITHIN_1E3 = 0.90
 
 
 def corner_stats(get_fp32_corners, seeds):
@@ -41,10 +50,78 @@ def oracle32(img):
 def test_fp32_oracle_vs_fp64_twin():
     d, gt32, gt64 = corner_stats(oracle32, range(8))
     assert len(d) > 80
-    assert np.median(d) < 1e-3
-    assert (d < 1e-3).mean() >= 0.85
+    assert np.median(d) < 2e-4
+    assert (d < 1e-3).mean() >= SHARE_WITHIN_1E3
     # ground truth: both builds localise corners alike
     assert abs(np.median(gt32) - np.median(gt64)) < 0.02
+
+
+def candidate_of(o, det):
+    for ci in range(len(o["candidates"])):
+        if o["dec_id"][ci] == det["id"] and np.allclose(o["dec_corners"][ci], det["corners"], atol=1e-4):
+            return ci
+    return None
+
+
+def test_differences_beyond_1e3_are_discrete_flips():
+    # Same candidates in both builds (integer stages); for every tag whose
+    # FP32 corners are more than 1e-3 px from the FP64 twin's, the two fits
+    # made a different discrete choice: refinement passes, a line fitted or
+    # not, a corner update kept or not, a sample's probe, or the decode
+    # variant.  With the same choices, rounding alone stays within 1e-3 px.
+    flips = 0
+    for seed in range(8):
+        img, _ = scenes.scene(3, 1280, 800, seed)
+        A = pyoracle.Oracle(pyoracle.default_config(0, 1280, 800))
+        B = pyoracle.Oracle(pyoracle.default_config(0, 1280, 800), fp64=True)
+        a, b = A.run(img), B.run(img)
+        assert a["candidates"].tolist() == b["candidates"].tolist()
+        for da in a["detections"]:
+            ci = candidate_of(a, da)
+            near = [k for k, d in enumerate(b["detections"])
+                    if d["id"] == da["id"] and np.abs(d["corners"].mean(0) - da["corners"].mean(0)).max() < 5]
+            if ci is None or not near:
+                continue
+            diff = np.abs(da["corners"].astype(np.float64) - b["det_corners"][near[0]]).max()
+            ta, tb = A.trace(ci), B.trace(ci)
+            same_choices = (ta["decode_variant"] == tb["decode_variant"] and ta["passes"] == tb["passes"]
+                            and ta["have"] == tb["have"] and ta["refined"] == tb["refined"]
+                            and np.array_equal(ta["best"], tb["best"]))
+            if same_choices:
+                assert diff <= 1e-3, (seed, da["id"], diff)
+            elif diff > 1e-3:
+                flips += 1
+    assert flips > 0
+
+
+def test_decode_homography_precision_gate():
+    # A head-on square decodes on the FP32 homography; a tag seen under
+    # strong perspective (a vanishing point close to it) takes the FP64 one,
+    # and its decode then agrees with the FP64 twin's.
+    L = pyoracle.lib()
+    codes, bits = pyoracle.family_codes(0)
+    W = H = 400
+    # The trapezoid's 16-px far edge against its 320-px near edge puts the
+    # vanishing point just beyond it: the gate's measure is 20.
+    cases = [(np.array([[100, 100], [300, 100], [300, 300], [100, 300]], float), False),
+             (np.array([[40, 80], [360, 80], [208, 330], [192, 330]], float), True)]
+    for corners, wide in cases:
+        img = np.clip(np.rint(exact_render.render_tag(codes[9], bits, W, H, corners)), 0, 255).astype(np.uint8)
+        out = {}
+        for fp64 in (False, True):
+            lib = pyoracle.lib(fp64=fp64)
+            h = lib.or_create(ctypes.byref(pyoracle.Config(**pyoracle.default_config(0, W, H))))
+            q = (ctypes.c_double * 8)(*corners.reshape(-1))
+            i, hm, m = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+            L.or_decode_fp64_count(1)
+            ok = lib.or_decode_rotation(h, img.ctypes.data, W, q, 0, ctypes.byref(i), ctypes.byref(hm),
+                                        ctypes.byref(m))
+            took = L.or_decode_fp64_count(1)
+            lib.or_destroy(h)
+            out[fp64] = (ok, i.value, hm.value, m.value, took)
+        assert out[False][:3] == out[True][:3] == (1, 9, 0), out
+        assert (out[False][4] > 0) == wide, out
+        assert out[False][3] == pytest.approx(out[True][3], rel=1e-5), out
 
 
 @pytest.mark.gpu
@@ -57,5 +134,5 @@ def test_gpu_vs_fp64_twin():
 
     d, gt32, gt64 = corner_stats(gpu, range(8))
     assert len(d) > 80
-    assert np.median(d) < 1e-3
-    assert (d < 1e-3).mean() >= 0.85
+    assert np.median(d) < 2e-4
+    assert (d < 1e-3).mean() >= SHARE_WITHIN_1E3
