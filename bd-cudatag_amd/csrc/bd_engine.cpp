@@ -208,13 +208,10 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     };
     input_off_.assign(nf, 0);
     input_bytes_ = 0;
-    int widest_frame = 0;
-    for (int f = 0; f < nf; ++f) widest_frame = std::max(widest_frame, w_[f]);
-    const int row_ext_log2 = row_ext_log2_for_width(widest_frame);
     for (int f = 0; f < nf; ++f) {
         const int W = w_[f], H = h_[f];
         maxc[f] = max_candidates_for(p_, W, H);
-        off[f] = layout_frame(W, H, maxc[f], row_ext_log2, &cur);
+        off[f] = layout_frame(W, H, maxc[f], row_ext_log2_for_width(W), &cur);
         input_off_[f] = input_bytes_;
         input_bytes_ = align_up(input_bytes_ + static_cast<size_t>(W) * H, 256);
     }
@@ -291,7 +288,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     std::vector<int32_t> prefix(static_cast<size_t>(SH_COUNT) * (nf + 1), 0);
     for (int f = 0; f < nf; ++f) {
         FrameDesc& d = desc_[f];
-        d = bind_frame(p_, w_[f], h_[f], maxc[f], max_det_, row_ext_log2, base, off[f], d_codes_);
+        d = bind_frame(p_, w_[f], h_[f], maxc[f], max_det_, row_ext_log2_for_width(w_[f]), base, off[f], d_codes_);
         d.gray = d_input_ + input_off_[f];
         d.scal = d_scal_ + static_cast<size_t>(f) * S_COUNT;
         d.out = out_dev + static_cast<size_t>(f) * max_det_;
@@ -301,6 +298,11 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         const int rows_per_wg = pairs >= 256 ? 1 : 256 / pairs;
         prefix[SH_TSTAT * (nf + 1) + f + 1] =
             prefix[SH_TSTAT * (nf + 1) + f] + (d.tile_h + rows_per_wg - 1) / rows_per_wg;
+        for (int c = 0; c < kWidthClasses; ++c) {
+            const int32_t* pc = prefix.data() + static_cast<size_t>(SH_CLASS + c) * (nf + 1);
+            const bool mine = d.row_ext_log2 == kRowExtLog2Min + c;
+            prefix[static_cast<size_t>(SH_CLASS + c) * (nf + 1) + f + 1] = pc[f] + (mine ? d.tile_h : 0);
+        }
     }
     std::memcpy(h_desc_dev_, desc_.data(), desc_.size() * sizeof(FrameDesc));
     // Per-lane prefix tables: the whole-batch table restricted to the lane's
@@ -342,18 +344,22 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         return false;
     }
 
+    plan_host_.tune = default_tune();
     plan_host_.frames = d_desc_host_;
     plan_host_.prefix = d_prefix_;
     plan_host_.nframes = nf;
     plan_host_.work = d_work_;
     plan_host_.work_cap = static_cast<int32_t>(work_cap);
-    const auto band_runs_of = [this](int first, int n) {
-        int widest = 0;
-        for (int f = first; f < first + n; ++f) widest = std::max(widest, w_[f]);
-        return band_runs_for_width(widest);
+    // Each width class's LDS band capacity, from its widest frame in the plan.
+    const auto set_band_runs = [this](Plan* plan, int first, int n) {
+        for (int c = 0; c < kWidthClasses; ++c) {
+            int widest = 0;
+            for (int f = first; f < first + n; ++f)
+                if (row_ext_log2_for_width(w_[f]) == kRowExtLog2Min + c) widest = std::max(widest, w_[f]);
+            plan->class_band_runs[c] = widest > 0 ? band_runs_for_width(widest) : 0;
+        }
     };
-    plan_host_.band_runs = band_runs_of(0, nf);
-    plan_host_.row_ext_log2 = row_ext_log2;
+    set_band_runs(&plan_host_, 0, nf);
     for (int k = 0; k < SH_COUNT; ++k) plan_host_.blocks[k] = prefix[k * (nf + 1) + nf];
     plan_dev_ = plan_host_;
     plan_dev_.frames = d_desc_dev_;
@@ -366,7 +372,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         p.work = reinterpret_cast<uint32_t*>(base + o_lane_work[l]);
         p.nframes = n;
         p.work_cap = static_cast<int32_t>(lane_cap[l]);
-        p.band_runs = band_runs_of(lane_first[l], n);
+        set_band_runs(&p, lane_first[l], n);
         for (int k = 0; k < SH_COUNT; ++k) p.blocks[k] = lane_prefix[l][static_cast<size_t>(k) * (n + 1) + n];
         lane_dev_[l] = p;
         lane_dev_[l].frames = d_desc_dev_ + lane_first[l];

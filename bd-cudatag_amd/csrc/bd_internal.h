@@ -54,12 +54,15 @@ inline int band_runs_for_width(int width) {
     const int want = (width * 8 / 5 + 511) / 512 * 512;
     return want < kBandRunsMin ? kBandRunsMin : want > kBandRunsMax ? kBandRunsMax : want;
 }
-// Component slots per row of the row-extreme tables (log2), one value per
-// batch from its widest frame: the least power of two of at least width / 20,
-// 64 to 256.  Rows of noisy eight-tag scenes hold 8.7 components on average
-// and 44 at most at 1280 px (64 slots), 28 and 48 at 1920 px (128), 68 and
-// 100 at 4096 px (256); a row with more components than slots is scanned
-// whole by k_fit.  k_stats is compiled once per size.
+// Component slots per row of the row-extreme tables (log2), per frame from
+// its width: the least power of two of at least width / 20, 64 to 256.  Rows
+// of noisy eight-tag scenes hold 8.7 components on average and 44 at most
+// at 1280 px (64 slots), 28 and 48 at 1920 px (128), 68 and 100 at 4096 px
+// (256); a row with more components than slots is scanned whole by k_fit.
+// The three sizes are the plan's width classes: k_stats is compiled once
+// per size, and k_binarize_runs and k_stats are launched once per class
+// present, each launch over that class's frames only, with the class's own
+// LDS band capacity (from its widest frame).
 constexpr int kRowExtLog2Min = 6, kRowExtLog2Max = 8;
 inline int row_ext_log2_for_width(int width) {
     int lg = kRowExtLog2Min;
@@ -124,7 +127,10 @@ struct FrameDesc {
 
 // Prefix tables over frames: prefix[k][f] = first block of frame f for
 // kernel shape k; prefix[k][nframes] = total blocks.
-enum Shape : int { SH_TILE_ROW = 0, SH_FRAME = 1, SH_TSTAT = 2, SH_COUNT = 3 };
+// SH_CLASS + c: tile rows of the frames of width class c (row-table log2
+// kRowExtLog2Min + c), zero blocks for every other frame.
+constexpr int kWidthClasses = kRowExtLog2Max - kRowExtLog2Min + 1;
+enum Shape : int { SH_TILE_ROW = 0, SH_FRAME = 1, SH_TSTAT = 2, SH_CLASS = 3, SH_COUNT = SH_CLASS + kWidthClasses };
 
 // Batch candidate list: a small header, then one entry per selected
 // candidate of any frame (frame << 10 | candidate index), appended by
@@ -158,8 +164,9 @@ struct Plan {
     uint32_t* work;         // kWorkItems + kHeadSlots * nframes + work_cap entries
     int32_t nframes;
     int32_t work_cap;       // sum of max_candidates over frames
-    int32_t band_runs;      // LDS band capacity of k_binarize_runs (runs)
-    int32_t row_ext_log2;   // the batch's component slots per row (log2)
+    // LDS band capacity of k_binarize_runs (runs) per width class, from the
+    // widest frame of the class in this plan (0: no frame of the class).
+    int32_t class_band_runs[kWidthClasses];
     Tune tune;
     int32_t blocks[SH_COUNT];
 };
@@ -167,6 +174,10 @@ struct Plan {
 // Kernels in pipeline order: tile_stats, frame_ref, binarize_runs, merge,
 // stats, collect, select, fit, decode, finalize.
 constexpr int kStageCount = 10;
+
+// Default launch parameters: persistent grids of one full round of the
+// workgroups the device's CUs hold for the compiled k_fit and k_decode.
+Tune default_tune();
 
 // Enqueues one kernel (0..kStageCount-1) for every frame of the plan.
 hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream);

@@ -22,6 +22,7 @@
 // to it as well.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cstdint>
 
@@ -54,6 +55,23 @@ __constant__ float c_dir_sin[32] = {
     -0x1p+0f,         -0x1.f6297cp-1f,  -0x1.d906bap-1f,  -0x1.a9b65ep-1f,
     -0x1.6a09dep-1f,  -0x1.1c73b6p-1f,  -0x1.87de2ap-2f,  -0x1.8f8b78p-3f,
 };
+
+// Frame bytes in the global address space, addressed by 32-bit offsets (the
+// host rejects device frames spanning 4 GiB or more), so that loads take
+// the frame base from scalar registers and are issued as global loads.
+typedef const uint8_t __attribute__((address_space(1))) gbyte;
+typedef unsigned int vec4u __attribute__((ext_vector_type(4)));
+typedef const vec4u __attribute__((address_space(1))) gvec16;
+
+__device__ inline const gbyte* frame_bytes(const uint8_t* gray) { return (const gbyte*)gray; }
+
+typedef const uint64_t __attribute__((address_space(1))) gword64;
+
+// 16 frame bytes at byte offset o (16-byte aligned frame rows).
+__device__ inline uint4 load16(const gbyte* image, size_t o) {
+    const vec4u v = *reinterpret_cast<const gvec16*>(image + o);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
 
 // Frame of a block for one grid shape (prefix tables are tiny: one entry
 // per frame).
@@ -213,9 +231,23 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
             uint32_t mnA = 255, mxA = 0, sumA = 0, mnB = 255, mxB = 0, sumB = 0;
             const int nA = min(8, W - x0);
             const int nB = min(8, W - x0 - 8);
-            if (vec && x0 + 16 <= W) {
+            if (vec && x0 + 16 <= W && rows == kTile) {
+                // A whole 16x8 block: all eight row loads in flight before
+                // any is used.
+                const gbyte* src = frame_bytes(gray) + static_cast<size_t>(y0) * stride + x0;
+                uint4 v[kTile];
+#pragma unroll
+                for (int r = 0; r < kTile; ++r) v[r] = load16(src, static_cast<size_t>(r) * stride);
+#pragma unroll
+                for (int r = 0; r < kTile; ++r) {
+                    acc_word(v[r].x, mnA, mxA, sumA);
+                    acc_word(v[r].y, mnA, mxA, sumA);
+                    acc_word(v[r].z, mnB, mxB, sumB);
+                    acc_word(v[r].w, mnB, mxB, sumB);
+                }
+            } else if (vec && x0 + 16 <= W) {
                 for (int y = y0; y < y1; ++y) {
-                    const uint4 v = *reinterpret_cast<const uint4*>(gray + static_cast<size_t>(y) * stride + x0);
+                    const uint4 v = load16(frame_bytes(gray), static_cast<size_t>(y) * stride + x0);
                     acc_word(v.x, mnA, mxA, sumA);
                     acc_word(v.y, mnA, mxA, sumA);
                     acc_word(v.z, mnB, mxB, sumB);
@@ -485,12 +517,14 @@ __device__ inline uint32_t lds_union(uint32_t* lab, uint32_t a, uint32_t b) {
 // differ and else by the low seven bits, whose difference borrows into the
 // top bit (bytes cannot borrow from each other: each top bit is set before
 // the subtraction).  The four flags are gathered by one multiply.
+// No pixel is above 255: that threshold is selected at the end rather than
+// branched on, so the loads feeding the compare are never predicated.
 __device__ inline uint32_t bytes_above(uint32_t x, uint32_t t) {
-    if (t >= 255u) return 0u;
-    const uint32_t t1 = (t + 1u) * 0x01010101u;
+    const uint32_t t1 = (min(t, 254u) + 1u) * 0x01010101u;
     const uint32_t low = (x | 0x80808080u) - (t1 & 0x7f7f7f7fu);
     const uint32_t ge = ((x ^ t1) & x) | (~(x ^ t1) & low);
-    return ((ge & 0x80808080u) * 0x00204081u) >> 28;
+    const uint32_t flags = ((ge & 0x80808080u) * 0x00204081u) >> 28;
+    return t >= 255u ? 0u : flags;
 }
 
 // One workgroup per tile row (8 pixel rows).
@@ -511,9 +545,9 @@ __device__ inline uint32_t bytes_above(uint32_t x, uint32_t t) {
 //     more runs than the LDS buffer leaves every run a root and lets k_merge
 //     join all of its rows.
 template <int kChunkWords>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_binarize_runs(Plan plan) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_binarize_runs(Plan plan, int cls) {
     int ty;
-    const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
+    const FrameDesc* d = plan.frames + frame_of(plan, SH_CLASS + cls, blockIdx.x, &ty);
     if (!d->active) return;
     const int W = d->width, H = d->height, tw = d->tile_w, th = d->tile_h;
     const int w64 = d->words64, row_cap = d->row_cap, stride = d->stride;
@@ -526,9 +560,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     __shared__ uint8_t thr[1024];
     __shared__ uint32_t s_off[kTile + 1];
     __shared__ uint32_t s_cnt[kTile];  // runs per band row
-    // Band arrays in dynamic LDS: plan.band_runs runs each.
+    // Band arrays in dynamic LDS: the width class's band capacity each.
     extern __shared__ __align__(16) uint32_t band_lds[];
-    const int band_cap = plan.band_runs;
+    const int band_cap = plan.class_band_runs[cls];
     uint32_t* b_run = band_lds;
     uint32_t* b_lab = band_lds + band_cap;
     // Short-lived buffers share storage with the band arrays: the tile
@@ -597,6 +631,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         for (int cw = 0; cw < kChunkWords; ++cw) {
             const int w = (it - r * wchunks) * kChunkWords + cw;
             if (w >= w64) break;
+            // The word's 64 pixels as four 16-byte loads (two per 32-px
+            // group inside the row's ROI), all issued before any is used.
+            const gbyte* row_px = frame_bytes(gray) + static_cast<size_t>(y) * stride;
+            // The pixel left of the word (for its run starts), loaded with
+            // the word: thresholded by the same rule, white outside the
+            // row's ROI groups.
+            const int tile0p = (2 * w - 1) * 4;
+            const bool left_on = y != 0 && w > 0 && tile0p <= re && tile0p + 3 >= rs;
+            const uint32_t left_px = left_on ? static_cast<uint32_t>(row_px[64 * w - 1]) : 0u;
+            bool full[2];
+            uint4 px[2][2];
+#pragma unroll
+            for (int g = 0; g < 2; ++g) {
+                const int grp = 2 * w + g;
+                const int xs = grp * 32;
+                const int tile0 = grp * 4;
+                full[g] = vec && xs + 32 <= W && tile0 <= re && tile0 + 3 >= rs;
+                px[g][0] = px[g][1] = make_uint4(0u, 0u, 0u, 0u);
+                if (full[g]) {
+                    px[g][0] = load16(row_px, xs);
+                    px[g][1] = load16(row_px, xs + 16);
+                }
+            }
             uint64_t m = ~0ull;
             for (int g = 0; g < 2; ++g) {
                 const int grp = 2 * w + g;
@@ -606,10 +663,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                 if (!(tile0 <= re && tile0 + 3 >= rs)) continue;  // 32-px ROI test (:376-377)
                 const uint8_t* src = gray + static_cast<size_t>(y) * stride + xs;
                 uint32_t m32 = 0;
-                if (vec && xs + 32 <= W) {
+                if (full[g]) {
+#pragma unroll
                     for (int q = 0; q < 2; ++q) {
-                        const uint4 v = *reinterpret_cast<const uint4*>(src + 16 * q);
+                        const uint4 v = px[g][q];
                         const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
                         for (int j = 0; j < 4; ++j) {
                             const uint32_t tv = thr[(xs >> 3) + 2 * q + (j >> 1)];
                             m32 |= bytes_above(wd[j], tv) << (16 * q + 4 * j);
@@ -639,11 +698,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             uint64_t prev = 1ull;
             if (y == 0) {
                 prev = 1ull | (1ull << 32);
-            } else if (w > 0) {
-                const int xp = 64 * w - 1;
-                const int tile0p = (2 * w - 1) * 4;
-                if (tile0p <= re && tile0p + 3 >= rs)
-                    prev = gray[static_cast<size_t>(y) * stride + xp] > thr[xp >> 3] ? 1ull : 0ull;
+            } else if (left_on) {
+                prev = left_px > thr[(64 * w - 1) >> 3] ? 1ull : 0ull;
             }
             atomicAdd(&s_cnt[r], static_cast<uint32_t>(__popcll(~m & ((m << 1) | prev))));
             for (int h = 0; h < 2; ++h) {
@@ -1053,7 +1109,7 @@ template <int kRowLog2>
 __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     constexpr int kRowSlots = 1 << kRowLog2;
     int ty;
-    const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
+    const FrameDesc* d = plan.frames + frame_of(plan, SH_CLASS + (kRowLog2 - kRowExtLog2Min), blockIdx.x, &ty);
     if (!d->active) return;
     const int32_t* roi = d->row_roi + 4 * ty;
     if (!roi[0]) {
@@ -1598,14 +1654,6 @@ __global__ __launch_bounds__(256) void k_select(Plan plan) {
 // (d) quad fit
 // ---------------------------------------------------------------------------
 
-// Frame bytes in the global address space, addressed by 32-bit offsets (the
-// host rejects device frames spanning 4 GiB or more), so that loads take
-// the frame base from scalar registers.
-typedef const uint8_t __attribute__((address_space(1))) gbyte;
-
-__device__ inline const gbyte* frame_bytes(const uint8_t* gray) { return (const gbyte*)gray; }
-
-typedef const uint64_t __attribute__((address_space(1))) gword64;
 
 // Bytes of a frame from its base to the last pixel, (height - 1) * stride +
 // width.
@@ -2327,12 +2375,11 @@ __device__ inline bool homography_from_unit_quad(const float (*q)[2], float h[9]
 // corners; when their product exceeds 16 (0.2 % of the quads of heavily
 // tilted scenes) the homography and the sample positions are computed in
 // FP64.  The measure is FP32, evaluated exactly as the oracle does.
-__device__ inline bool decode_needs_fp64(const float (*q)[2]) {
-    const float dx1 = q[1][0] - q[2][0], dx2 = q[3][0] - q[2][0], dx3 = q[0][0] - q[1][0] + q[2][0] - q[3][0];
-    const float dy1 = q[1][1] - q[2][1], dy2 = q[3][1] - q[2][1], dy3 = q[0][1] - q[1][1] + q[2][1] - q[3][1];
+// g and k are the FP32 homography's h[6] and h[7] (the same expressions).
+__device__ inline bool decode_needs_fp64(const float (*q)[2], float g, float k) {
+    const float dx1 = q[1][0] - q[2][0], dx2 = q[3][0] - q[2][0];
+    const float dy1 = q[1][1] - q[2][1], dy2 = q[3][1] - q[2][1];
     const float det = dx1 * dy2 - dx2 * dy1;
-    const float g = (dx3 * dy2 - dx2 * dy3) / det;
-    const float k = (dx1 * dy3 - dx3 * dy1) / det;
     const float kappa = (fabsf(dx1 * dy2) + fabsf(dx2 * dy1)) / fabsf(det);
     const float den = fminf(1.0f, fminf(fabsf(1.0f + g), fminf(fabsf(1.0f + g + k), fabsf(1.0f + k))));
     const float amplification = kappa * (1.0f + fabsf(g) + fabsf(k)) / den;
@@ -2385,9 +2432,7 @@ __device__ void decode_variant(const FrameDesc* d, int ci, int v, int lane, floa
     }
     float h[9];
     bool ok = homography_from_unit_quad(corners, h);
-    const bool wide = ok && decode_needs_fp64(corners);
-    double h64[9];
-    if (wide) homography_from_unit_quad_f64(corners, h64);
+    const bool wide = ok && decode_needs_fp64(corners, h[6], h[7]);
     const int cx = lane % dim, cy = lane / dim;
     float raw = 0.0f;
     bool cell_ok = true;
@@ -2396,6 +2441,8 @@ __device__ void decode_variant(const FrameDesc* d, int ci, int v, int lane, floa
         const float vv = (static_cast<float>(cy) + 0.5f) / static_cast<float>(dim);
         float x = 0.0f, y = 0.0f;
         if (wide) {
+            double h64[9];
+            homography_from_unit_quad_f64(corners, h64);
             const double du = u, dv = vv;
             const double den = h64[6] * du + h64[7] * dv + 1.0;
             cell_ok = !(fabs(den) < 1e-9);
@@ -2737,29 +2784,52 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
 
 }  // namespace
 
+Tune default_tune() {
+    // The persistent grids default to whole rounds of resident workgroups:
+    // k_fit five per CU (its occupancy target, measured faster than six),
+    // k_decode as many as one CU holds of the compiled kernel (its registers
+    // decide).  Computed once per process.
+    static const Tune tuned = [] {
+        Tune t;
+        int dev = 0, cus = 0, dec = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            return t;
+        t.fit_grid = std::min(std::max(cus * 5, 64), 16384);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&dec, k_decode<8>, 512, 0) == hipSuccess && dec > 0)
+            t.decode_grid = std::min(std::max(cus * dec, 64), 16384);
+        return t;
+    }();
+    return tuned;
+}
+
 hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
     const int nf = plan.nframes;
     switch (stage) {
         case 0: hipLaunchKernelGGL(k_tile_stats, dim3(plan.blocks[SH_TSTAT]), dim3(256), 0, stream, plan); break;
         case 1: hipLaunchKernelGGL(k_frame_ref, dim3(nf), dim3(256), 0, stream, plan); break;
         case 2:
-            if (plan.tune.chunk_px == 128)
-                hipLaunchKernelGGL(k_binarize_runs<2>, dim3(plan.blocks[SH_TILE_ROW]), dim3(256),
-                                   static_cast<size_t>(plan.band_runs) * 8, stream, plan);
-            else
-                hipLaunchKernelGGL(k_binarize_runs<1>, dim3(plan.blocks[SH_TILE_ROW]), dim3(256),
-                                   static_cast<size_t>(plan.band_runs) * 8, stream, plan);
+            // One launch per width class present, with the class's band capacity.
+            for (int c = 0; c < kWidthClasses; ++c) {
+                const int blocks = plan.blocks[SH_CLASS + c];
+                if (blocks == 0) continue;
+                const size_t lds = static_cast<size_t>(plan.class_band_runs[c]) * 8;
+                if (plan.tune.chunk_px == 128)
+                    hipLaunchKernelGGL(k_binarize_runs<2>, dim3(blocks), dim3(256), lds, stream, plan, c);
+                else
+                    hipLaunchKernelGGL(k_binarize_runs<1>, dim3(blocks), dim3(256), lds, stream, plan, c);
+            }
             break;
         case 3: hipLaunchKernelGGL(k_merge, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
         case 4: {
-            const dim3 grid(plan.blocks[SH_TILE_ROW]);
-            if (plan.row_ext_log2 <= 6) {
-                hipLaunchKernelGGL(k_stats<6>, grid, dim3(256), 0, stream, plan);
-            } else if (plan.row_ext_log2 == 7) {
-                hipLaunchKernelGGL(k_stats<7>, grid, dim3(256), 0, stream, plan);
-            } else {
-                hipLaunchKernelGGL(k_stats<8>, grid, dim3(256), 0, stream, plan);
-            }
+            // One launch per width class present, with the class's row tables.
+            static_assert(kWidthClasses == 3, "one k_stats instance per width class");
+            if (plan.blocks[SH_CLASS + 0])
+                hipLaunchKernelGGL(k_stats<kRowExtLog2Min + 0>, dim3(plan.blocks[SH_CLASS + 0]), dim3(256), 0, stream, plan);
+            if (plan.blocks[SH_CLASS + 1])
+                hipLaunchKernelGGL(k_stats<kRowExtLog2Min + 1>, dim3(plan.blocks[SH_CLASS + 1]), dim3(256), 0, stream, plan);
+            if (plan.blocks[SH_CLASS + 2])
+                hipLaunchKernelGGL(k_stats<kRowExtLog2Min + 2>, dim3(plan.blocks[SH_CLASS + 2]), dim3(256), 0, stream, plan);
             break;
         }
         case 5: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;

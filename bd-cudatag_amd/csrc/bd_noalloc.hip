@@ -6,7 +6,7 @@
 // queued on the caller's stream; everything else the pipeline needs (the
 // frame's stage buffers, its descriptor, the candidate work list, the
 // scalars and the codeword table) lives in `scratch`, a file-scope device
-// array of kScratchBytes that every GPU the library runs on holds until the
+// array of kNoAllocScratchBytes that every GPU the library runs on holds until the
 // process ends.  The kernels are the detector's own, so results are those of
 // bd_detector_detect_device.
 //
@@ -145,10 +145,13 @@ int detect_noalloc(const Params& p, int width, int height, const uint8_t* gray, 
     a.work = reinterpret_cast<uint32_t*>(base + o_work);
     a.codes = reinterpret_cast<uint64_t*>(base + o_codes);
     a.family = p.family;
-    // One frame: every prefix table is {0, blocks of the frame}.
+    // One frame: every prefix table is {0, blocks of the frame}; only the
+    // frame's own width class has blocks.
     const int pairs = (a.desc.tile_w + 1) / 2;
     const int rows_per_wg = pairs >= 256 ? 1 : 256 / pairs;
-    const int32_t blocks[SH_COUNT] = {a.desc.tile_h, 1, (a.desc.tile_h + rows_per_wg - 1) / rows_per_wg};
+    int32_t blocks[SH_COUNT] = {a.desc.tile_h, 1, (a.desc.tile_h + rows_per_wg - 1) / rows_per_wg};
+    const int cls = row_ext_log2 - kRowExtLog2Min;
+    for (int c = 0; c < kWidthClasses; ++c) blocks[SH_CLASS + c] = c == cls ? a.desc.tile_h : 0;
     for (int k = 0; k < SH_COUNT; ++k) {
         a.prefix[2 * k] = 0;
         a.prefix[2 * k + 1] = blocks[k];
@@ -160,8 +163,8 @@ int detect_noalloc(const Params& p, int width, int height, const uint8_t* gray, 
     plan.work = a.work;
     plan.nframes = 1;
     plan.work_cap = maxc;
-    plan.band_runs = band_runs_for_width(width);
-    plan.row_ext_log2 = row_ext_log2;
+    plan.class_band_runs[cls] = band_runs_for_width(width);
+    plan.tune = default_tune();
     plan.tune.streams = 1;
     for (int k = 0; k < SH_COUNT; ++k) plan.blocks[k] = blocks[k];
 

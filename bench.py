@@ -136,6 +136,8 @@ def uneven_batch(bdcudatag, devmem, scenes, steps):
     batch = bdcudatag.Batch(0, all_sizes)
     batch.detect_device(ptrs)
     default_ms = batch.replay(steps, device=True) / steps
+    batch.detect([buf.download()[i] for buf in bufs for i in range(per)])  # stage frames for time_stages
+    stages = batch.time_stages(reps=3)
     tune = batch.autotune(reps=5)
     batch.prepare_device(ptrs)
     for _ in range(2):
@@ -150,6 +152,7 @@ def uneven_batch(bdcudatag, devmem, scenes, steps):
     return {"frames": len(all_sizes), "mpix_per_step": round(px / 1e6, 2),
             "mpix_s": round(px * steps / t / 1e6, 1), "ms_per_step": round(t / steps * 1e3, 3),
             "device_ms_per_step": round(dev_ms, 3), "device_ms_default_launch": round(default_ms, 3),
+            "stages_ms": {k: round(v, 4) for k, v in stages.items()},
             "tune": {k: tune[k] for k in bdcudatag.TUNE_KEYS}}
 
 
