@@ -133,6 +133,8 @@ SIGNATURES = {
     "bd_batch_autotune": (_I, [_P, _I, ctypes.POINTER(Tune), ctypes.POINTER(ctypes.c_float)]),
     "bd_server_create": (_P, [_I, _I, ctypes.POINTER(ServerClass), ctypes.POINTER(Config),
                               ctypes.POINTER(_I)]),
+    "bd_server_create_resident": (_P, [_I, _I, ctypes.POINTER(ServerClass), ctypes.POINTER(Config),
+                                       ctypes.c_double, ctypes.POINTER(_I)]),
     "bd_server_destroy": (None, [_P]),
     "bd_server_submit": (ctypes.c_int64, [_P, _I, _P, _I]),
     "bd_server_poll": (_I, [_P, ctypes.POINTER(ServerResult), _I, ctypes.POINTER(Detection), _I]),
@@ -427,9 +429,11 @@ class Server:
     """Continuous-batching server over device-resident frames.
 
     classes: list of dicts {width, height, lanes, lane_frames, queue_depth}.
+    resident=True: admission on the device (bd_server_create_resident), each
+    lane launch waiting up to admit_wait_us for requests.
     """
 
-    def __init__(self, family: int, classes, **config):
+    def __init__(self, family: int, classes, resident: bool = False, admit_wait_us: float = 500.0, **config):
         arr = (ServerClass * len(classes))()
         for i, c in enumerate(classes):
             arr[i] = ServerClass(c["width"], c["height"], c["lanes"], c["lane_frames"],
@@ -438,8 +442,13 @@ class Server:
         status = ctypes.c_int32(0)
         cfg = default_config(**config)
         self.max_detections = max(1, min(cfg.max_detections, 256))
-        self.handle = lib().bd_server_create(family, len(classes), arr, ctypes.byref(cfg),
-                                             ctypes.byref(status))
+        if resident:
+            self.handle = lib().bd_server_create_resident(family, len(classes), arr, ctypes.byref(cfg),
+                                                          admit_wait_us, ctypes.byref(status))
+        else:
+            self.handle = lib().bd_server_create(family, len(classes), arr, ctypes.byref(cfg),
+                                                 ctypes.byref(status))
+        self.resident = resident
         if not self.handle:
             msg = lib().bd_server_last_error()
             raise RuntimeError(f"bd_server_create failed ({status.value}): {msg.decode() if msg else ''}")

@@ -153,6 +153,8 @@ void Engine::release() {
     if (stream2_) (void)hipStreamSynchronize(stream2_);
     if (graph_host_) (void)hipGraphExecDestroy(graph_host_);
     if (graph_dev_) (void)hipGraphExecDestroy(graph_dev_);
+    if (graph_resident_) (void)hipGraphExecDestroy(graph_resident_);
+    graph_resident_ = nullptr;
     for (int l = 0; l < 2; ++l) {
         if (lane_graph_host_[l]) (void)hipGraphExecDestroy(lane_graph_host_[l]);
         if (lane_graph_dev_[l]) (void)hipGraphExecDestroy(lane_graph_dev_[l]);
@@ -694,6 +696,53 @@ int Engine::collect_async(int index, bd_detection* out, int capacity) const {
                     static_cast<size_t>(written) * sizeof(bd_detection));
     }
     return found;
+}
+
+hipError_t Engine::enqueue_resident() {
+    hipError_t st = launch_admit(plan_dev_, resident_, stream_);
+    if (st == hipSuccess) st = launch_pipeline(plan_dev_, stream_);
+    if (st == hipSuccess) st = launch_publish(plan_dev_, resident_, stream_);
+    return st;
+}
+
+bool Engine::prepare_resident(const ResidentArgs& args, std::string* error) {
+    const std::lock_guard<std::mutex> lock(mutex_);
+    resident_ = args;
+    resident_ready_ = true;
+    if (graph_resident_) (void)hipGraphExecDestroy(graph_resident_);
+    graph_resident_ = nullptr;
+    if (!p_.use_graph) return true;
+    hipError_t st = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
+    if (st != hipSuccess) {
+        if (error) *error = hip_error("hipStreamBeginCapture", st);
+        return false;
+    }
+    const hipError_t queued = enqueue_resident();
+    hipGraph_t graph = nullptr;
+    st = hipStreamEndCapture(stream_, &graph);
+    if (queued == hipSuccess && st == hipSuccess && graph) st = hipGraphInstantiate(&graph_resident_, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (queued != hipSuccess || st != hipSuccess) {
+        graph_resident_ = nullptr;
+        if (error) *error = hip_error("resident graph capture", queued != hipSuccess ? queued : st);
+        return false;
+    }
+    return true;
+}
+
+bool Engine::launch_resident(std::string* error) {
+    const std::lock_guard<std::mutex> lock(mutex_);
+    if (!resident_ready_) {
+        if (error) *error = "resident launch not prepared";
+        return false;
+    }
+    hipError_t st = graph_resident_ ? hipGraphLaunch(graph_resident_, stream_) : enqueue_resident();
+    if (st == hipSuccess) st = hipEventRecord(ev_stop_, stream_);
+    if (st != hipSuccess) {
+        if (error) *error = hip_error("resident launch", st);
+        return false;
+    }
+    return true;
 }
 
 bool Engine::time_stages(int reps, float* ms, std::string* error) {

@@ -117,6 +117,15 @@ public:
     // Detections of position `index` of the finished async launch.
     int collect_async(int index, bd_detection* out, int capacity) const;
 
+    // Device-resident serving: every launch admits requests from the ring of
+    // `args` on the device (k_admit), runs the pipeline over the admitted
+    // positions and publishes their results (k_publish), captured once as a
+    // graph on the engine's stream.  The engine is the serving lane's alone
+    // from then on: its device-path descriptors belong to the admissions.
+    bool prepare_resident(const ResidentArgs& args, std::string* error);
+    // Queues one resident launch; query_async() reports its completion.
+    bool launch_resident(std::string* error);
+
 private:
     Engine() = default;
     bool init(const Params& params, const std::vector<int>& widths,
@@ -182,6 +191,10 @@ private:
     hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr;
     hipEvent_t stage_events_[kStageCount + 1] = {};
     hipGraphExec_t graph_host_ = nullptr, graph_dev_ = nullptr;
+    ResidentArgs resident_{};
+    bool resident_ready_ = false;
+    hipGraphExec_t graph_resident_ = nullptr;
+    hipError_t enqueue_resident();
 
     std::vector<FrameStats> stats_;
     int async_frames_ = 0;

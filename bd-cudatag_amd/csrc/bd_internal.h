@@ -171,6 +171,35 @@ struct Plan {
     int32_t blocks[SH_COUNT];
 };
 
+// Device-resident serving (bd_server_create_resident): requests of one frame
+// size class arrive in a ring in pinned host memory; a lane's launch admits
+// them on the device (k_admit) and publishes their results (k_publish).
+// A request slot is published by writing seq = ticket + 1 after gray and
+// stride; a completion slot by writing seq = ticket + 1 after count.
+struct RequestSlot {
+    const uint8_t* gray;
+    int32_t stride;
+    uint32_t seq;
+};
+struct CompletionSlot {
+    int32_t count;  // detections (min(found, max_detections))
+    int32_t batch;  // requests admitted by the same launch
+    uint32_t seq;
+    uint32_t reserved;
+};
+struct ResidentArgs {
+    RequestSlot* slots;        // ring of `depth` request slots (pinned host memory)
+    CompletionSlot* done;      // ring of `depth` completion slots (pinned host memory)
+    bd_detection* out;         // depth x max_det records (pinned host memory)
+    const uint32_t* stop;      // pinned host memory: nonzero ends every wait
+    uint32_t* head;            // device memory: the class's next ticket to admit
+    uint32_t* lane_tickets;    // device memory: tickets of this lane's positions, count last
+    bd_detection* lane_out;    // device memory: max_det records per position (k_finalize's output)
+    int32_t depth, max_det;
+    uint64_t wait_ticks;       // device wall-clock ticks an admission waits for a request
+    uint32_t max_polls;        // and at most this many polls of the ring
+};
+
 // Kernels in pipeline order: tile_stats, frame_ref, binarize_runs, merge,
 // stats, collect, select, fit, decode, finalize.
 constexpr int kStageCount = 10;
@@ -183,5 +212,9 @@ Tune default_tune();
 hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream);
 // Enqueues the whole pipeline for every frame of the plan on `stream`.
 hipError_t launch_pipeline(const Plan& plan, hipStream_t stream);
+// Device-resident serving: admission before the pipeline and publication
+// after it, for the plan's frame positions (one wave each).
+hipError_t launch_admit(const Plan& plan, const ResidentArgs& args, hipStream_t stream);
+hipError_t launch_publish(const Plan& plan, const ResidentArgs& args, hipStream_t stream);
 
 }  // namespace bd

@@ -2782,7 +2782,107 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
     if (t == 0) d->scal[S_COUNTS + 1] = static_cast<uint32_t>(kept);
 }
 
+// ---------------------------------------------------------------------------
+// device-resident serving
+// ---------------------------------------------------------------------------
+
+// Admission of one lane's launch (one wave): waits, bounded by a wall-clock
+// budget, a poll count and the host's stop flag, for the class's next
+// request to be published; claims the run of published requests from the
+// class head, up to the lane's positions, with one compare-and-swap on the
+// head (lanes of a class admit concurrently); and points the claimed
+// positions' descriptors at their frames.  Unclaimed positions are inactive
+// for this launch.  Every path through the loop ends: a claim, the stop
+// flag, the budget or the poll cap.
+__global__ __launch_bounds__(64) void k_admit(Plan plan, ResidentArgs a) {
+    const int lane = threadIdx.x;
+    const int cap = plan.nframes;
+    uint32_t first = 0, count = 0;
+    if (lane == 0) {
+        const uint64_t start = wall_clock64();
+        for (uint32_t polls = 0;; ++polls) {
+            const uint32_t h = __hip_atomic_load(a.head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t n = 0;
+            while (static_cast<int>(n) < cap &&
+                   __hip_atomic_load(&a.slots[(h + n) % static_cast<uint32_t>(a.depth)].seq, __ATOMIC_ACQUIRE,
+                                     __HIP_MEMORY_SCOPE_SYSTEM) == h + n + 1u)
+                ++n;
+            if (n > 0) {
+                uint32_t expected = h;
+                if (__hip_atomic_compare_exchange_strong(a.head, &expected, h + n, __ATOMIC_ACQ_REL,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    first = h;
+                    count = n;
+                    break;
+                }
+                // Another lane took them: look again from the new head, after
+                // the same checks as an empty poll.
+            }
+            if (__hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u ||
+                wall_clock64() - start > a.wait_ticks || polls >= a.max_polls)
+                break;
+            if (n == 0) __builtin_amdgcn_s_sleep(8);
+        }
+    }
+    first = static_cast<uint32_t>(__shfl(static_cast<int>(first), 0));
+    count = static_cast<uint32_t>(__shfl(static_cast<int>(count), 0));
+    for (int k = lane; k < cap; k += 64) {
+        FrameDesc* d = const_cast<FrameDesc*>(plan.frames) + k;
+        const bool on = static_cast<uint32_t>(k) < count;
+        if (on) {
+            const uint32_t t = first + static_cast<uint32_t>(k);
+            const RequestSlot* r = a.slots + t % static_cast<uint32_t>(a.depth);
+            d->gray = r->gray;
+            d->stride = r->stride;
+            d->out = a.lane_out + static_cast<size_t>(k) * a.max_det;
+            a.lane_tickets[k] = t;
+        }
+        d->active = on ? 1 : 0;
+    }
+    if (lane == 0) a.lane_tickets[cap] = count;
+}
+
+// Publication of one lane's launch (one wave): each admitted request's
+// records (from the lane's device output) and count into its ticket's
+// slots in pinned host memory, a system-scope release fence and a barrier,
+// then each request's completion flag.
+__global__ __launch_bounds__(64) void k_publish(Plan plan, ResidentArgs a) {
+    const uint32_t count = a.lane_tickets[plan.nframes];
+    const uint32_t words = sizeof(bd_detection) / sizeof(uint32_t);
+    static_assert(sizeof(bd_detection) % sizeof(uint32_t) == 0, "records copy as words");
+    for (uint32_t k = 0; k < count; ++k) {
+        const FrameDesc* d = plan.frames + k;
+        const uint32_t t = a.lane_tickets[k];
+        const uint32_t slot = t % static_cast<uint32_t>(a.depth);
+        const int n = min(static_cast<int>(d->scal[S_COUNTS + 1]), d->max_detections);
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(d->out);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(a.out + static_cast<size_t>(slot) * a.max_det);
+        for (uint32_t w = threadIdx.x; w < static_cast<uint32_t>(n) * words; w += 64) dst[w] = src[w];
+        if (threadIdx.x == 0) {
+            a.done[slot].count = n;
+            a.done[slot].batch = static_cast<int32_t>(count);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < count; k += 64) {
+        const uint32_t t = a.lane_tickets[k];
+        __hip_atomic_store(&a.done[t % static_cast<uint32_t>(a.depth)].seq, t + 1u, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_admit(const Plan& plan, const ResidentArgs& args, hipStream_t stream) {
+    hipLaunchKernelGGL(k_admit, dim3(1), dim3(64), 0, stream, plan, args);
+    return hipGetLastError();
+}
+
+hipError_t launch_publish(const Plan& plan, const ResidentArgs& args, hipStream_t stream) {
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, stream, plan, args);
+    return hipGetLastError();
+}
 
 Tune default_tune() {
     // The persistent grids default to whole rounds of resident workgroups:

@@ -15,7 +15,9 @@ returns every frame's detections to the host.
 Besides the headline line it reports the same batch from host memory
 (H2D copy included), an uneven batch (16 frames each of 640x480, 1280x800,
 1920x1200 and 4096x3072 in one launch), a 60 s serving run (six Poisson
-camera streams, continuous batching, p50/p99/p99.9 latency), the corner
+camera streams, admission on the device, continuous batching,
+p50/p99/p99.9 latency; then 10 s at three times the load and 10 s with the
+host scheduler instead), the corner
 accuracy on 256 degraded frames against the rendered ground truth and the
 FP64 oracle, and the reference's own benchmark protocol (its 32 scenes at
 640x480, 1280x720, 1280x800 and 1920x1080 through bd_detector_detect_host).
@@ -252,11 +254,18 @@ def reference_protocol(bdcudatag, scenes, frames):
     return out
 
 
-def serving(bdcudatag, devmem, scenes, seconds):
+def serving(bdcudatag, devmem, scenes, seconds, resident=True, load=1.0):
     """Six simulated cameras with Poisson arrivals (4 x 1280x800 at 120 fps,
-    2 x 1600x1200 at 60 fps), frames already in HBM, continuous batching."""
-    classes = [{"width": 1280, "height": 800, "lanes": 4, "lane_frames": 4, "queue_depth": 64},
-               {"width": 1600, "height": 1200, "lanes": 2, "lane_frames": 2, "queue_depth": 32}]
+    2 x 1600x1200 at 60 fps, times `load`), frames already in HBM,
+    continuous batching.  resident: admission on the device (each lane's
+    launch claims published requests itself); else the host scheduler."""
+    if resident:
+        # Two lanes per class: four lanes, one per hardware queue.
+        classes = [{"width": 1280, "height": 800, "lanes": 2, "lane_frames": 4, "queue_depth": 256},
+                   {"width": 1600, "height": 1200, "lanes": 2, "lane_frames": 2, "queue_depth": 128}]
+    else:
+        classes = [{"width": 1280, "height": 800, "lanes": 4, "lane_frames": 4, "queue_depth": 64},
+                   {"width": 1600, "height": 1200, "lanes": 2, "lane_frames": 2, "queue_depth": 32}]
     threads = min(16, os.cpu_count() or 1)
     bufs, streams = [], []
     for ci, (c, n, fps) in enumerate(((classes[0], 4, 120.0), (classes[1], 2, 60.0))):
@@ -265,15 +274,18 @@ def serving(bdcudatag, devmem, scenes, seconds):
             buf = devmem.DeviceBuffer(imgs)
             bufs.append(buf)
             px = c["width"] * c["height"]
-            streams.append({"class_index": ci, "fps": fps, "stride": c["width"],
+            streams.append({"class_index": ci, "fps": fps * load, "stride": c["width"],
                             "frames": [buf.ptr + i * px for i in range(16)]})
-    srv = bdcudatag.Server(0, classes)
+    srv = bdcudatag.Server(0, classes, resident=resident)
     srv.simulate(streams, seconds=min(1.0, seconds), seed=99)  # warm-up
     rep = srv.simulate(streams, seconds=seconds, seed=1)
     srv.close()
     keys = ("offered", "completed", "dropped", "launches", "seconds", "fps", "mpix_s", "p50_ms",
             "p99_ms", "p999_ms", "max_ms", "mean_ms", "mean_batch")
-    return {k: (round(rep[k], 4) if isinstance(rep[k], float) else rep[k]) for k in keys}
+    out = {k: (round(rep[k], 4) if isinstance(rep[k], float) else rep[k]) for k in keys}
+    out["admission"] = "device (bd_server_create_resident)" if resident else "host scheduler thread"
+    out["offered_load"] = f"{load:g} x (4 x 1280x800 @120 fps + 2 x 1600x1200 @60 fps)"
+    return out
 
 
 def main():
@@ -446,6 +458,10 @@ def main():
         out["uneven_batch"] = uneven_batch(bdcudatag, devmem, scenes, args.uneven_steps)
     if args.serve_seconds > 0:
         out["serving"] = serving(bdcudatag, devmem, scenes, args.serve_seconds)
+        short = min(8.0, args.serve_seconds)
+        out["serving_load_3x"] = serving(bdcudatag, devmem, scenes, short, load=3.0)
+        out["serving_load_10x"] = serving(bdcudatag, devmem, scenes, short, load=10.0)
+        out["serving_host_scheduler"] = serving(bdcudatag, devmem, scenes, short, resident=False)
     if args.precision_frames > 0:
         out["precision"] = precision(bdcudatag, devmem, scenes, args.precision_frames)
     if args.protocol_frames > 0:

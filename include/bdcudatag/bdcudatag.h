@@ -322,6 +322,21 @@ typedef struct bd_serve_report {
 BD_API bd_server* bd_server_create(int32_t family, int32_t nclasses,
                                    const bd_server_class* classes, const bd_config* config,
                                    int32_t* status_out);
+/* The same server with admission on the device: each class's requests are
+ * published into a ring of queue_depth slots in pinned host memory (0: 256;
+ * beyond it they are dropped), and every lane launch starts with a kernel
+ * that waits on the device, at most admit_wait_us per launch or until the
+ * server stops, for published requests and claims up to lane_frames of
+ * them; the launch's last kernel publishes each request's records, count
+ * and completion flag into the ring.  The host only re-launches each lane's
+ * captured graph and timestamps completions.  At most 16 lanes in all (keep
+ * them within the device's hardware queues, 4 by default, so that no lane
+ * waits behind another's admission).  submit/poll/wait_idle/simulate and
+ * destroy work as for bd_server_create. */
+BD_API bd_server* bd_server_create_resident(int32_t family, int32_t nclasses,
+                                            const bd_server_class* classes,
+                                            const bd_config* config, double admit_wait_us,
+                                            int32_t* status_out);
 BD_API void bd_server_destroy(bd_server* server);
 /* Non-blocking admission of a device-resident frame; returns the request id
  * (>= 0) or BD_ERROR_OUT_OF_MEMORY when the class queue is full (dropped). */

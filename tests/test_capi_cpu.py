@@ -87,3 +87,14 @@ def test_null_handles_are_rejected():
     status = ctypes.c_int32(0)
     assert not L.bd_batch_create(0, 0, w, h, None, ctypes.byref(status))
     assert status.value == bdcudatag.BD_ERROR_INVALID_ARGUMENT
+
+
+def test_resident_server_rejects_invalid_arguments():
+    # Checked before any GPU work.
+    L = bdcudatag.lib()
+    cls = (bdcudatag.ServerClass * 1)(bdcudatag.ServerClass(1280, 800, 2, 4, 64, 0))
+    many = (bdcudatag.ServerClass * 1)(bdcudatag.ServerClass(1280, 800, 17, 4, 64, 0))
+    status = ctypes.c_int32(0)
+    for classes, n, wait in [(cls, 1, 0.0), (cls, 1, -5.0), (cls, 1, 2e6), (many, 1, 500.0), (cls, 0, 500.0)]:
+        assert not L.bd_server_create_resident(0, n, classes, None, wait, ctypes.byref(status))
+        assert status.value == bdcudatag.BD_ERROR_INVALID_ARGUMENT

@@ -1,4 +1,6 @@
-"""Serving: continuous batching over the preallocated arena.
+"""Serving: continuous batching over the preallocated arena, with the
+scheduling on the host (bd_server_create) or the admission on the device
+(bd_server_create_resident).
 
 Each request's result must be the same alone (single-frame detector, device
 pointer path) and in the mix (admitted while other requests are on the GPU,
@@ -32,7 +34,8 @@ def frames():
     return out
 
 
-def test_same_alone_and_in_the_mix(frames):
+@pytest.mark.parametrize("resident", [False, True])
+def test_same_alone_and_in_the_mix(frames, resident):
     alone = []
     for ci, c in enumerate(CLASSES):
         det = bdcudatag.Detector(0, c["width"], c["height"])
@@ -44,7 +47,7 @@ def test_same_alone_and_in_the_mix(frames):
             for a, b in zip(got, want):
                 assert np.array_equal(a["corners"], b["corners"])
                 assert np.array_equal(a["homography"], b["homography"])
-    srv = bdcudatag.Server(0, CLASSES)
+    srv = bdcudatag.Server(0, CLASSES, resident=resident)
     order = []
     for rep in range(3):
         for i in range(10):
@@ -76,8 +79,9 @@ def test_same_alone_and_in_the_mix(frames):
     srv.close()
 
 
-def test_simulated_streams_short(frames):
-    srv = bdcudatag.Server(0, CLASSES)
+@pytest.mark.parametrize("resident", [False, True])
+def test_simulated_streams_short(frames, resident):
+    srv = bdcudatag.Server(0, CLASSES, resident=resident)
     streams = [{"class_index": 0, "fps": 120.0, "frames": frames[0][2], "stride": 1280} for _ in range(4)]
     streams += [{"class_index": 1, "fps": 60.0, "frames": frames[1][2], "stride": 1600} for _ in range(2)]
     rep = srv.simulate(streams, seconds=2.0, seed=1)
@@ -85,3 +89,14 @@ def test_simulated_streams_short(frames):
     assert rep["offered"] > 800
     assert rep["p50_ms"] > 0 and rep["p99_ms"] >= rep["p50_ms"]
     srv.close()
+
+
+def test_resident_server_stops_when_idle():
+    # Lanes cycle through bounded admission waits while nothing arrives;
+    # destroying the server sets the stop flag, which ends every wait.
+    import time
+    srv = bdcudatag.Server(0, CLASSES, resident=True, admit_wait_us=2000.0)
+    time.sleep(0.05)
+    t0 = time.perf_counter()
+    srv.close()
+    assert time.perf_counter() - t0 < 1.0
