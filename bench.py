@@ -16,9 +16,8 @@ Besides the headline line it reports the same batch from host memory
 (H2D copy included), an uneven batch (16 frames each of 640x480, 1280x800,
 1920x1200 and 4096x3072 in one launch), a 60 s serving run (six Poisson
 camera streams, admission on the device, continuous batching,
-p50/p99/p99.9 latency; then 10 s at three times the load and 10 s with the
-host scheduler instead), the corner
-accuracy on 256 degraded frames against the rendered ground truth and the
+p50/p99/p99.9 latency; then 8 s each at three and ten times the load and
+with the host scheduler instead), the corner accuracy on 256 degraded frames against the rendered ground truth and the
 FP64 oracle, and the reference's own benchmark protocol (its 32 scenes at
 640x480, 1280x720, 1280x800 and 1920x1080 through bd_detector_detect_host).
 
