@@ -92,7 +92,7 @@ class Tune(ctypes.Structure):
     _fields_ = [("fit_grid", ctypes.c_int32), ("decode_grid", ctypes.c_int32),
                 ("head_ranks", ctypes.c_int32), ("streams", ctypes.c_int32),
                 ("decode_waves", ctypes.c_int32), ("chunk_px", ctypes.c_int32),
-                ("head_area", ctypes.c_int32)]
+                ("head_area", ctypes.c_int32), ("band_rows", ctypes.c_int32)]
 
 
 TUNE_KEYS = tuple(name for name, _ in Tune._fields_)
@@ -322,8 +322,8 @@ class Batch:
         return {k: getattr(t, k) for k in TUNE_KEYS}
 
     def set_tune(self, fit_grid: int, decode_grid: int, head_ranks: int, streams: int = 2,
-                 decode_waves: int = 8, chunk_px: int = 64, head_area: int = 1000) -> None:
-        t = Tune(fit_grid, decode_grid, head_ranks, streams, decode_waves, chunk_px, head_area)
+                 decode_waves: int = 8, chunk_px: int = 64, head_area: int = 1000, band_rows: int = 8) -> None:
+        t = Tune(fit_grid, decode_grid, head_ranks, streams, decode_waves, chunk_px, head_area, band_rows)
         rc = lib().bd_batch_set_tune(self.handle, ctypes.byref(t))
         if rc != BD_OK:
             raise (ValueError if rc == BD_ERROR_INVALID_ARGUMENT else RuntimeError)(
@@ -348,7 +348,8 @@ class Batch:
             if key in table:
                 t = table[key]
                 self.set_tune(t["fit_grid"], t["decode_grid"], t["head_ranks"], t.get("streams", 2),
-                              t.get("decode_waves", 8), t.get("chunk_px", 64), t.get("head_area", 1000))
+                              t.get("decode_waves", 8), t.get("chunk_px", 64), t.get("head_area", 1000),
+                              t.get("band_rows", 8))
                 return dict(t, cached=True)
         best = Tune()
         ms = ctypes.c_float()

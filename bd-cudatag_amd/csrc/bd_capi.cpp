@@ -387,7 +387,8 @@ int32_t bd_batch_get_tune(bd_batch* batch, bd_tune* out) {
         return BD_ERROR_INVALID_ARGUMENT;
     }
     const bd::Tune t = batch->engine->tune();
-    *out = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, t.streams, t.decode_waves, t.chunk_px, t.head_area};
+    *out = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, t.streams, t.decode_waves, t.chunk_px, t.head_area,
+                   t.band_rows};
     return BD_OK;
 }
 
@@ -404,6 +405,7 @@ int32_t bd_batch_set_tune(bd_batch* batch, const bd_tune* tune) {
     t.decode_waves = tune->decode_waves == 0 ? bd::Tune{}.decode_waves : tune->decode_waves;
     t.chunk_px = tune->chunk_px == 0 ? bd::Tune{}.chunk_px : tune->chunk_px;
     t.head_area = tune->head_area == 0 ? bd::Tune{}.head_area : tune->head_area;
+    t.band_rows = tune->band_rows == 0 ? bd::Tune{}.band_rows : tune->band_rows;
     std::string error;
     if (!batch->engine->set_tune(t, &error)) {
         set_error(error);
@@ -424,7 +426,9 @@ int32_t bd_batch_autotune(bd_batch* batch, int32_t reps, bd_tune* best, float* b
         set_error(error);
         return BD_ERROR_CUDA;
     }
-    if (best) *best = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, t.streams, t.decode_waves, t.chunk_px, t.head_area};
+    if (best)
+        *best = bd_tune{t.fit_grid, t.decode_grid, t.head_ranks, t.streams, t.decode_waves, t.chunk_px, t.head_area,
+                        t.band_rows};
     if (best_ms) *best_ms = ms;
     return BD_OK;
 }

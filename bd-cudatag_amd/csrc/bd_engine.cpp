@@ -301,9 +301,11 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         prefix[SH_TSTAT * (nf + 1) + f + 1] =
             prefix[SH_TSTAT * (nf + 1) + f] + (d.tile_h + rows_per_wg - 1) / rows_per_wg;
         for (int c = 0; c < kWidthClasses; ++c) {
-            const int32_t* pc = prefix.data() + static_cast<size_t>(SH_CLASS + c) * (nf + 1);
             const bool mine = d.row_ext_log2 == kRowExtLog2Min + c;
-            prefix[static_cast<size_t>(SH_CLASS + c) * (nf + 1) + f + 1] = pc[f] + (mine ? d.tile_h : 0);
+            int32_t* pc = prefix.data() + static_cast<size_t>(SH_CLASS + c) * (nf + 1);
+            int32_t* pp = prefix.data() + static_cast<size_t>(SH_CLASS_PAIR + c) * (nf + 1);
+            pc[f + 1] = pc[f] + (mine ? d.tile_h : 0);
+            pp[f + 1] = pp[f] + (mine ? (d.tile_h + 1) / 2 : 0);
         }
     }
     std::memcpy(h_desc_dev_, desc_.data(), desc_.size() * sizeof(FrameDesc));
@@ -828,7 +830,7 @@ bool Engine::set_tune_locked(const Tune& t, std::string* error) {
     if (t.fit_grid < 64 || t.fit_grid > 16384 || t.decode_grid < 64 || t.decode_grid > 16384 ||
         t.head_ranks < 0 || t.head_ranks > kHeadRanksMax || t.streams < 1 || t.streams > 2 ||
         (t.decode_waves != 4 && t.decode_waves != 8 && t.decode_waves != 16) ||
-        (t.chunk_px != 64 && t.chunk_px != 128) || t.head_area < 1) {
+        (t.chunk_px != 64 && t.chunk_px != 128) || t.head_area < 1 || (t.band_rows != 8 && t.band_rows != 16)) {
         if (error) *error = "launch parameters out of range";
         return false;
     }
@@ -892,6 +894,11 @@ bool Engine::autotune(int reps, Tune* best, float* best_ms, std::string* error) 
     {
         Tune t = cur;
         t.chunk_px = cur.chunk_px == 64 ? 128 : 64;
+        if (!consider(t)) return false;
+    }
+    {
+        Tune t = cur;
+        t.band_rows = cur.band_rows == 8 ? 16 : 8;
         if (!consider(t)) return false;
     }
     for (int32_t h = 0; h <= kHeadRanksMax; ++h) {

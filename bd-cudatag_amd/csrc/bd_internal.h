@@ -128,9 +128,17 @@ struct FrameDesc {
 // Prefix tables over frames: prefix[k][f] = first block of frame f for
 // kernel shape k; prefix[k][nframes] = total blocks.
 // SH_CLASS + c: tile rows of the frames of width class c (row-table log2
-// kRowExtLog2Min + c), zero blocks for every other frame.
+// kRowExtLog2Min + c), zero blocks for every other frame; SH_CLASS_PAIR + c:
+// pairs of tile rows (16-row execution tiling of k_binarize_runs).
 constexpr int kWidthClasses = kRowExtLog2Max - kRowExtLog2Min + 1;
-enum Shape : int { SH_TILE_ROW = 0, SH_FRAME = 1, SH_TSTAT = 2, SH_CLASS = 3, SH_COUNT = SH_CLASS + kWidthClasses };
+enum Shape : int {
+    SH_TILE_ROW = 0,
+    SH_FRAME = 1,
+    SH_TSTAT = 2,
+    SH_CLASS = 3,
+    SH_CLASS_PAIR = SH_CLASS + kWidthClasses,
+    SH_COUNT = SH_CLASS_PAIR + kWidthClasses
+};
 
 // Batch candidate list: a small header, then one entry per selected
 // candidate of any frame (frame << 10 | candidate index), appended by
@@ -146,8 +154,10 @@ constexpr int kHeadSlots = 32;
 // Launch parameters that change speed only, never results (autotuned per
 // batch shape): persistent grid sizes of k_fit and k_decode, the number of
 // each frame's largest candidates scheduled first and the area above which
-// a candidate is scheduled first whatever its rank, and whether a batch of
-// two or more frames runs as two concurrent lanes (streams = 2) or as one.
+// a candidate is scheduled first whatever its rank, whether a batch of two
+// or more frames runs as two concurrent lanes (streams = 2) or as one, the
+// decode waves per workgroup, the pixels per binarization item and the
+// pixel rows per binarization workgroup.
 struct Tune {
     int32_t fit_grid = 1280;     // 5 resident 256-thread workgroups per CU
     int32_t decode_grid = 1024;  // 4 resident 512-thread workgroups per CU
@@ -156,6 +166,7 @@ struct Tune {
     int32_t decode_waves = 8;    // 4, 8 or 16 waves per decode workgroup
     int32_t chunk_px = 64;       // pixels packed per k_binarize_runs item: 64 or 128
     int32_t head_area = 1000;    // pixels; larger candidates join the head region
+    int32_t band_rows = 8;       // pixel rows per k_binarize_runs workgroup: 8 or 16
 };
 
 struct Plan {

@@ -544,11 +544,9 @@ __device__ inline uint32_t bytes_above(uint32_t x, uint32_t t) {
 //     word and only they get component statistics initialised.  A band with
 //     more runs than the LDS buffer leaves every run a root and lets k_merge
 //     join all of its rows.
+// Band ty of frame d (every branch below is uniform across the workgroup).
 template <int kChunkWords>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_binarize_runs(Plan plan, int cls) {
-    int ty;
-    const FrameDesc* d = plan.frames + frame_of(plan, SH_CLASS + cls, blockIdx.x, &ty);
-    if (!d->active) return;
+__device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, int ty) {
     const int W = d->width, H = d->height, tw = d->tile_w, th = d->tile_h;
     const int w64 = d->words64, row_cap = d->row_cap, stride = d->stride;
     const int y0 = ty * kTile;
@@ -903,6 +901,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             uint4* bb = reinterpret_cast<uint4*>(d->bbox + 4 * static_cast<size_t>(slot));
             *bb = make_uint4(0xffffffffu, 0xffffffffu, 0u, 0u);
         }
+    }
+}
+
+// One workgroup per kBands consecutive bands of one frame (the band-rows
+// launch parameter: 8 or 16 pixel rows per workgroup, an execution tiling
+// only: the 8x8 threshold tiles and the 8-row union bands are the same),
+// the bands in turn, reusing the LDS.
+template <int kChunkWords, int kBands>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_binarize_runs(Plan plan, int cls) {
+    int unit;
+    const FrameDesc* d =
+        plan.frames + frame_of(plan, (kBands == 1 ? SH_CLASS : SH_CLASS_PAIR) + cls, blockIdx.x, &unit);
+    if (!d->active) return;
+#pragma unroll 1
+    for (int b = 0; b < kBands; ++b) {
+        const int ty = unit * kBands + b;
+        if (ty >= d->tile_h) break;
+        if (b > 0) __syncthreads();  // the previous band's LDS reads are done
+        binarize_band<kChunkWords>(plan, cls, d, ty);
     }
 }
 
@@ -2911,13 +2928,19 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
         case 2:
             // One launch per width class present, with the class's band capacity.
             for (int c = 0; c < kWidthClasses; ++c) {
-                const int blocks = plan.blocks[SH_CLASS + c];
+                const bool pairs = plan.tune.band_rows == 16;
+                const int blocks = plan.blocks[(pairs ? SH_CLASS_PAIR : SH_CLASS) + c];
                 if (blocks == 0) continue;
                 const size_t lds = static_cast<size_t>(plan.class_band_runs[c]) * 8;
-                if (plan.tune.chunk_px == 128)
-                    hipLaunchKernelGGL(k_binarize_runs<2>, dim3(blocks), dim3(256), lds, stream, plan, c);
+                const bool wide = plan.tune.chunk_px == 128;
+                if (wide && pairs)
+                    hipLaunchKernelGGL((k_binarize_runs<2, 2>), dim3(blocks), dim3(256), lds, stream, plan, c);
+                else if (wide)
+                    hipLaunchKernelGGL((k_binarize_runs<2, 1>), dim3(blocks), dim3(256), lds, stream, plan, c);
+                else if (pairs)
+                    hipLaunchKernelGGL((k_binarize_runs<1, 2>), dim3(blocks), dim3(256), lds, stream, plan, c);
                 else
-                    hipLaunchKernelGGL(k_binarize_runs<1>, dim3(blocks), dim3(256), lds, stream, plan, c);
+                    hipLaunchKernelGGL((k_binarize_runs<1, 1>), dim3(blocks), dim3(256), lds, stream, plan, c);
             }
             break;
         case 3: hipLaunchKernelGGL(k_merge, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;

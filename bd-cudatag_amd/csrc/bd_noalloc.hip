@@ -151,7 +151,10 @@ int detect_noalloc(const Params& p, int width, int height, const uint8_t* gray, 
     const int rows_per_wg = pairs >= 256 ? 1 : 256 / pairs;
     int32_t blocks[SH_COUNT] = {a.desc.tile_h, 1, (a.desc.tile_h + rows_per_wg - 1) / rows_per_wg};
     const int cls = row_ext_log2 - kRowExtLog2Min;
-    for (int c = 0; c < kWidthClasses; ++c) blocks[SH_CLASS + c] = c == cls ? a.desc.tile_h : 0;
+    for (int c = 0; c < kWidthClasses; ++c) {
+        blocks[SH_CLASS + c] = c == cls ? a.desc.tile_h : 0;
+        blocks[SH_CLASS_PAIR + c] = c == cls ? (a.desc.tile_h + 1) / 2 : 0;
+    }
     for (int k = 0; k < SH_COUNT; ++k) {
         a.prefix[2 * k] = 0;
         a.prefix[2 * k + 1] = blocks[k];

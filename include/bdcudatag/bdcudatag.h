@@ -238,8 +238,9 @@ BD_API int32_t bd_batch_replay(bd_batch* batch, int32_t reps, int32_t device_pat
  * change speed only: every setting gives bit-identical detections.
  */
 typedef struct bd_tune {
-    int32_t fit_grid;    /* persistent quad-fit workgroups, 64..16384 (default 1280) */
-    int32_t decode_grid; /* persistent decode workgroups, 64..16384 (default 1024) */
+    int32_t fit_grid;    /* persistent quad-fit workgroups, 64..16384 (default five per CU) */
+    int32_t decode_grid; /* persistent decode workgroups, 64..16384 (default: as many as the
+                            device's CUs hold at once) */
     int32_t head_ranks;  /* each frame's largest candidates scheduled first, 0..4 (default 2) */
     int32_t streams;     /* 2: a batch of two or more frames runs as two concurrent halves of
                             about equal pixel count, on two streams; 1: as one launch sequence
@@ -253,6 +254,9 @@ typedef struct bd_tune {
     int32_t head_area;    /* candidates of at least this many pixels are also scheduled first
                              (at most 32 per frame), >= 1; INT32_MAX turns this off (default
                              1000; 0 in bd_batch_set_tune selects the default) */
+    int32_t band_rows;    /* pixel rows per binarization workgroup: 8 (one band) or 16 (two bands
+                             in turn), an execution tiling only (default 8; 0 in bd_batch_set_tune
+                             selects the default) */
 } bd_tune;
 
 BD_API int32_t bd_batch_get_tune(bd_batch* batch, bd_tune* out);

@@ -27,21 +27,21 @@ def test_launch_parameters_keep_results(tmp_path):
     # Default persistent grids are whole rounds of resident workgroups.
     t0 = batch.get_tune()
     assert t0["fit_grid"] > 0 and t0["decode_grid"] > 0
-    assert {k: t0[k] for k in ("head_ranks", "streams", "decode_waves", "chunk_px", "head_area")} == \
-        {"head_ranks": 2, "streams": 2, "decode_waves": 8, "chunk_px": 64, "head_area": 1000}
+    assert {k: t0[k] for k in ("head_ranks", "streams", "decode_waves", "chunk_px", "head_area", "band_rows")} == \
+        {"head_ranks": 2, "streams": 2, "decode_waves": 8, "chunk_px": 64, "head_area": 1000, "band_rows": 8}
     ref = batch.detect(imgs)
     assert sum(len(r) for r in ref) > 0
     # head_area 1 puts every candidate (up to 32 per frame) in the head
     # region, 2**31 - 1 none beyond the head ranks.
-    for fg, dg, hr, ns, dw, cp, ha in [(64, 64, 0, 1, 4, 128, 1), (256, 2048, 4, 2, 16, 64, 2**31 - 1),
-                                       (4096, 512, 1, 1, 8, 128, 500), (1280, 1024, 2, 2, 4, 64, 1),
-                                       (1280, 100, 2, 1, 16, 128, 8000)]:
-        batch.set_tune(fg, dg, hr, ns, dw, cp, ha)
+    for fg, dg, hr, ns, dw, cp, ha, br in [(64, 64, 0, 1, 4, 128, 1, 16), (256, 2048, 4, 2, 16, 64, 2**31 - 1, 8),
+                                           (4096, 512, 1, 1, 8, 128, 500, 16), (1280, 1024, 2, 2, 4, 64, 1, 16),
+                                           (1280, 100, 2, 1, 16, 128, 8000, 8)]:
+        batch.set_tune(fg, dg, hr, ns, dw, cp, ha, br)
         assert batch.get_tune() == {"fit_grid": fg, "decode_grid": dg, "head_ranks": hr, "streams": ns,
-                                    "decode_waves": dw, "chunk_px": cp, "head_area": ha}
+                                    "decode_waves": dw, "chunk_px": cp, "head_area": ha, "band_rows": br}
         same(batch.detect(imgs), ref)
     for bad in [(10, 1024, 2), (1024, 1 << 20, 2), (1024, 1024, 5), (1024, 1024, 2, 3), (1024, 1024, 2, 2, 12),
-                (1024, 1024, 2, 2, 8, 96), (1024, 1024, 2, 2, 8, 64, -5)]:
+                (1024, 1024, 2, 2, 8, 96), (1024, 1024, 2, 2, 8, 64, -5), (1024, 1024, 2, 2, 8, 64, 1000, 12)]:
         with pytest.raises(ValueError):
             batch.set_tune(*bad)
     cache = str(tmp_path / "tune.json")
