@@ -210,10 +210,19 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
     const int ty0 = blk * R;
     __shared__ uint32_t hs[256 + 64];
     __shared__ int lo_s[256], hi_s[256];
+    // The workgroup's tiles (rows ty0 .. ty0 + R - 1, contiguous in the tile
+    // arrays) are staged here and written with coalesced word stores.
+    constexpr int kStage = 1024;
+    __shared__ __align__(4) uint8_t s_min[kStage], s_max[kStage], s_mean[kStage];
+    __shared__ int s_top, s_bottom;
     for (int i = t; i < 320; i += blockDim.x) hs[i] = 0;
     if (t < R) {
         lo_s[t] = tw;
         hi_s[t] = -1;
+    }
+    if (t == 0) {
+        s_top = INT32_MAX;
+        s_bottom = -1;
     }
     __syncthreads();
 
@@ -273,10 +282,10 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
             const int ta = 2 * p;
             const uint32_t cA = static_cast<uint32_t>(nA * rows);
             const uint32_t meanA = (sumA + cA / 2) / cA;
-            const size_t ia = static_cast<size_t>(ty) * tw + ta;
-            d->tile_min[ia] = static_cast<uint8_t>(mnA);
-            d->tile_max[ia] = static_cast<uint8_t>(mxA);
-            d->tile_mean[ia] = static_cast<uint8_t>(meanA);
+            const int ia = sub * tw + ta;  // staging index (R * tw <= kStage)
+            s_min[ia] = static_cast<uint8_t>(mnA);
+            s_max[ia] = static_cast<uint8_t>(mxA);
+            s_mean[ia] = static_cast<uint8_t>(meanA);
             atomicAdd(&hs[(mxA - mnA) & 0xffu], 1u);
             atomicAdd(&hs[256 + (meanA >> 2)], 1u);
             int lo = tw, hi = -1;
@@ -284,9 +293,9 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
             if (nB > 0) {
                 const uint32_t cB = static_cast<uint32_t>(nB * rows);
                 const uint32_t meanB = (sumB + cB / 2) / cB;
-                d->tile_min[ia + 1] = static_cast<uint8_t>(mnB);
-                d->tile_max[ia + 1] = static_cast<uint8_t>(mxB);
-                d->tile_mean[ia + 1] = static_cast<uint8_t>(meanB);
+                s_min[ia + 1] = static_cast<uint8_t>(mnB);
+                s_max[ia + 1] = static_cast<uint8_t>(mxB);
+                s_mean[ia + 1] = static_cast<uint8_t>(meanB);
                 atomicAdd(&hs[(mxB - mnB) & 0xffu], 1u);
                 atomicAdd(&hs[256 + (meanB >> 2)], 1u);
                 if (static_cast<int>(mxB - mnB) >= kMinTileContrast) {
@@ -309,9 +318,38 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
         d->trow_lo[row] = lo_s[t];
         d->trow_hi[row] = hi_s[t];
         if (hi_s[t] >= 0) {
-            atomicMin(&d->scal[S_TOP], static_cast<uint32_t>(row));
-            atomicMax(&d->scal[S_BOTTOM1], static_cast<uint32_t>(row + 1));
+            atomicMin(&s_top, row);
+            atomicMax(&s_bottom, row);
         }
+    }
+    // The staged tiles: bytes [first, first + count) of each tile array, as
+    // aligned words with the unaligned ends byte by byte.
+    const int rows_here = min(R, th - ty0);
+    const int first = ty0 * tw, count = rows_here * tw;
+    const int a0 = min((first + 3) & ~3, first + count), a1 = max(a0, (first + count) & ~3);
+    for (int i = t; i < (a1 - a0) / 4; i += blockDim.x) {
+        const int o = a0 - first + 4 * i;
+        const auto word = [&](const uint8_t* src) {
+            return static_cast<uint32_t>(src[o]) | (static_cast<uint32_t>(src[o + 1]) << 8) |
+                   (static_cast<uint32_t>(src[o + 2]) << 16) | (static_cast<uint32_t>(src[o + 3]) << 24);
+        };
+        reinterpret_cast<uint32_t*>(d->tile_min + a0)[i] = word(s_min);
+        reinterpret_cast<uint32_t*>(d->tile_max + a0)[i] = word(s_max);
+        reinterpret_cast<uint32_t*>(d->tile_mean + a0)[i] = word(s_mean);
+    }
+    const int head = a0 - first, tail = first + count - a1;
+    if (t < head + tail) {
+        const int o = t < head ? t : a1 - first + (t - head);
+        d->tile_min[first + o] = s_min[o];
+        d->tile_max[first + o] = s_max[o];
+        d->tile_mean[first + o] = s_mean[o];
+    }
+    __syncthreads();
+    // One pair of global atomics per workgroup for the frame's first and
+    // last active tile rows.
+    if (t == 0 && s_bottom >= 0) {
+        atomicMin(&d->scal[S_TOP], static_cast<uint32_t>(s_top));
+        atomicMax(&d->scal[S_BOTTOM1], static_cast<uint32_t>(s_bottom + 1));
     }
 }
 
