@@ -64,6 +64,7 @@ inline int band_runs_for_width(int width) {
 // present, each launch over that class's frames only, with the class's own
 // LDS band capacity (from its widest frame).
 constexpr int kRowExtLog2Min = 6, kRowExtLog2Max = 8;
+constexpr uint32_t kRowTableFull = 0xffffffffu;
 inline int row_ext_log2_for_width(int width) {
     int lg = kRowExtLog2Min;
     while (lg < kRowExtLog2Max && (20 << lg) < width) ++lg;
@@ -114,10 +115,11 @@ struct FrameDesc {
     uint64_t* cand_keys;  // height * row_cap
     uint32_t* roots;      // height * row_cap: band b's component roots from b * 8 * row_cap
     uint32_t* band_nroots;  // tile_h: roots listed per band
-    uint64_t* row_ext;    // height << row_ext_log2: per row, a hash table of the
-                          // components in the row: root << 32 | x0 << 16 | x1
-                          // (leftmost start, rightmost end), ~0 when empty
-    uint32_t* row_ext_full;  // height: 1 when the row's table overflowed
+    uint64_t* row_ext;    // height << row_ext_log2: per row, the row's components
+                          // root << 32 | x0 << 16 | x1 (leftmost start, rightmost
+                          // end), packed from the row's first slot
+    uint32_t* row_ext_full;  // height: the row's entries, or kRowTableFull when
+                             // its table overflowed (the row is scanned instead)
     uint32_t* candidates; // max_candidates
     float* quads;         // max_candidates * 8
     float* hull_quads;    // max_candidates * 8

@@ -1517,14 +1517,33 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     }
     __syncthreads();
     if (t == 0) d->band_nroots[ty] = s_nroots;
-    for (int e = t; e < nrows * kRowSlots; e += blockDim.x) {
-        const uint32_t key = (&s_rkey[0][0])[e];
-        const uint32_t rx0 = (&s_rx0[0][0])[e], rx1 = (&s_rx1[0][0])[e];
-        d->row_ext[static_cast<size_t>(y0) * kRowSlots + e] =
-            key == kEmpty ? ~0ull : (static_cast<uint64_t>(key) << 32) | (rx0 << 16) | rx1;
-        if (key != kEmpty) fold_extent(key, rx0, rx1, static_cast<uint32_t>(y0 + e / kRowSlots));
+    // Each row's occupied entries, packed to the front of the row's slots in
+    // slot order (a row holds 9 components on average: the empty slots are
+    // not written); the row's count, or kRowTableFull when its table
+    // overflowed, goes to row_ext_full.  One wave per row.
+    {
+        const int wave = t / kWave;
+        const int nwaves = static_cast<int>(blockDim.x) / kWave;
+        for (int r = wave; r < nrows; r += nwaves) {
+            uint64_t* row_out = d->row_ext + (static_cast<size_t>(y0 + r) << kRowLog2);
+            uint32_t placed = 0;
+#pragma unroll
+            for (int e0 = 0; e0 < kRowSlots; e0 += kWave) {
+                const int e = e0 + lane;
+                const uint32_t key = s_rkey[r][e];
+                const bool occupied = key != kEmpty;
+                const uint64_t m = __ballot(occupied);
+                if (occupied) {
+                    const uint32_t rx0 = s_rx0[r][e], rx1 = s_rx1[r][e];
+                    const uint32_t pos = placed + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+                    row_out[pos] = (static_cast<uint64_t>(key) << 32) | (rx0 << 16) | rx1;
+                    fold_extent(key, rx0, rx1, static_cast<uint32_t>(y0 + r));
+                }
+                placed += static_cast<uint32_t>(__popcll(m));
+            }
+            if (lane == 0) d->row_ext_full[y0 + r] = s_rfull[r] ? kRowTableFull : placed;
+        }
     }
-    if (t < nrows) d->row_ext_full[y0 + t] = s_rfull[t];
     __syncthreads();
     // Occupied table entries are compacted first so every thread flushes
     // about the same number of roots.
@@ -1952,18 +1971,25 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
     for (int s = t; s < kept; s += blockDim.x) {
         const int y = fit_row(s, kept, rows, by0);
         uint32_t lo = 0xffffffffu, hi = 0u;  // min x0, max x1 + 1 (0: none)
-        if (!d->row_ext_full[y]) {
-            const int rlg = d->row_ext_log2, rs = 1 << rlg;
-            const uint64_t* tab = d->row_ext + (static_cast<size_t>(y) << rlg);
-            const uint32_t h = (root * 2654435761u) >> (32 - rlg);
-            for (int probe = 0; probe < rs; ++probe) {
-                const uint64_t e = tab[(h + static_cast<uint32_t>(probe)) & (rs - 1)];
-                if (static_cast<uint32_t>(e >> 32) == root) {
-                    lo = static_cast<uint32_t>(e >> 16) & 0xffffu;
-                    hi = (static_cast<uint32_t>(e) & 0xffffu) + 1u;
+        const uint32_t entries = d->row_ext_full[y];
+        if (entries != kRowTableFull) {
+            // The row's packed entries, four per step (two 16-byte loads in
+            // flight; slots past `entries` hold no entry of this row).
+            const uint64_t* tab = d->row_ext + (static_cast<size_t>(y) << d->row_ext_log2);
+            for (uint32_t k = 0; k < entries; k += 4) {
+                const ulonglong2 e01 = *reinterpret_cast<const ulonglong2*>(tab + k);
+                const ulonglong2 e23 = *reinterpret_cast<const ulonglong2*>(tab + k + 2);
+                const uint64_t ev[4] = {e01.x, e01.y, e23.x, e23.y};
+                int hit = -1;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    if (hit < 0 && k + c < entries && static_cast<uint32_t>(ev[c] >> 32) == root) hit = c;
+                }
+                if (hit >= 0) {
+                    lo = static_cast<uint32_t>(ev[hit] >> 16) & 0xffffu;
+                    hi = (static_cast<uint32_t>(ev[hit]) & 0xffffu) + 1u;
                     break;
                 }
-                if (e == ~0ull) break;
             }
         } else {
             const uint32_t base = static_cast<uint32_t>(y) * row_cap;
