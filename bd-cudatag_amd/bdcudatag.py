@@ -130,6 +130,7 @@ SIGNATURES = {
     "bd_batch_replay": (_I, [_P, _I, _I, ctypes.POINTER(ctypes.c_float)]),
     "bd_batch_get_tune": (_I, [_P, ctypes.POINTER(Tune)]),
     "bd_batch_set_tune": (_I, [_P, ctypes.POINTER(Tune)]),
+    "bd_batch_set_capture": (_I, [_P, _I]),
     "bd_batch_autotune": (_I, [_P, _I, ctypes.POINTER(Tune), ctypes.POINTER(ctypes.c_float)]),
     "bd_server_create": (_P, [_I, _I, ctypes.POINTER(ServerClass), ctypes.POINTER(Config),
                               ctypes.POINTER(_I)]),
@@ -297,6 +298,13 @@ class Batch:
         if rc < 0:
             raise RuntimeError(f"bd_batch_debug_copy failed: {last_error()}")
         return buf
+
+    def capture_stages(self, on: bool = True) -> None:
+        """Stage capture: later calls also keep the runs and labels of the
+        rows inside each band, so debug() shows every run (a run's label is
+        its band root; a band root's label is its component root)."""
+        if lib().bd_batch_set_capture(self.handle, 1 if on else 0) != BD_OK:
+            raise RuntimeError(f"bd_batch_set_capture failed: {last_error()}")
 
     def time_stages(self, reps: int = 10) -> dict:
         ms = (ctypes.c_float * (len(KERNELS) + 1))()

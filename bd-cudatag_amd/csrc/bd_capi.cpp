@@ -392,6 +392,19 @@ int32_t bd_batch_get_tune(bd_batch* batch, bd_tune* out) {
     return BD_OK;
 }
 
+int32_t bd_batch_set_capture(bd_batch* batch, int32_t on) {
+    if (!batch || !batch->engine) {
+        set_error("null batch");
+        return BD_ERROR_INVALID_ARGUMENT;
+    }
+    std::string error;
+    if (!batch->engine->set_capture(on != 0, &error)) {
+        set_error(error);
+        return BD_ERROR_CUDA;
+    }
+    return BD_OK;
+}
+
 int32_t bd_batch_set_tune(bd_batch* batch, const bd_tune* tune) {
     if (!batch || !batch->engine || !tune) {
         set_error("null batch or parameters");

@@ -74,6 +74,8 @@ FrameLayout layout_frame(int W, int H, int maxc, int row_ext_log2, size_t* curso
     o.keys = take(slots * 8);
     o.roots = take(slots * 4);
     o.nroots = take(static_cast<size_t>(th) * 4);
+    o.items = take(slots * 8);
+    o.rowi = take(static_cast<size_t>(H) * 4);
     o.rext = take((static_cast<size_t>(H) << row_ext_log2) * 8);
     o.rxfull = take(static_cast<size_t>(H) * 4);
     o.cands = take(static_cast<size_t>(maxc) * 4);
@@ -123,6 +125,8 @@ FrameDesc bind_frame(const Params& p, int W, int H, int maxc, int max_det, int r
     d.cand_keys = reinterpret_cast<uint64_t*>(base + o.keys);
     d.roots = reinterpret_cast<uint32_t*>(base + o.roots);
     d.band_nroots = reinterpret_cast<uint32_t*>(base + o.nroots);
+    d.items = reinterpret_cast<uint64_t*>(base + o.items);
+    d.row_items = reinterpret_cast<uint32_t*>(base + o.rowi);
     d.row_ext = reinterpret_cast<uint64_t*>(base + o.rext);
     d.row_ext_full = reinterpret_cast<uint32_t*>(base + o.rxfull);
     d.candidates = reinterpret_cast<uint32_t*>(base + o.cands);
@@ -846,6 +850,25 @@ bool Engine::set_tune_locked(const Tune& t, std::string* error) {
         lane_host_[l].tune = t;
         lane_dev_[l].tune = t;
     }
+    capture_all();
+    return true;
+}
+
+bool Engine::set_capture(bool on, std::string* error) {
+    const std::lock_guard<std::mutex> lock(mutex_);
+    if (resident_ready_) {
+        if (error) *error = "stage capture is not available on a resident serving lane";
+        return false;
+    }
+    hipError_t st = hipStreamSynchronize(stream_);
+    if (st == hipSuccess && stream2_) st = hipStreamSynchronize(stream2_);
+    if (st != hipSuccess) {
+        if (error) *error = hip_error("hipStreamSynchronize", st);
+        return false;
+    }
+    const int32_t v = on ? 1 : 0;
+    plan_host_.capture = plan_dev_.capture = v;
+    for (int l = 0; l < 2; ++l) lane_host_[l].capture = lane_dev_[l].capture = v;
     capture_all();
     return true;
 }

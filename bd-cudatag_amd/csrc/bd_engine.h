@@ -32,7 +32,7 @@ struct Params {
 // that holds them (an engine's arena or the no-allocation scratch array).
 struct FrameLayout {
     size_t tmin, tmax, tmean, tthr, trlo, trhi, roi, hist, bits, rowc, runs, label, area, bbox, keys, roots,
-        nroots, rext, rxfull, cands, quads, hquads, dec;
+        nroots, items, rowi, rext, rxfull, cands, quads, hquads, dec;
 };
 
 // Candidates kept per frame: the configured count clamped to [16, 1024], or
@@ -107,6 +107,10 @@ public:
     // launch parameters (one parameter at a time, keeping the best so far)
     // and leaves the fastest set in place.  *best_ms is per pipeline run.
     bool autotune(int reps, Tune* best, float* best_ms, std::string* error);
+    // Stage capture: with `on`, later runs also write the runs and labels of
+    // the rows inside each band (no kernel reads them; debug_copy shows
+    // them).  Waits for the streams and re-captures the graphs.
+    bool set_capture(bool on, std::string* error);
 
     // Asynchronous device-path launch of positions [0, n) (the rest are
     // inactive for this launch).  The caller must not launch again before

@@ -115,6 +115,15 @@ struct FrameDesc {
     uint64_t* cand_keys;  // height * row_cap
     uint32_t* roots;      // height * row_cap: band b's component roots from b * 8 * row_cap
     uint32_t* band_nroots;  // tile_h: roots listed per band
+    // Row items (k_binarize_runs -> k_stats, and k_fit's scan of a row whose
+    // component table overflowed) of the rows of bands joined in LDS:
+    // stretches of consecutive runs of one row with one band root, folded.
+    // height * row_cap, row y's row_items[y] items from y * row_cap:
+    // x0 | x1 << 13 (leftmost start, rightmost end) | (pixels - 1) << 26 |
+    // band-root flag << 39 | band root's band row << 40 | its index in that
+    // row << 43 (the flag marks the item holding the band root's own run).
+    uint64_t* items;
+    uint32_t* row_items;  // height
     uint64_t* row_ext;    // height << row_ext_log2: per row, the row's components
                           // root << 32 | x0 << 16 | x1 (leftmost start, rightmost
                           // end), packed from the row's first slot
@@ -182,6 +191,10 @@ struct Plan {
     int32_t class_band_runs[kWidthClasses];
     Tune tune;
     int32_t blocks[SH_COUNT];
+    // Nonzero: k_binarize_runs also writes the runs and labels of the rows
+    // inside each band, which no later kernel reads (stage capture for
+    // stage-by-stage comparisons).
+    int32_t capture;
 };
 
 // Device-resident serving (bd_server_create_resident): requests of one frame

@@ -926,19 +926,67 @@ __device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, int
     __syncthreads();
     for (uint32_t i = t; i < nb; i += blockDim.x) b_lab[i] = lds_find(b_lab, i);
     __syncthreads();
-    for (uint32_t i = t; i < nb; i += blockDim.x) {
-        const int r = row_of(i);
-        const uint32_t slot = static_cast<uint32_t>(y0 + r) * static_cast<uint32_t>(row_cap) + (i - s_off[r]);
-        const uint32_t root = b_lab[i];
-        const int rr = row_of(root);
-        const uint32_t rslot = static_cast<uint32_t>(y0 + rr) * static_cast<uint32_t>(row_cap) + (root - s_off[rr]);
-        d->runs[slot] = b_run[i] | (root == i ? kRootFlag : 0u);
-        d->label[slot] = rslot;
-        if (root == i) {
-            d->area[slot] = 0;
-            uint4* bb = reinterpret_cast<uint4*>(d->bbox + 4 * static_cast<size_t>(slot));
-            *bb = make_uint4(0xffffffffu, 0xffffffffu, 0u, 0u);
+    // Write-out, one wave per row, 64 runs at a time.  Global memory gets
+    // the runs and labels that later kernels read: the band's first and last
+    // rows (k_merge joins them to the neighbouring bands) and every band root
+    // (its label is the node the joins and the root walks go through); all
+    // runs under stage capture.  The run statistics leave as row items: each
+    // stretch of consecutive runs (among the 64) with one band root, folded
+    // by one wave scan (its leftmost start is its first run's, its rightmost
+    // end its last run's, as a row's runs are x-ordered), so k_stats folds a
+    // few dozen items per row instead of every run.  A band root's own run
+    // always starts a stretch (no earlier run of the row is in its set).
+    const bool capture = plan.capture != 0;
+    for (int r = wave; r < nrows; r += nwaves) {
+        const uint32_t rbase = so[r], rn = so[r + 1] - so[r];
+        const uint32_t row_slot = static_cast<uint32_t>(y0 + r) * static_cast<uint32_t>(row_cap);
+        const bool whole_row = capture || r == 0 || r == nrows - 1;
+        uint64_t* row_items = d->items + row_slot;
+        uint32_t placed = 0;
+        for (uint32_t c = 0; c < rn; c += kWave) {
+            const uint32_t k = c + static_cast<uint32_t>(lane);
+            const bool valid = k < rn;
+            const uint32_t i = rbase + k;
+            uint32_t run = 0, root = 0;
+            if (valid) {
+                run = b_run[i];
+                root = b_lab[i];
+            }
+            const bool is_root = valid && root == i;
+            const int rr = row_of(root);
+            const uint32_t ridx = root - so[rr];
+            if (valid && (whole_row || is_root)) {
+                d->runs[row_slot + k] = run | (is_root ? kRootFlag : 0u);
+                d->label[row_slot + k] = static_cast<uint32_t>(y0 + rr) * static_cast<uint32_t>(row_cap) + ridx;
+            }
+            if (is_root) {
+                d->area[row_slot + k] = 0;
+                uint4* bb = reinterpret_cast<uint4*>(d->bbox + 4 * static_cast<size_t>(row_slot + k));
+                *bb = make_uint4(0xffffffffu, 0xffffffffu, 0u, 0u);
+            }
+            const uint32_t x0 = static_cast<uint32_t>(run_x0(run)), x1 = static_cast<uint32_t>(run_x1(run));
+            const uint32_t len = valid ? x1 - x0 + 1u : 0u;
+            const uint32_t key = valid ? root : 0xffffffffu;
+            // A stretch starts where the band root changes; it ends before
+            // the next start or at the last valid lane.
+            const bool head = valid && key != dpp_from_lower(0xffffffffu, key);
+            const uint64_t heads = __ballot(head);
+            const uint32_t incl = wave_incl_scan(len);
+            const uint64_t later = lane == kWave - 1 ? 0ull : heads & (~0ull << (lane + 1));
+            const int last = static_cast<int>(min(rn - c, static_cast<uint32_t>(kWave))) - 1;
+            const int end = later ? __ffsll(static_cast<long long>(later)) - 2 : last;
+            const uint32_t end_x1 = static_cast<uint32_t>(__shfl(static_cast<int>(x1), end));
+            const uint32_t end_incl = static_cast<uint32_t>(__shfl(static_cast<int>(incl), end));
+            if (head) {
+                const uint64_t area = end_incl - incl + len;
+                row_items[placed + static_cast<uint32_t>(__popcll(heads & ((1ull << lane) - 1ull)))] =
+                    static_cast<uint64_t>(x0) | (static_cast<uint64_t>(end_x1) << 13) | ((area - 1u) << 26) |
+                    (static_cast<uint64_t>(is_root ? 1u : 0u) << 39) | (static_cast<uint64_t>(rr) << 40) |
+                    (static_cast<uint64_t>(ridx) << 43);
+            }
+            placed += static_cast<uint32_t>(__popcll(heads));
         }
+        if (lane == 0) d->row_items[y0 + r] = placed;
     }
 }
 
@@ -1145,21 +1193,23 @@ __global__ __launch_bounds__(256) void k_merge(Plan plan) {
 constexpr int kStatSlots = 256;
 constexpr uint32_t kEmpty = 0xffffffffu;
 
-// One workgroup per band of 8 rows, after all joins.
-//  1. Band roots (flagged runs: the LDS-joined band's roots, or every run of
-//     a band joined in global memory) walk to their component root, which
-//     is written to their label.
-//  2. Every run takes its root (its own label if it is a band root, else its
-//     band root's label) and writes it to its label; the run's length goes
-//     into an LDS table keyed by root and its extent into a per-row LDS table
-//     keyed by root (the row's components with their leftmost start and
-//     rightmost end).  The root table takes each root's x extent and last row
-//     from the row tables and is flushed with one set of global atomics per
-//     (band, root), so a component with thousands of runs does not serialise
-//     on its root's counters (reference k_region_stats :562-586 does one
+// One workgroup per band of 8 rows, after all joins, over the band's band
+// roots and items (k_binarize_runs: stretches of consecutive runs of one
+// row and one band-local component, folded; a band joined in global memory
+// has every run an item and a band root of its own).
+//  1. Each band root walks to its component root, which is written to its
+//     label; a band root that is its own component root is a component
+//     root: its row is the component's top row (the root is the component's
+//     first run in raster order) and it joins the band's root list (the
+//     roots k_collect filters).
+//  2. Every item takes its band root's component root; its pixel count
+//     goes into an LDS table keyed by root and its extent into a per-row LDS
+//     table keyed by root (the row's components with their leftmost start
+//     and rightmost end).  The root table takes each root's x extent and
+//     last row from the row tables and is flushed with one set of global
+//     atomics per (band, root) (reference k_region_stats :562-586 does one
 //     atomic per run); the row tables are written out for k_fit's row
-//     extremes (1 << kRowLog2 slots per row).  The root is the component's
-//     first run in raster order, so its row is the component's top row.
+//     extremes (1 << kRowLog2 slots per row).
 template <int kRowLog2>
 __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     constexpr int kRowSlots = 1 << kRowLog2;
@@ -1171,6 +1221,7 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         if (threadIdx.x == 0) d->band_nroots[ty] = 0;  // an inactive band has no runs
         return;
     }
+    const bool local = roi[3] != 0;
     const int H = d->height;
     const int y0 = ty * kTile;
     const int nrows = min(kTile, H - y0);
@@ -1198,8 +1249,9 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         hy1[e] = 0;
     }
     if (t < kWave) {
-        // Row offsets of the band: one lane per row, a wave prefix sum.
-        const uint32_t c = t < nrows ? d->row_count[y0 + t] : 0u;
+        // Item offsets of the band's rows (a band joined in global memory:
+        // its runs): one lane per row, a wave prefix sum.
+        const uint32_t c = t < nrows ? (local ? d->row_items[y0 + t] : d->row_count[y0 + t]) : 0u;
         uint32_t total;
         const uint32_t ex = wave_excl_scan(c, t, &total);
         if (t <= kTile) s_off[t] = ex;
@@ -1213,48 +1265,56 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     uint32_t so[kTile + 1];
 #pragma unroll
     for (int q = 0; q <= kTile; ++q) so[q] = s_off[q];
-    const uint32_t nb = so[kTile];
-    const auto row_of = [so](uint32_t i) {
+    const uint32_t band_base = static_cast<uint32_t>(y0) * row_cap;
+    const uint32_t nitems = so[kTile];
+    uint32_t* label = d->label;
+    // Item j of the band: x0, x1, pixels, band row, and its band root's slot
+    // (flagged on the band root's own item).  A band joined in global memory
+    // has every run an item and a band root of its own.
+    struct Item {
+        uint32_t x0, x1, area, broot;
+        int row;
+        bool flag;
+    };
+    const auto read_item = [&](uint32_t j) {
+        Item it;
         int r = 0;
 #pragma unroll
-        for (int q = 1; q < kTile; ++q) r += i >= so[q] ? 1 : 0;
-        return r;
+        for (int q = 1; q < kTile; ++q) r += j >= so[q] ? 1 : 0;
+        const uint32_t slot = band_base + static_cast<uint32_t>(r) * row_cap + (j - so[r]);
+        it.row = r;
+        if (local) {
+            const uint64_t v = d->items[slot];
+            it.x0 = static_cast<uint32_t>(v) & 0x1fffu;
+            it.x1 = static_cast<uint32_t>(v >> 13) & 0x1fffu;
+            it.area = (static_cast<uint32_t>(v >> 26) & 0x1fffu) + 1u;
+            it.flag = ((v >> 39) & 1u) != 0;
+            it.broot = band_base + static_cast<uint32_t>((v >> 40) & 7u) * row_cap + static_cast<uint32_t>(v >> 43);
+        } else {
+            const uint32_t run = d->runs[slot];
+            it.x0 = static_cast<uint32_t>(run_x0(run));
+            it.x1 = static_cast<uint32_t>(run_x1(run));
+            it.area = it.x1 - it.x0 + 1u;
+            it.flag = true;
+            it.broot = slot;
+        }
+        return it;
     };
-    const uint32_t band_base = static_cast<uint32_t>(y0) * row_cap;
-    uint32_t* label = d->label;
 
     // Band roots walk to their root, splitting the path on the way: each
     // visited label is lowered to its grandparent with atomicMin, which can
     // only move it towards the root, so concurrent walkers never undo each
-    // other.  A thread advances its walks in lockstep so their loads
+    // other.  A thread advances two walks in lockstep so their loads
     // overlap.  Walks only visit band roots (a band root's label always
-    // names another band root), so the labels of the other runs are final
-    // from the start.
+    // names another band root).  On return root[u] is the component root of
+    // band root slot[u], also written to its label.
     constexpr int kItems = 2;
-    const auto locate = [&](uint32_t i0, int* row, uint32_t* slot) {
-#pragma unroll
-        for (int u = 0; u < kItems; ++u) {
-            const uint32_t i = i0 + u * blockDim.x;
-            row[u] = i < nb ? row_of(i) : -1;
-            slot[u] = row[u] >= 0 ? band_base + static_cast<uint32_t>(row[u]) * row_cap + (i - s_off[row[u]]) : 0u;
-        }
-    };
-    const auto load = [&](const int* row, const uint32_t* slot, uint32_t* run, uint32_t* lab) {
-#pragma unroll
-        for (int u = 0; u < kItems; ++u) {
-            run[u] = row[u] >= 0 ? d->runs[slot[u]] : 0u;
-            lab[u] = row[u] >= 0 ? label[slot[u]] : 0u;
-        }
-    };
-    // On return a band root's lab[] is its component root, written to its
-    // label as well.
-    const auto walk = [&](const uint32_t* slot, const uint32_t* run, uint32_t* lab) {
+    const auto walk = [&](const uint32_t* slot, const bool* on, uint32_t* root) {
         uint32_t x[kItems], p[kItems];
 #pragma unroll
         for (int u = 0; u < kItems; ++u) {
-            const bool br = (run[u] & kRootFlag) != 0;
-            x[u] = br ? slot[u] : 0u;   // x == p ends a walk
-            p[u] = br ? lab[u] : 0u;
+            x[u] = on[u] ? slot[u] : 0u;  // x == p ends a walk
+            p[u] = on[u] ? label[slot[u]] : 0u;
         }
         bool live = true;
         while (live) {
@@ -1273,111 +1333,72 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         }
 #pragma unroll
         for (int u = 0; u < kItems; ++u) {
-            if (run[u] & kRootFlag) {
-                atomicMin(&label[slot[u]], x[u]);
-                lab[u] = x[u];
-            }
+            if (on[u]) atomicMin(&label[slot[u]], x[u]);
+            root[u] = x[u];
         }
     };
-    // The band roots (a few dozen among the band's hundreds of runs) are
-    // gathered into an LDS list and walk in one batch, so a band pays the
-    // walk's load latency once however many runs it has.  Band roots beyond
-    // the list's capacity (a band joined in global memory has every run a
-    // band root) walk where they are found.
+    // After its walk: a band root that is its own component root is a
+    // component root, whose row is the component's top row (the root is the
+    // component's first run in raster order); it joins the band's root list
+    // (the roots k_collect filters).
+    const auto settle = [&](uint32_t slot, uint32_t root) {
+        if (root == slot) {
+            d->bbox[4 * static_cast<size_t>(root) + 1] = slot / row_cap;
+            d->roots[band_base + atomicAdd(&s_nroots, 1u)] = root;
+        }
+    };
+    // The band roots (a few dozen per band) are gathered into an LDS list
+    // and walk in one batch, so a band pays the walk's load latency once.
+    // Band roots beyond the list's capacity walk where they are found.
     constexpr uint32_t kWalkCap = kItems * 256;
-    __shared__ uint32_t s_wslot[kWalkCap], s_wlab[kWalkCap];
-    // Component roots of the first kItems * 256 runs' band roots, by compact
-    // index.  A band root is the smallest run of its band-local component,
-    // so every one of these runs finds its band root here.
-    __shared__ uint32_t s_broot[kItems * 256];
-    const auto gather = [&](const int* row, const uint32_t* slot, const uint32_t* run, uint32_t* lab, bool first) {
-        uint32_t rest[kItems];
-        bool any = false;
-#pragma unroll
-        for (int u = 0; u < kItems; ++u) {
-            rest[u] = 0;
-            if (row[u] >= 0 && (run[u] & kRootFlag)) {
+    __shared__ uint32_t s_wslot[kWalkCap];
+    for (uint32_t j0 = 0; j0 < nitems; j0 += blockDim.x) {
+        const uint32_t j = j0 + static_cast<uint32_t>(t);
+        bool spill = false;
+        uint32_t spill_slot = 0;
+        if (j < nitems) {
+            const Item it = read_item(j);
+            if (it.flag) {
                 const uint32_t k = atomicAdd(&s_nwalk, 1u);
                 if (k < kWalkCap) {
-                    s_wslot[k] = slot[u];
-                    s_wlab[k] = lab[u];
+                    s_wslot[k] = it.broot;
                 } else {
-                    rest[u] = kRootFlag;
-                    any = true;
+                    spill = true;
+                    spill_slot = it.broot;
                 }
             }
         }
-        if (__any(any)) {
-            walk(slot, rest, lab);
-#pragma unroll
-            for (int u = 0; u < kItems; ++u) {
-                if (first && rest[u]) s_broot[t + u * blockDim.x] = lab[u];
-            }
+        if (__any(spill)) {
+            const uint32_t sl[kItems] = {spill_slot, 0u};
+            const bool on[kItems] = {spill, false};
+            uint32_t rt[kItems];
+            walk(sl, on, rt);
+            if (spill) settle(spill_slot, rt[0]);
         }
-    };
-    // The band's first kItems * blockDim.x runs stay in registers from here
-    // to the accumulation; a longer band reloads the rest.  (One loop for
-    // both keeps a single copy of the code.)
-    int row0[kItems];
-    uint32_t slot0[kItems], run0[kItems], lab0[kItems];
-#pragma unroll
-    for (int u = 0; u < kItems; ++u) {
-        row0[u] = -1;
-        slot0[u] = run0[u] = lab0[u] = 0;
-    }
-    for (uint32_t b0 = 0; b0 < nb; b0 += kItems * blockDim.x) {
-        int row[kItems];
-        uint32_t slot[kItems], run[kItems], lab[kItems];
-        locate(b0 + t, row, slot);
-        load(row, slot, run, lab);
-        if (b0 == 0) {
-#pragma unroll
-            for (int u = 0; u < kItems; ++u) {
-                row0[u] = row[u];
-                slot0[u] = slot[u];
-                run0[u] = run[u];
-                lab0[u] = lab[u];
-            }
-        }
-        gather(row, slot, run, lab, b0 == 0);
     }
     __syncthreads();
     const uint32_t nwalk = min(s_nwalk, kWalkCap);
     if (static_cast<uint32_t>(t) < nwalk) {
-        uint32_t slot[kItems], run[kItems], lab[kItems];
+        uint32_t sl[kItems], rt[kItems];
+        bool on[kItems];
 #pragma unroll
         for (int u = 0; u < kItems; ++u) {
             const uint32_t k = t + u * blockDim.x;
-            run[u] = k < nwalk ? kRootFlag : 0u;
-            slot[u] = k < nwalk ? s_wslot[k] : 0u;
-            lab[u] = k < nwalk ? s_wlab[k] : 0u;
+            on[u] = k < nwalk;
+            sl[u] = on[u] ? s_wslot[k] : 0u;
         }
-        walk(slot, run, lab);
+        walk(sl, on, rt);
 #pragma unroll
         for (int u = 0; u < kItems; ++u) {
-            if (!run[u]) continue;
-            const uint32_t rel = slot[u] - band_base;
-            int r = 0;
-#pragma unroll
-            for (int q = 1; q < kTile; ++q) r += rel >= static_cast<uint32_t>(q) * row_cap ? 1 : 0;
-            const uint32_t i = s_off[r] + (rel - static_cast<uint32_t>(r) * row_cap);
-            if (i < kItems * 256) s_broot[i] = lab[u];
+            if (on[u]) settle(sl[u], rt[u]);
         }
     }
     __syncthreads();
 
-    // Every run takes its root and is accounted.  One call per item with
-    // scalar arguments (a loop over the item arrays with the probe loop
-    // inside would not unroll and would put the arrays in scratch memory),
-    // made by the whole wave: a wave's 64 runs are consecutive runs of the
-    // band and mostly belong to one or two components, so the runs sharing
-    // the first lane's root are reduced across the wave (DPP) and folded
-    // into the table once; the other runs fold one by one.
-    const int lane = t & (kWave - 1);
-    // Area per root here; a root's x extent and last row are folded from
-    // the row tables at the end (or straight into global memory for a run
-    // whose row table is full).  A root that finds no table slot goes to
-    // global memory too.
+    // Every item is accounted under its component root.  Pixels per root
+    // here; a root's x extent and last row are folded from the row tables at
+    // the end (or straight into global memory for an item whose row table is
+    // full).  A root that finds no table slot goes to global memory too.
     const auto root_slot = [&](uint32_t root, bool insert) {
         const uint32_t h = (root * 2654435761u) >> 24;
         for (int probe = 0; probe < 32; ++probe) {
@@ -1422,98 +1443,14 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 2], x1);
         atomicMax(&d->bbox[4 * static_cast<size_t>(root) + 3], static_cast<uint32_t>(y0 + r));
     };
-    constexpr int kWaveFoldMin = 4;  // fewer shared runs fold one by one
-    const auto account = [&](int rowu, uint32_t slotu, uint32_t runu, uint32_t labu) {
-        const bool valid = rowu >= 0;
-        const uint32_t root = labu;
-        const uint32_t y = static_cast<uint32_t>(y0 + rowu);
-        const uint32_t x0 = static_cast<uint32_t>(run_x0(runu)), x1 = static_cast<uint32_t>(run_x1(runu));
-        if (valid) {
-            label[slotu] = root;
-            if (root == slotu) {
-                // A component root: its top row, and an entry in the band's
-                // root list (the roots k_collect filters).
-                d->bbox[4 * static_cast<size_t>(root) + 1] = y;
-                d->roots[band_base + atomicAdd(&s_nroots, 1u)] = root;
-            }
-        }
-        // What this lane folds: its own run, or the leader's wave total for
-        // the root table; for the row tables, its own run, or the first of
-        // the root's runs in each row of the wave with the row's last one's
-        // end (a row's runs are x-ordered and its lanes contiguous).
-        uint32_t f_area = x1 - x0 + 1, row_x1 = x1;
-        bool folds = valid, folds_row = valid;
-        const uint64_t vm = __ballot(valid);
-        if (vm) {
-            const int leader = __ffsll(static_cast<long long>(vm)) - 1;
-            const uint32_t lead_root = wave_lane(root, leader);
-            const bool same = valid && root == lead_root;
-            const uint64_t sm = __ballot(same);
-            if (__popcll(sm) >= kWaveFoldMin) {
-                const uint32_t area = wave_sum(same ? f_area : 0u);
-                const uint32_t lower_row = dpp_from_lower(0xfffffffeu, static_cast<uint32_t>(rowu));
-                const uint64_t starts = __ballot(static_cast<uint32_t>(rowu) != lower_row) | 1ull;
-                const uint64_t below = (1ull << lane) - 1ull;
-                const uint64_t seg_lo = starts & (below | (1ull << lane));
-                const int s0 = 63 - __clzll(static_cast<long long>(seg_lo));
-                const uint64_t above = starts & ~(below | (1ull << lane));
-                const int s1 = above ? __ffsll(static_cast<long long>(above)) - 1 : 64;
-                const uint64_t seg = (s1 >= 64 ? ~0ull : ((1ull << s1) - 1ull)) & ~((1ull << s0) - 1ull);
-                const uint64_t mine = sm & seg;
-                const int last = mine ? 63 - __clzll(static_cast<long long>(mine)) : lane;
-                const uint32_t last_x1 = static_cast<uint32_t>(__shfl(static_cast<int>(x1), last));
-                folds = valid && (!same || lane == leader);
-                folds_row = valid && (!same || (mine & below) == 0);
-                if (same) row_x1 = last_x1;
-                if (lane == leader) f_area = area;
-            }
-        }
-        if (folds) fold(root, f_area);
-        if (folds_row) fold_row(root, rowu, x0, row_x1);
-    };
-    // A run that is not a band root points at its band root, whose label
-    // is now the component root.
-    const auto resolve = [&](const int* row, const uint32_t* run, uint32_t* lab) {
-#pragma unroll
-        for (int u = 0; u < kItems; ++u) {
-            if (row[u] >= 0 && !(run[u] & kRootFlag)) lab[u] = label[lab[u]];
-        }
-    };
-    // The first runs take their band root's component root from LDS: the
-    // band root's compact index is its row's offset plus its place in the
-    // row, and a band root's own entry holds its component root.
-#pragma unroll
-    for (int u = 0; u < kItems; ++u) {
-        if (row0[u] < 0) continue;
-        uint32_t idx = t + u * blockDim.x;
-        if (!(run0[u] & kRootFlag)) {
-            const uint32_t rel = lab0[u] - band_base;
-            int r = 0;
-#pragma unroll
-            for (int q = 1; q < kTile; ++q) r += rel >= static_cast<uint32_t>(q) * row_cap ? 1 : 0;
-            idx = s_off[r] + (rel - static_cast<uint32_t>(r) * row_cap);
-        }
-        lab0[u] = s_broot[idx];
-    }
-    for (uint32_t b0 = 0; b0 < nb; b0 += kItems * blockDim.x) {
-        int row[kItems];
-        uint32_t slot[kItems], run[kItems], lab[kItems];
-        if (b0 == 0) {
-#pragma unroll
-            for (int u = 0; u < kItems; ++u) {
-                row[u] = row0[u];
-                slot[u] = slot0[u];
-                run[u] = run0[u];
-                lab[u] = lab0[u];
-            }
-        } else {
-            locate(b0 + t, row, slot);
-            load(row, slot, run, lab);
-            // A band root's label is its root (written by the walk).
-            resolve(row, run, lab);
-        }
-        account(row[0], slot[0], run[0], lab[0]);
-        account(row[1], slot[1], run[1], lab[1]);
+    for (uint32_t j = t; j < nitems; j += blockDim.x) {
+        const Item it = read_item(j);
+        // The band root's label is its component root, written by its walk
+        // before the barrier above; it is read past the vector L1, where the
+        // walks' plain loads may have left the label's old value.
+        const uint32_t root = __hip_atomic_load(&label[it.broot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        fold(root, it.area);
+        fold_row(root, it.row, it.x0, it.x1);
     }
     __syncthreads();
     if (t == 0) d->band_nroots[ty] = s_nroots;
@@ -1522,7 +1459,7 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     // not written); the row's count, or kRowTableFull when its table
     // overflowed, goes to row_ext_full.  One wave per row.
     {
-        const int wave = t / kWave;
+        const int wave = t / kWave, lane = t % kWave;
         const int nwaves = static_cast<int>(blockDim.x) / kWave;
         for (int r = wave; r < nrows; r += nwaves) {
             uint64_t* row_out = d->row_ext + (static_cast<size_t>(y0 + r) << kRowLog2);
@@ -1991,7 +1928,23 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
                     break;
                 }
             }
+        } else if (d->row_roi[4 * (y / kTile) + 3]) {
+            // The row's items whose band root's component root (its label,
+            // final after k_stats) is this root.
+            const uint32_t band_base = static_cast<uint32_t>(y / kTile) * kTile * row_cap;
+            const uint64_t* items = d->items + static_cast<size_t>(y) * row_cap;
+            const uint32_t n = d->row_items[y];
+            for (uint32_t k = 0; k < n; ++k) {
+                const uint64_t it = items[k];
+                const uint32_t broot =
+                    band_base + static_cast<uint32_t>((it >> 40) & 7u) * row_cap + static_cast<uint32_t>(it >> 43);
+                if (d->label[broot] != root) continue;
+                lo = min(lo, static_cast<uint32_t>(it) & 0x1fffu);
+                hi = max(hi, (static_cast<uint32_t>(it >> 13) & 0x1fffu) + 1u);
+            }
         } else {
+            // A band joined in global memory: every run of the row is a band
+            // root, its label its component root.
             const uint32_t base = static_cast<uint32_t>(y) * row_cap;
             const uint32_t n = d->row_count[y];
             for (uint32_t k = 0; k < n; ++k) {

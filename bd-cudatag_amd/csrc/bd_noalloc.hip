@@ -26,9 +26,9 @@
 
 namespace bd {
 
-// 48 MiB: the stage buffers of one frame up to 1920x1200 (45.8 MiB) plus
+// 64 MiB: the stage buffers of one frame up to 1920x1200 (63.4 MiB) plus
 // the descriptor, work list, scalars, codewords and output records.
-constexpr size_t kNoAllocScratchBytes = size_t(48) << 20;
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
 
 namespace {
 

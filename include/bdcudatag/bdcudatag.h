@@ -201,8 +201,10 @@ enum {
     BD_STAGE_ROW_ROI = 5,      /* i32[4 * tile_h]: active, start, end, band joined locally */
     BD_STAGE_BITS = 6,         /* u64[height * ceil(width/64)], bit x&63, 1 = white */
     BD_STAGE_ROW_COUNT = 7,    /* u32[height]: black runs per row (maximal; 32-px fragments in row 0) */
-    BD_STAGE_RUNS = 8,         /* u32[height * row_cap]: x0 | x1 << 16, bit 31 = band root */
-    BD_STAGE_LABELS = 9,       /* u32[height * row_cap]: root slot of each run */
+    BD_STAGE_RUNS = 8,         /* u32[height * row_cap]: x0 | x1 << 16, bit 31 = band root
+                                  (rows inside a band only under stage capture) */
+    BD_STAGE_LABELS = 9,       /* u32[height * row_cap]: a run's band root slot; a band root's
+                                  is its component root (so two hops name the root) */
     BD_STAGE_AREA = 10,        /* u32[height * row_cap], valid at roots */
     BD_STAGE_BBOX = 11,        /* u32[height * row_cap * 4]: x0, y0, x1, y1 at roots */
     BD_STAGE_CANDIDATES = 12,  /* u32[max_candidates]: root slots, largest first */
@@ -213,6 +215,12 @@ enum {
 };
 BD_API int64_t bd_batch_debug_copy(bd_batch* batch, int32_t index, int32_t stage, void* dst,
                                    int64_t capacity);
+/* Stage capture (no counterpart in the reference; for stage-by-stage
+ * comparisons): with on != 0, later calls also write the runs and labels of
+ * the rows inside each 8-row band, which the pipeline itself does not keep
+ * (it keeps each band's first and last rows and its band roots).  Detections
+ * are the same either way.  Waits for the batch's streams. */
+BD_API int32_t bd_batch_set_capture(bd_batch* batch, int32_t on);
 /* Geometry of frame `index`: out[0..7] = width, height, tile_w, tile_h,
  * words64, row_cap, max_candidates, max_detections. */
 BD_API int32_t bd_batch_frame_info(bd_batch* batch, int32_t index, int32_t out[8]);
@@ -371,7 +379,7 @@ BD_API const char* bd_server_last_error(void);
  * *count_device (device memory) holds min(found, max_detections) and
  * out_device (device memory) its first min(count, out_capacity) records,
  * homographies included.  Nothing is allocated: the working memory is one
- * static 48 MiB device array per GPU, held until the process ends, which
+ * static 64 MiB device array per GPU, held until the process ends, which
  * fits frames up to 1920x1200; larger frames are rejected with
  * BD_ERROR_INVALID_ARGUMENT before any GPU work.  Calls that overlap on one
  * GPU share that array and race: serialise them (one stream per GPU).

@@ -64,14 +64,24 @@ def oracle_root_name(o: dict, slot: int):
     return (ry, int(o["runs"][ry, ri]) & 0xFFFF)
 
 
+def gpu_root_of(labels: np.ndarray, slot: int) -> int:
+    """The component root of a run: its label names its band root, whose
+    label names the component root (a band root that is the component root
+    names itself)."""
+    flat = labels.reshape(-1)
+    first = int(flat[slot])
+    return int(flat[first])
+
+
 def gpu_components(runs: np.ndarray, labels: np.ndarray, row_count: np.ndarray, row_cap: int):
+    """Needs every run and label: a dump of a batch under stage capture."""
     H = runs.shape[0]
     rows = []
     for y in range(H):
         out = []
         for i in range(int(row_count[y])):
             r = int(runs[y, i])
-            root = int(labels[y, i])
+            root = gpu_root_of(labels, y * row_cap + i)
             ry, ri = divmod(root, row_cap)
             out.append((r & 0xFFFF, (r >> 16) & 0x7FFF, (ry, int(runs[ry, ri]) & 0xFFFF)))
         rows.append(merge_touching(out))
@@ -105,8 +115,41 @@ def component_stats_gpu(runs, labels, row_count, row_cap, area, bbox):
     return out
 
 
+def final_outputs(batch, index: int = 0) -> dict:
+    """The buffers every run writes in full, stage capture or not: the
+    candidate list, refined and hull quads, per-candidate decode records and
+    the counts (n candidates), for comparing a run without capture with one
+    under capture."""
+    sc = batch.debug("scalars", index).view(np.uint32)
+    n = int(sc[S_NCAND])
+    mc = batch.frames[index]["max_candidates"]
+    out = {"counts": sc[S_COUNTS:S_COUNTS + 4].copy(), "n": n,
+           "candidates": batch.debug("candidates", index).view(np.uint32)[:n].copy()}
+    out["quads"] = batch.debug("quads", index).view(np.uint32).reshape(mc, 8)[:n].copy()
+    out["hull_quads"] = batch.debug("hull_quads", index).view(np.uint32).reshape(mc, 8)[:n].copy()
+    out["decode"] = batch.debug("decode", index).reshape(mc, -1)[:n].copy()
+    return out
+
+
+def same_final_outputs(a: dict, b: dict) -> bool:
+    return a["n"] == b["n"] and all(np.array_equal(a[k], b[k]) for k in
+                                    ("counts", "candidates", "quads", "hull_quads", "decode"))
+
+
+def same_detections(a: list, b: list) -> bool:
+    return len(a) == len(b) and all(
+        x["id"] == y["id"] and x["hamming"] == y["hamming"]
+        and np.array_equal(np.asarray(x["corners"], np.float32).view(np.uint32),
+                           np.asarray(y["corners"], np.float32).view(np.uint32))
+        and np.array_equal(np.asarray(x["homography"], np.float32).view(np.uint32),
+                           np.asarray(y["homography"], np.float32).view(np.uint32))
+        for x, y in zip(a, b))
+
+
 def gpu_stage_dump(batch, index: int = 0) -> dict:
-    """All stage buffers of frame `index` of a bdcudatag.Batch as numpy arrays."""
+    """All stage buffers of frame `index` of a bdcudatag.Batch as numpy
+    arrays.  Runs and labels are whole only after a call under stage capture
+    (Batch.capture_stages)."""
     f = batch.frames[index]
     W, H, tw, th = f["width"], f["height"], f["tile_w"], f["tile_h"]
     w64, row_cap, mc = f["words64"], f["row_cap"], f["max_candidates"]

@@ -22,7 +22,7 @@ import pytest
 import bdcudatag
 import pyoracle
 import scenes
-from parity import compare_frame, gpu_stage_dump
+from parity import compare_frame, final_outputs, gpu_root_of, gpu_stage_dump, same_detections, same_final_outputs
 
 pytestmark = pytest.mark.gpu
 
@@ -51,7 +51,7 @@ def branch_counts(g: dict) -> dict:
         for y in range(8 * b, min(8 * b + 8, H)):
             n = int(rc[y])
             flagged += int(((runs[y, :n] & ROOT_FLAG) != 0).sum())
-            comps.update(labels[y, :n].tolist())
+            comps.update(gpu_root_of(labels, y * g["row_cap"] + i) for i in range(n))
         out["max_band_roots"] = max(out["max_band_roots"], flagged)
         if roi[b, 3]:
             out["max_band_roots_local"] = max(out["max_band_roots_local"], flagged)
@@ -60,13 +60,21 @@ def branch_counts(g: dict) -> dict:
 
 
 def check_batch(imgs, family=0, **cfg):
+    # A run as the pipeline runs it, then one under stage capture for the
+    # stage-by-stage comparison: the two give the same candidates, quads,
+    # decode records and detections.
     sizes = [(im.shape[1], im.shape[0]) for im in imgs]
     batch = bdcudatag.Batch(family, sizes, **cfg)
+    plain = batch.detect(imgs)
+    plain_out = [final_outputs(batch, i) for i in range(len(imgs))]
+    batch.capture_stages(True)
     dets = batch.detect(imgs)
     dumps = []
     for i, img in enumerate(imgs):
         W, H = sizes[i]
         print(f"  frame {i}: {W}x{H} family {family} cfg {cfg}", flush=True)
+        assert same_final_outputs(plain_out[i], final_outputs(batch, i)), i
+        assert same_detections(plain[i], dets[i]), i
         g = gpu_stage_dump(batch, i)
         ocfg = {k: v for k, v in cfg.items() if k != "use_cuda_graph"}
         o = pyoracle.Oracle(pyoracle.default_config(family, W, H, **ocfg)).run(img)

@@ -12,7 +12,8 @@ import pytest
 import bdcudatag
 import pyoracle
 import scenes
-from parity import compare_frame, gpu_stage_dump, oracle_components
+from parity import (compare_frame, final_outputs, gpu_stage_dump, oracle_components, same_detections,
+                    same_final_outputs)
 
 pytestmark = pytest.mark.gpu
 
@@ -20,7 +21,12 @@ pytestmark = pytest.mark.gpu
 def run_both(img, family=0, **cfg):
     H, W = img.shape
     det = bdcudatag.Detector(family, W, H, **cfg)
+    plain = det.detect(img)
+    plain_out = final_outputs(det.batch, 0)
+    det.batch.capture_stages(True)
     dets = det.detect(img)
+    assert same_final_outputs(plain_out, final_outputs(det.batch, 0))
+    assert same_detections(plain, dets)
     g = gpu_stage_dump(det.batch, 0)
     o = pyoracle.Oracle(pyoracle.default_config(family, W, H, **cfg)).run(img)
     return det, dets, g, o

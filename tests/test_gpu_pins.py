@@ -46,6 +46,7 @@ def test_nearest_peak_halo_gpu(level, inner):
     img, tag, outer = halo_frame(level)
     H, W = img.shape
     det = bdcudatag.Detector(0, W, H)
+    det.batch.capture_stages(True)
     dets = det.detect(img)
     o = pyoracle.Oracle(pyoracle.default_config(0, W, H)).run(img)
     bad = compare_frame(gpu_stage_dump(det.batch, 0), o, dets, det.stats())
@@ -60,6 +61,7 @@ def test_decode_fp64_gate_gpu():
     L = pyoracle.lib()
     L.or_decode_fp64_count(1)
     det = bdcudatag.Detector(0, 1280, 800)
+    det.batch.capture_stages(True)
     for seed in (0, 1, 2, 3):
         img, _ = scenes.scene(3, 1280, 800, seed)
         dets = det.detect(img)
