@@ -40,15 +40,18 @@ inline bool frame_span_fits(int width, int height, int stride) {
 // k_fit workgroups hold.
 constexpr int kSortCap = 2048;
 // Runs of one 8-row band joined in LDS by k_binarize_runs (dynamic LDS of
-// Plan::band_runs runs, 8 bytes each); a band with more runs is joined row by
-// row in global memory by k_merge instead.  The capacity follows the widest
-// frame of the plan, band_runs = clamp(1.6 * width rounded up to 512,
-// kBandRunsMin, kBandRunsMax): 3072 runs (24 KB, the least that holds the
-// band's pixel masks and end counts; six workgroups per CU) up to 1920 px,
-// where noisy scenes reach 930 runs per band on average and 2200 at most;
-// 6656 runs at 4096 px (3500 on average, about 1 % of bands above).  The
-// maximum keeps the workgroup within 64 KB of LDS.
-constexpr int kBandRunsMin = 3072;
+// Plan::class_band_runs runs, 8 bytes each); a band with more runs is joined
+// row by row in global memory by k_merge instead.  The capacity follows the
+// widest frame of the width class, band_runs = clamp(1.6 * width rounded up
+// to 512, kBandRunsMin, kBandRunsMax): 2048 runs (16 KB, eight workgroups
+// per CU) at 1280 px, where noisy scenes reach 983 runs per band on average,
+// 1826 at the 99th percentile and 1989 at most; 3072 at 1920 px (1605, 2746,
+// 2876); 6656 runs at 4096 px (3500 on average, about 1 % of bands above).
+// The maximum keeps the workgroup within 64 KB of LDS.  The band's pixel
+// masks and end counts (96 bytes per 64 pixels of width) and the pooled
+// threshold staging (6 bytes per tile column) fit in any capacity of at
+// least 1.6 * width.
+constexpr int kBandRunsMin = 1024;
 constexpr int kBandRunsMax = 7680;
 inline int band_runs_for_width(int width) {
     const int want = (width * 8 / 5 + 511) / 512 * 512;

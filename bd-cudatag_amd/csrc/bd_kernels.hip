@@ -605,12 +605,15 @@ __device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, int
     // staging for the pooled thresholds lives in b_run until the runs are
     // written, the pixel masks live in b_lab until the band union starts.
     uint8_t* pool_min = reinterpret_cast<uint8_t*>(b_run);
-    uint8_t* pool_max = pool_min + 3 * 1026;
-    uint64_t (*masks)[128] = reinterpret_cast<uint64_t (*)[128]>(b_lab);
+    uint8_t* pool_max = pool_min + 3 * (tw + 2);
+    // Row r's masks at masks + r * w64, its end counts at
+    // end_before + r * 2 * w64: 96 * w64 bytes in all, within b_lab's
+    // 4 * band_cap (band_cap >= 1.6 * width), as the 6 * (tw + 2) bytes of
+    // the pooled staging are within b_run's.
+    uint64_t* masks = reinterpret_cast<uint64_t*>(b_lab);
     // Per row and 32-px half word: runs of the row ending before it (pass B),
     // kept until the band union starts, after the masks in b_lab.
-    uint16_t (*end_before)[256] = reinterpret_cast<uint16_t (*)[256]>(b_lab + kTile * 128 * 2);
-    static_assert(kTile * 128 * 8 + kTile * 256 * 2 <= kBandRunsMin * 4, "masks and end counts fit in b_lab");
+    uint16_t* end_before = reinterpret_cast<uint16_t*>(masks + kTile * w64);
 
     // Pooled 3x3 thresholds for this tile row (k_tile_threshold): the three
     // tile rows are staged in LDS (padded with neutral values), then pooled.
@@ -723,7 +726,7 @@ __device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, int
                 }
                 m = (m & ~(0xffffffffull << (32 * g))) | (static_cast<uint64_t>(m32) << (32 * g));
             }
-            masks[r][w] = m;
+            masks[r * w64 + w] = m;
             d->bits[static_cast<size_t>(y) * w64 + w] = m;
             // The word's run starts, counted here rather than in a pass over
             // the masks: the bit before the word is the pixel left of it,
@@ -751,9 +754,9 @@ __device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, int
     const int nwaves = static_cast<int>(blockDim.x) / kWave;
     // Start/end masks of word w of band row r.
     auto edges = [&](int r, int w, uint64_t* starts, uint64_t* ends) {
-        const uint64_t m = masks[r][w];
-        uint64_t prev = w > 0 ? (masks[r][w - 1] >> 63) : 1ull;
-        uint64_t next = w + 1 < w64 ? (masks[r][w + 1] & 1ull) : 1ull;
+        const uint64_t m = masks[r * w64 + w];
+        uint64_t prev = w > 0 ? (masks[r * w64 + w - 1] >> 63) : 1ull;
+        uint64_t next = w + 1 < w64 ? (masks[r * w64 + w + 1] & 1ull) : 1ull;
         if (y0 + r == 0) {
             // Row 0: a fragment starts at bit 0 of every 32-px word and ends
             // at its last bit (reference k_find_runs :454, :485-487).
@@ -805,7 +808,7 @@ __device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, int
             uint32_t tot_s, tot_e;
             uint32_t ks = carry_s + wave_excl_scan(__popc(starts), lane, &tot_s);
             uint32_t ke = carry_e + wave_excl_scan(__popc(ends), lane, &tot_e);
-            if (hw < nh) end_before[r][hw] = static_cast<uint16_t>(ke);
+            if (hw < nh) end_before[r * 2 * w64 + hw] = static_cast<uint16_t>(ke);
             const int xb = hw * 32;
             if (local) {
                 // A run's start and end fill the low and high halves of its
@@ -882,7 +885,7 @@ __device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, int
                 uint64_t st, en;
                 edges(r - 1, h >> 1, &st, &en);
                 const uint32_t ends = static_cast<uint32_t>(en >> (32 * (h & 1)));
-                before = end_before[r - 1][h] + static_cast<uint32_t>(__popc(ends & ((1u << bit) - 1u)));
+                before = end_before[(r - 1) * 2 * w64 + h] + static_cast<uint32_t>(__popc(ends & ((1u << bit) - 1u)));
             }
             first_ahead[k] = static_cast<int>(s_off[r - 1] + before);
         }

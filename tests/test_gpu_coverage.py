@@ -97,7 +97,7 @@ def test_batch_4096x3072_beside_640x480():
 
 
 def test_stripes_1280x800_band_overflow():
-    # 1-px stripes: 450 runs per row, 3600 per band, above the 3072 runs a
+    # 1-px stripes: 450 runs per row, 3600 per band, above the 2048 runs a
     # 1280-px band joins in LDS.
     img = tag_frame(1280, 800, 3, 160, 400, 160)
     img[100:700, 300:1200:2] = 20
