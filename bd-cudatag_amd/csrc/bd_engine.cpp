@@ -897,7 +897,7 @@ bool Engine::autotune(int reps, Tune* best, float* best_ms, std::string* error) 
         }
         return true;
     };
-    static const int32_t kGrids[] = {256, 512, 1024, 1280, 1536, 2048, 4096};
+    static const int32_t kGrids[] = {256, 512, 1024, 1280, 1536, 1792, 2048, 4096};
     for (const int32_t g : kGrids) {
         Tune t = cur;
         t.fit_grid = g;

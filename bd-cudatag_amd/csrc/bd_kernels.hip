@@ -1878,15 +1878,18 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
         hull_out[t] = __int_as_float(0x7fc00000);
     }
 
-    __shared__ __align__(16) uint32_t boundary[2 * kMaxBoundaryRows];
     __shared__ float pdot[8][kFitDirections];
     __shared__ uint32_t ppoint[8][kFitDirections];
     __shared__ uint32_t extreme[kFitDirections];
     __shared__ uint32_t hull_pts[kFitDirections];
-    // Probe magnitudes of the refinement, rows padded to a multiple of four
-    // floats for 16-byte LDS reads.
+    // One LDS buffer holds the boundary points (until the hull is built) and
+    // then the refinement's probe magnitudes, rows padded to a multiple of
+    // four floats for 16-byte LDS reads.
     constexpr int kProbeStride = (kMaxProbes + 3) & ~3;
-    __shared__ __align__(16) uint32_t work_lds[4 * kMaxSamples * kProbeStride];
+    constexpr int kFitLdsWords = 2 * kMaxBoundaryRows > 4 * kMaxSamples * kProbeStride
+                                     ? 2 * kMaxBoundaryRows : 4 * kMaxSamples * kProbeStride;
+    __shared__ __align__(16) uint32_t work_lds[kFitLdsWords];
+    uint32_t* boundary = work_lds;
     float (*mags)[kMaxSamples][kProbeStride] = reinterpret_cast<float (*)[kMaxSamples][kProbeStride]>(work_lds);
     __shared__ float fit_quad[4][2];
     __shared__ float edge_lines[4][4];
@@ -2362,7 +2365,7 @@ __device__ inline uint32_t work_item(const Plan& plan, uint32_t item, uint32_t n
 
 // Persistent workgroups over the batch's candidate list (k_select appends
 // frame << 10 | candidate for every selected candidate of every frame).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_fit(Plan plan) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_fit(Plan plan) {
     const uint32_t nhead = plan.work[kWorkHeadCount];
     const uint32_t total = nhead + plan.work[kWorkCount];
     for (uint32_t item = blockIdx.x; item < total; item = next_item(&plan.work[kWorkFitCursor])) {
@@ -2922,17 +2925,17 @@ hipError_t launch_publish(const Plan& plan, const ResidentArgs& args, hipStream_
 }
 
 Tune default_tune() {
-    // The persistent grids default to whole rounds of resident workgroups:
-    // k_fit five per CU (its occupancy target, measured faster than six),
-    // k_decode as many as one CU holds of the compiled kernel (its registers
+    // The persistent grids default to one round of resident workgroups: as
+    // many per CU as one CU holds of the compiled kernel (registers and LDS
     // decide).  Computed once per process.
     static const Tune tuned = [] {
         Tune t;
-        int dev = 0, cus = 0, dec = 0;
+        int dev = 0, cus = 0, fit = 0, dec = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             return t;
-        t.fit_grid = std::min(std::max(cus * 5, 64), 16384);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_fit, 256, 0) == hipSuccess && fit > 0)
+            t.fit_grid = std::min(std::max(cus * fit, 64), 16384);
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&dec, k_decode<8>, 512, 0) == hipSuccess && dec > 0)
             t.decode_grid = std::min(std::max(cus * dec, 64), 16384);
         return t;

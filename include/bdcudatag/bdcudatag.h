@@ -246,7 +246,7 @@ BD_API int32_t bd_batch_replay(bd_batch* batch, int32_t reps, int32_t device_pat
  * change speed only: every setting gives bit-identical detections.
  */
 typedef struct bd_tune {
-    int32_t fit_grid;    /* persistent quad-fit workgroups, 64..16384 (default five per CU) */
+    int32_t fit_grid;    /* persistent quad-fit workgroups, 64..16384 (default: as many per CU as one CU holds) */
     int32_t decode_grid; /* persistent decode workgroups, 64..16384 (default: as many as the
                             device's CUs hold at once) */
     int32_t head_ranks;  /* each frame's largest candidates scheduled first, 0..4 (default 2) */
