@@ -150,6 +150,7 @@ def uneven_batch(bdcudatag, devmem, scenes, steps):
     devmem.sync()
     t = time.perf_counter() - t0
     dev_ms = batch.replay(steps, device=True) / steps
+    batch.close()
     return {"frames": len(all_sizes), "mpix_per_step": round(px / 1e6, 2),
             "mpix_s": round(px * steps / t / 1e6, 1), "ms_per_step": round(t / steps * 1e3, 3),
             "device_ms_per_step": round(dev_ms, 3), "device_ms_default_launch": round(default_ms, 3),
@@ -453,6 +454,11 @@ def main():
                               "frac": round(stream_achieved / HBM_PEAK_GBS, 4)},
         },
     }
+    # The engines above hold streams; a process has a few hardware queues
+    # (four by default), and the serving lanes below want one each.
+    batch.close()
+    if single is not None:
+        det.close()
     if args.uneven_steps > 0:
         out["uneven_batch"] = uneven_batch(bdcudatag, devmem, scenes, args.uneven_steps)
     if args.serve_seconds > 0:

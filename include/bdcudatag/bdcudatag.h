@@ -342,9 +342,11 @@ BD_API bd_server* bd_server_create(int32_t family, int32_t nclasses,
  * them; the launch's last kernel publishes each request's records, count
  * and completion flag into the ring.  The host only re-launches each lane's
  * captured graph and timestamps completions.  At most 16 lanes in all (keep
- * them within the device's hardware queues, 4 by default, so that no lane
- * waits behind another's admission).  submit/poll/wait_idle/simulate and
- * destroy work as for bd_server_create. */
+ * them within the process's hardware queues, 4 by default, which the streams
+ * of every other detector and batch the process holds share: a lane whose
+ * stream shares a queue waits behind another stream's work, another lane's
+ * admission wait included).  submit/poll/wait_idle/simulate and destroy work
+ * as for bd_server_create. */
 BD_API bd_server* bd_server_create_resident(int32_t family, int32_t nclasses,
                                             const bd_server_class* classes,
                                             const bd_config* config, double admit_wait_us,
