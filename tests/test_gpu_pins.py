@@ -15,7 +15,7 @@ import bdcudatag
 import pyoracle
 import scenes
 from parity import compare_frame, gpu_stage_dump
-from test_oracle_pins_fit import corner_distance, halo_frame
+from test_oracle_pins_fit import HALO_PINS, corner_distance, halo_frame
 
 pytestmark = pytest.mark.gpu
 
@@ -53,6 +53,20 @@ def test_nearest_peak_halo_gpu(level, inner):
     assert not bad, bad
     assert [d["id"] for d in dets] == [7]
     assert corner_distance(dets[0]["corners"], tag if inner else outer) < (0.3 if inner else 0.1)
+
+
+@pytest.mark.parametrize("side,ring,level,edge", HALO_PINS)
+def test_second_pass_range_and_valley_gpu(side, ring, level, edge):
+    img, tag, outer = halo_frame(level, ring=ring, side=side)
+    H, W = img.shape
+    det = bdcudatag.Detector(0, W, H)
+    det.batch.capture_stages(True)
+    dets = det.detect(img)
+    o = pyoracle.Oracle(pyoracle.default_config(0, W, H)).run(img)
+    bad = compare_frame(gpu_stage_dump(det.batch, 0), o, dets, det.stats())
+    assert not bad, bad
+    assert [d["id"] for d in dets] == [7]
+    assert corner_distance(dets[0]["corners"], tag if edge == "tag" else outer) < 0.1
 
 
 def test_decode_fp64_gate_gpu():

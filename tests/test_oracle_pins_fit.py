@@ -3,10 +3,11 @@
 Each check is chosen so that a plausible mistake in the restatement fails
 it.  Mutants of the oracle (a flipped parabola offset, no parabola, the rise
 probes at 0.5 px instead of 0.7, samples at (k+1)/(n+2) along the edge, the
-first-pass range 1.5 cells instead of 1.15, the nearest-peak acceptance at
-0.5 instead of 0.7, the hull's last-on-ties removal, a 0.45 px corner push,
-a flipped sharpening sign, max instead of min for the margin, cell centres
-at 0.45 of a cell) each fail at least one test here:
+first-pass range 1.5 cells instead of 1.15, a 3 px second-pass range
+instead of 2, the nearest-peak acceptance at 0.5 instead of 0.7, its valley
+bound at 0.9 instead of 0.5, the hull's last-on-ties removal, a 0.45 px
+corner push, a flipped sharpening sign, max instead of min for the margin,
+cell centres at 0.45 of a cell) each fail at least one test here:
 
   - the reference benchmark's own scenes scored by its own protocol must
     give its published accuracy (README.md:140-155);
@@ -16,7 +17,8 @@ at 0.45 of a cell) each fail at least one test here:
     id, hamming distance and margin;
   - hull reductions and rejections with hand-derived answers;
   - halos around a tag on either side of the nearest-peak rule's 0.7
-    acceptance;
+    acceptance, and narrow halos at the second pass's 2 px probe range and
+    at the rule's valley bound;
   - the build that restates the reference's transcendental calls and raw
     edge moments literally stays within a measured distance of this one,
     and farther than this one from the FP64 twin.
@@ -260,6 +262,30 @@ def test_nearest_peak_halo(level, inner):
     assert [d["id"] for d in dets] == [7]
     target = tag if inner else outer
     assert corner_distance(dets[0]["corners"], target) < (0.3 if inner else 0.1)
+
+
+# Narrow halos pin the second pass's probe range and the nearest-peak rule's
+# valley (gpu_detector.cu:871-877, :952-966):
+#  - a 3 px ring at 130 (inner rise 100, outer 110): the first pass takes the
+#    tag's border; the second pass probes at most 2 px, so the ring's outer
+#    edge 3 px away cannot pull it off (a 3 px range lands between the two);
+#  - 2.2-2.3 px rings at 115-120: the two rises overlap, so the dip between
+#    them stays above half the inner peak and the rule keeps the outer edge
+#    (a valley bound of 0.9 instead of 0.5 takes inner rises on some edges).
+HALO_PINS = [  # side, ring, level, the edge the fit must land on
+    (120.0, 3.0, 130, "tag"),
+    (80.0, 2.2, 120, "outer"),
+    (160.0, 2.3, 115, "outer"),
+]
+
+
+@pytest.mark.parametrize("side,ring,level,edge", HALO_PINS)
+def test_second_pass_range_and_valley(side, ring, level, edge):
+    img, tag, outer = halo_frame(level, ring=ring, side=side)
+    H, W = img.shape
+    dets = pyoracle.Oracle(pyoracle.default_config(0, W, H)).run(img)["detections"]
+    assert [d["id"] for d in dets] == [7]
+    assert corner_distance(dets[0]["corners"], tag if edge == "tag" else outer) < 0.1
 
 
 # ---------------------------------------------------------------------------
