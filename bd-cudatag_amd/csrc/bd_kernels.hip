@@ -1304,13 +1304,16 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         return it;
     };
 
-    // Band roots walk to their root, splitting the path on the way: each
-    // visited label is lowered to its grandparent with atomicMin, which can
-    // only move it towards the root, so concurrent walkers never undo each
-    // other.  A thread advances two walks in lockstep so their loads
-    // overlap.  Walks only visit band roots (a band root's label always
-    // names another band root).  On return root[u] is the component root of
-    // band root slot[u], also written to its label.
+    // Band roots walk to their root.  Every join is done (k_merge), so a
+    // label names an ancestor and the walk only reads; lowering the visited
+    // labels on the way (path splitting with atomicMin) was measured slower:
+    // the chains are short and the atomics cost more than they save.  A
+    // thread advances two walks in lockstep so their loads overlap.  Walks
+    // only visit band roots (a band root's label always names another band
+    // root).  On return root[u] is the component root of band root slot[u],
+    // which is written to its label (a plain store: only this walk writes
+    // that label, and concurrent walks through it read either value, both
+    // ancestors).
     constexpr int kItems = 2;
     const auto walk = [&](const uint32_t* slot, const bool* on, uint32_t* root) {
         uint32_t x[kItems], p[kItems];
@@ -1322,21 +1325,17 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
         bool live = true;
         while (live) {
             live = false;
-            uint32_t gp[kItems];
-#pragma unroll
-            for (int u = 0; u < kItems; ++u) gp[u] = p[u] != x[u] ? label[p[u]] : p[u];
 #pragma unroll
             for (int u = 0; u < kItems; ++u) {
                 if (p[u] == x[u]) continue;
-                if (gp[u] != p[u]) atomicMin(&label[x[u]], gp[u]);
                 x[u] = p[u];
-                p[u] = gp[u];
+                p[u] = label[p[u]];
                 live |= p[u] != x[u];
             }
         }
 #pragma unroll
         for (int u = 0; u < kItems; ++u) {
-            if (on[u]) atomicMin(&label[slot[u]], x[u]);
+            if (on[u] && x[u] != slot[u]) label[slot[u]] = x[u];
             root[u] = x[u];
         }
     };
