@@ -353,6 +353,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     }
 
     plan_host_.tune = default_tune();
+    plan_host_.wide_blocks = binarize_wide_blocks();
     plan_host_.frames = d_desc_host_;
     plan_host_.prefix = d_prefix_;
     plan_host_.nframes = nf;

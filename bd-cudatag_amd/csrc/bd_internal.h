@@ -198,6 +198,11 @@ struct Plan {
     // inside each band, which no later kernel reads (stage capture for
     // stage-by-stage comparisons).
     int32_t capture;
+    // A k_binarize_runs launch of at most this many workgroups (one round of
+    // 1024-thread workgroups, two per CU) runs them with 1024 threads, which
+    // shortens each band's chain; larger launches keep 256 threads per
+    // workgroup and more workgroups in flight.
+    int32_t wide_blocks;
 };
 
 // Device-resident serving (bd_server_create_resident): requests of one frame
@@ -236,6 +241,8 @@ constexpr int kStageCount = 10;
 // Default launch parameters: persistent grids of one full round of the
 // workgroups the device's CUs hold for the compiled k_fit and k_decode.
 Tune default_tune();
+// Plan::wide_blocks for the current device: twice its CU count.
+int32_t binarize_wide_blocks();
 
 // Enqueues one kernel (0..kStageCount-1) for every frame of the plan.
 hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream);

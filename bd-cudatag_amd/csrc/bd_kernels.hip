@@ -996,9 +996,11 @@ __device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, int
 // One workgroup per kBands consecutive bands of one frame (the band-rows
 // launch parameter: 8 or 16 pixel rows per workgroup, an execution tiling
 // only: the 8x8 threshold tiles and the 8-row union bands are the same),
-// the bands in turn, reusing the LDS.
-template <int kChunkWords, int kBands>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_binarize_runs(Plan plan, int cls) {
+// the bands in turn, reusing the LDS.  kThreads is 256, or 1024 for a
+// launch that fits in one round of 1024-thread workgroups (a single frame,
+// a few frames): the per-run phases of a band then run four times as wide.
+template <int kChunkWords, int kBands, int kThreads>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_binarize_runs(Plan plan, int cls) {
     int unit;
     const FrameDesc* d =
         plan.frames + frame_of(plan, (kBands == 1 ? SH_CLASS : SH_CLASS_PAIR) + cls, blockIdx.x, &unit);
@@ -2923,6 +2925,27 @@ hipError_t launch_publish(const Plan& plan, const ResidentArgs& args, hipStream_
     return hipGetLastError();
 }
 
+int32_t binarize_wide_blocks() {
+    static const int32_t blocks = [] {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            return 0;
+        return 2 * cus;
+    }();
+    return blocks;
+}
+
+template <int kChunkWords, int kBands>
+void launch_binarize(const Plan& plan, int cls, int blocks, size_t lds, hipStream_t stream) {
+    if (blocks <= plan.wide_blocks)
+        hipLaunchKernelGGL((k_binarize_runs<kChunkWords, kBands, 1024>), dim3(blocks), dim3(1024), lds, stream, plan,
+                           cls);
+    else
+        hipLaunchKernelGGL((k_binarize_runs<kChunkWords, kBands, 256>), dim3(blocks), dim3(256), lds, stream, plan,
+                           cls);
+}
+
 Tune default_tune() {
     // The persistent grids default to one round of resident workgroups: as
     // many per CU as one CU holds of the compiled kernel (registers and LDS
@@ -2956,13 +2979,13 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
                 const size_t lds = static_cast<size_t>(plan.class_band_runs[c]) * 8;
                 const bool wide = plan.tune.chunk_px == 128;
                 if (wide && pairs)
-                    hipLaunchKernelGGL((k_binarize_runs<2, 2>), dim3(blocks), dim3(256), lds, stream, plan, c);
+                    launch_binarize<2, 2>(plan, c, blocks, lds, stream);
                 else if (wide)
-                    hipLaunchKernelGGL((k_binarize_runs<2, 1>), dim3(blocks), dim3(256), lds, stream, plan, c);
+                    launch_binarize<2, 1>(plan, c, blocks, lds, stream);
                 else if (pairs)
-                    hipLaunchKernelGGL((k_binarize_runs<1, 2>), dim3(blocks), dim3(256), lds, stream, plan, c);
+                    launch_binarize<1, 2>(plan, c, blocks, lds, stream);
                 else
-                    hipLaunchKernelGGL((k_binarize_runs<1, 1>), dim3(blocks), dim3(256), lds, stream, plan, c);
+                    launch_binarize<1, 1>(plan, c, blocks, lds, stream);
             }
             break;
         case 3: hipLaunchKernelGGL(k_merge, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;

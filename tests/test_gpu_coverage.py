@@ -2,7 +2,9 @@
 reach, each test asserting from the GPU's own buffers that its path ran:
 
   - a batch holding a 4096x3072 frame (the largest frames of the uneven
-    batch workload) beside a 640x480 frame;
+    batch workload) beside a 640x480 frame, and a batch of eight 1280x800
+    frames (k_binarize_runs with 256-thread workgroups; smaller launches
+    use 1024);
   - bands with more runs than the LDS band capacity of k_binarize_runs
     (joined row by row in global memory by k_merge), rows with more runs
     than k_merge stages in LDS, bands with more band roots than k_stats'
@@ -94,6 +96,15 @@ def test_batch_4096x3072_beside_640x480():
     (gb, ob, db), (gs, os_, ds) = check_batch([big, small])
     assert gb["width"] == 4096 and int(ob["counts"][0]) > 0
     assert len(db) == len(ob["detections"]) and len(db) >= 1
+
+
+def test_batch_of_eight_1280x800():
+    # 800 bands in one launch: more than one round of 1024-thread
+    # workgroups, so k_binarize_runs runs with 256 threads per workgroup
+    # (a single frame's 100 bands run with 1024).
+    imgs = [scenes.scene(0, 1280, 800, 60 + i)[0] for i in range(8)]
+    dumps = check_batch(imgs)
+    assert sum(len(d) for _, _, d in dumps) >= 40
 
 
 def test_stripes_1280x800_band_overflow():
