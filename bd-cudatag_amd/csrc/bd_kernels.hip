@@ -1870,7 +1870,11 @@ __device__ inline int fit_row(int s, int kept, int rows, int by0) {
 //     evaluated in parallel into LDS, then one lane per (edge, sample) runs
 //     the peak rules and the parabola, and the moments are summed in the
 //     reference's 32-lane shuffle-down order (fit_edge_line_warp :881-1016).
+template <int kThreads>
 __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
+    // kThreads / 32 slices of the boundary in the direction probe; kThreads
+    // / 256 waves per edge in the refinement's rise loop.
+    constexpr int kParts = kThreads / 32, kSub = kThreads / 256;
     const int wave = t / kWave, lane = t % kWave;
     float* out = d->quads + 8 * static_cast<size_t>(ci);
     float* hull_out = d->hull_quads + 8 * static_cast<size_t>(ci);
@@ -1879,8 +1883,8 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
         hull_out[t] = __int_as_float(0x7fc00000);
     }
 
-    __shared__ float pdot[8][kFitDirections];
-    __shared__ uint32_t ppoint[8][kFitDirections];
+    __shared__ float pdot[kParts][kFitDirections];
+    __shared__ uint32_t ppoint[kParts][kFitDirections];
     __shared__ uint32_t extreme[kFitDirections];
     __shared__ uint32_t hull_pts[kFitDirections];
     // One LDS buffer holds the boundary points (until the hull is built) and
@@ -1973,7 +1977,7 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
         // read four at a time) x 32 directions; slices are merged in order,
         // so the result is still the first maximum in boundary order.
         const int dir = t & 31, part = t >> 5;
-        const int chunk = (((points + 7) / 8) + 3) & ~3;
+        const int chunk = (((points + kParts - 1) / kParts) + 3) & ~3;
         const int begin = part * chunk;
         const int end = min(points, begin + chunk);
         const float ux = c_dir_cos[dir], uy = c_dir_sin[dir];
@@ -2000,7 +2004,7 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
     if (t < kFitDirections) {
         float best = -FLT_MAX;
         uint32_t point = 0xffffffffu;
-        for (int p = 0; p < 8; ++p) {
+        for (int p = 0; p < kParts; ++p) {
             if (pdot[p][t] > best) {
                 best = pdot[p][t];
                 point = ppoint[p][t];
@@ -2150,14 +2154,15 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
         const uint32_t span = frame_span(W, H, stride);
         for (int iteration = 0; iteration < kFitIterations; ++iteration) {
             const bool first_pass = iteration == 0;
-            const EdgeGeom g = edge_geometry(fit_quad, wave, d->dim, first_pass);
+            const int edge = wave & 3, sub = wave >> 2;
+            const EdgeGeom g = edge_geometry(fit_quad, edge, d->dim, first_pass);
             if (g.ok) {
                 // Every (sample, probe) rise of this edge, one per lane per
                 // round (its four pixel-pair loads issued together); more per
                 // round measured slower: the larger code and registers cost
                 // more than the extra loads in flight gain.
                 const int total = g.samples * g.probes;
-                for (int it0 = lane; it0 < total; it0 += 1 * kWave) {
+                for (int it0 = sub * kWave + lane; it0 < total; it0 += kSub * kWave) {
                     Tap ta[1], tb[1];
                     int slot[1];
 #pragma unroll
@@ -2206,21 +2211,25 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
                         if (it0 + u * kWave >= total) continue;
                         const float ahead = tap_value(ta[u], va[u][0], va[u][1], va[u][2], va[u][3]);
                         const float behind = tap_value(tb[u], vb[u][0], vb[u][1], vb[u][2], vb[u][3]);
-                        (&mags[wave][0][0])[slot[u]] = (ahead < 0.0f || behind < 0.0f) ? 0.0f : ahead - behind;
+                        (&mags[edge][0][0])[slot[u]] = (ahead < 0.0f || behind < 0.0f) ? 0.0f : ahead - behind;
                     }
                 }
             }
-            // mags[wave] is written and read by this wave only.
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            {
+            // mags[edge] is written by the edge's waves and read by its first.
+            if constexpr (kSub == 1) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            } else {
+                __syncthreads();
+            }
+            if (sub == 0) {
                 float mx = 0.0f, my = 0.0f, mxx = 0.0f, mxy = 0.0f, myy = 0.0f;
                 int used = 0;
                 // Moments about the edge midpoint (see the oracle's note).
                 const float rx = g.ax + 0.5f * g.ex, ry = g.ay + 0.5f * g.ey;
                 if (g.ok && lane < g.samples) {
-                    const float* mg = mags[wave][lane];
+                    const float* mg = mags[edge][lane];
                     const float alpha = static_cast<float>(lane + 1) / static_cast<float>(g.samples + 1);
                     const float px = g.ax + alpha * g.ex;
                     const float py = g.ay + alpha * g.ey;
@@ -2306,13 +2315,13 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
                         const float cyy = myy * inv - cy * cy;
                         float lnx, lny;
                         axis_normal(cxx, cxy, cyy, &lnx, &lny);
-                        edge_lines[wave][0] = rx + cx;
-                        edge_lines[wave][1] = ry + cy;
-                        edge_lines[wave][2] = lnx;
-                        edge_lines[wave][3] = lny;
+                        edge_lines[edge][0] = rx + cx;
+                        edge_lines[edge][1] = ry + cy;
+                        edge_lines[edge][2] = lnx;
+                        edge_lines[edge][3] = lny;
                         have = 1;
                     }
-                    edge_have[wave] = have;
+                    edge_have[edge] = have;
                 }
             }
             __syncthreads();
@@ -2365,8 +2374,11 @@ __device__ inline uint32_t work_item(const Plan& plan, uint32_t item, uint32_t n
 }
 
 // Persistent workgroups over the batch's candidate list (k_select appends
-// frame << 10 | candidate for every selected candidate of every frame).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_fit(Plan plan) {
+// frame << 10 | candidate for every selected candidate of every frame):
+// 256 threads, or for a launch over one or two frames (one round of
+// candidates) 1024, four waves per edge in the refinement's rise loop.
+template <int kThreads>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7))) void k_fit(Plan plan) {
     const uint32_t nhead = plan.work[kWorkHeadCount];
     const uint32_t total = nhead + plan.work[kWorkCount];
     for (uint32_t item = blockIdx.x; item < total; item = next_item(&plan.work[kWorkFitCursor])) {
@@ -2377,7 +2389,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
         // them in registers across it and halve occupancy).
         int t = threadIdx.x;
         asm volatile("" : "+v"(t));
-        fit_candidate(plan.frames + (code >> 10), static_cast<int>(code & 1023u), t);
+        fit_candidate<kThreads>(plan.frames + (code >> 10), static_cast<int>(code & 1023u), t);
     }
 }
 
@@ -2956,7 +2968,7 @@ Tune default_tune() {
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             return t;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_fit, 256, 0) == hipSuccess && fit > 0)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_fit<256>, 256, 0) == hipSuccess && fit > 0)
             t.fit_grid = std::min(std::max(cus * fit, 64), 16384);
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&dec, k_decode<8>, 512, 0) == hipSuccess && dec > 0)
             t.decode_grid = std::min(std::max(cus * dec, 64), 16384);
@@ -3002,7 +3014,13 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
         }
         case 5: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
         case 6: hipLaunchKernelGGL(k_select, dim3(nf), dim3(256), 0, stream, plan); break;
-        case 7: hipLaunchKernelGGL(k_fit, dim3(min(plan.work_cap, plan.tune.fit_grid)), dim3(256), 0, stream, plan); break;
+        case 7:
+            if (plan.nframes <= 2 && plan.wide_blocks >= 2)
+                hipLaunchKernelGGL(k_fit<1024>, dim3(min(plan.work_cap, plan.wide_blocks / 2)), dim3(1024), 0, stream,
+                                   plan);
+            else
+                hipLaunchKernelGGL(k_fit<256>, dim3(min(plan.work_cap, plan.tune.fit_grid)), dim3(256), 0, stream, plan);
+            break;
         case 8: {
             const int w = plan.tune.decode_waves;
             const int per = w >= 16 ? w / 8 : 1;
