@@ -366,6 +366,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
             for (int f = first; f < first + n; ++f)
                 if (row_ext_log2_for_width(w_[f]) == kRowExtLog2Min + c) widest = std::max(widest, w_[f]);
             plan->class_band_runs[c] = widest > 0 ? band_runs_for_width(widest) : 0;
+            plan->class_merge_stage[c] = widest > 0 ? merge_stage_for_width(widest) : 0;
         }
     };
     set_band_runs(&plan_host_, 0, nf);

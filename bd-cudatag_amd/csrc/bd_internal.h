@@ -52,6 +52,13 @@ constexpr int kSortCap = 2048;
 // threshold staging (6 bytes per tile column) fit in any capacity of at
 // least 1.6 * width.
 constexpr int kBandRunsMin = 1024;
+// Largest k_merge row staging (runs per row; 32 KB of LDS with the labels of
+// both rows): every row of frames up to 4096 px wide.
+constexpr int kMergeStageMax = 2050;
+inline int merge_stage_for_width(int width) {
+    const int want = width / 2 + 2;
+    return want < kMergeStageMax ? want : kMergeStageMax;
+}
 constexpr int kBandRunsMax = 7680;
 inline int band_runs_for_width(int width) {
     const int want = (width * 8 / 5 + 511) / 512 * 512;
@@ -192,6 +199,9 @@ struct Plan {
     // LDS band capacity of k_binarize_runs (runs) per width class, from the
     // widest frame of the class in this plan (0: no frame of the class).
     int32_t class_band_runs[kWidthClasses];
+    // Runs per row that k_merge stages in LDS per width class: the widest
+    // frame's maximal runs per row (width / 2 + 2), at most kMergeStageMax.
+    int32_t class_merge_stage[kWidthClasses];
     Tune tune;
     int32_t blocks[SH_COUNT];
     // Nonzero: k_binarize_runs also writes the runs and labels of the rows

@@ -1045,11 +1045,11 @@ __device__ inline void uf_union(uint32_t* label, uint32_t a, uint32_t b) {
     }
 }
 
-// k_merge LDS: a pair hash set of kPairSlots entries and kMergeStage runs
-// per staged row (rows with more runs are read from global memory); under
-// 20 KB in all keeps eight workgroups resident per CU.
+// k_merge LDS: a pair hash set of kPairSlots entries (8 KB) and, in dynamic
+// LDS, the two boundary rows' runs and labels staged for up to
+// Plan::class_merge_stage runs per row (rows with more runs are read from
+// global memory).
 constexpr int kPairSlots = 1024;
-constexpr int kMergeStage = 760;
 
 // Records the (smaller, larger) label pair in the LDS hash set, or joins
 // the pair directly when 16 probes find no room.
@@ -1096,21 +1096,21 @@ __device__ inline void merge_run(unsigned long long* pairs, uint32_t* label, Run
     }
 }
 
-// One workgroup per band.  Its first row is joined to the last row of the
-// band above (8-connectivity: runs touching [x0-1, x1+1]; reference
-// k_merge_runs :509-557); a band that was not joined in LDS also joins all
-// of its own rows here.  For the boundary row both rows' runs and labels
-// are staged in LDS by coalesced loads and the row's runs are spread over
-// the whole workgroup.  The (label, label) pairs of all touching runs are
+// One workgroup per band, one launch per width class.  Its first row is
+// joined to the last row of the band above (8-connectivity: runs touching
+// [x0-1, x1+1]; reference k_merge_runs :509-557); a band that was not joined
+// in LDS also joins all of its own rows here.  Each pair of rows is staged
+// in LDS (runs and labels, coalesced loads) and the row's runs are spread
+// over the whole workgroup.  The (label, label) pairs of all touching runs are
 // first de-duplicated in an LDS hash set, so a large component whose runs
 // cross the boundary hundreds of times costs one union per distinct pair
 // instead of one per run.  Labels read at any time are ancestors of the
 // runs' roots, so the pairs join the same components whenever they are
 // read.  Runs of one row never need same-row joins: they are maximal, or in
 // row 0 they are fragments the reference leaves unjoined.
-__global__ __launch_bounds__(256) void k_merge(Plan plan) {
+__global__ __launch_bounds__(256) void k_merge(Plan plan, int cls) {
     int ty;
-    const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
+    const FrameDesc* d = plan.frames + frame_of(plan, SH_CLASS + cls, blockIdx.x, &ty);
     if (!d->active) return;
     const int32_t* roi = d->row_roi + 4 * ty;
     if (!roi[0]) return;
@@ -1122,55 +1122,46 @@ __global__ __launch_bounds__(256) void k_merge(Plan plan) {
     const int last_row = local ? 1 : nrows;  // exclusive, band-relative
     if (first_row >= last_row) return;
     const int t = threadIdx.x;
-    const int wave = t / kWave, lane = t % kWave;
     __shared__ unsigned long long pairs[kPairSlots];
-    __shared__ uint32_t s_cur[kMergeStage], s_cur_lab[kMergeStage], s_prev[kMergeStage], s_prev_lab[kMergeStage];
+    extern __shared__ uint32_t merge_stage[];
+    const int stage_cap = plan.class_merge_stage[cls];
+    uint32_t* s_cur = merge_stage;
+    uint32_t* s_cur_lab = merge_stage + stage_cap;
+    uint32_t* s_prev = merge_stage + 2 * stage_cap;
+    uint32_t* s_prev_lab = merge_stage + 3 * stage_cap;
     for (int e = t; e < kPairSlots; e += blockDim.x) pairs[e] = ~0ull;
     const uint32_t row_cap = static_cast<uint32_t>(d->row_cap);
     uint32_t* label = d->label;
-    if (first_row == 0) {
-        // The band's first row against the band above, whole workgroup.
-        const int y = y0;
-        const int n = static_cast<int>(d->row_count[y]);
-        const int pn = static_cast<int>(d->row_count[y - 1]);
-        const uint32_t cur_base = static_cast<uint32_t>(y) * row_cap;
-        const uint32_t prev_base = static_cast<uint32_t>(y - 1) * row_cap;
-        if (n > 0 && pn > 0) {
-            if (n <= kMergeStage && pn <= kMergeStage) {
-                for (int i = t; i < n; i += blockDim.x) {
-                    s_cur[i] = d->runs[cur_base + i];
-                    s_cur_lab[i] = label[cur_base + i];
-                }
-                for (int i = t; i < pn; i += blockDim.x) {
-                    s_prev[i] = d->runs[prev_base + i];
-                    s_prev_lab[i] = label[prev_base + i];
-                }
-                __syncthreads();
-                for (int i = t; i < n; i += blockDim.x)
-                    merge_run(pairs, label, s_cur, s_cur_lab, s_prev, s_prev_lab, pn, i);
-            } else {
-                __syncthreads();
-                for (int i = t; i < n; i += blockDim.x)
-                    merge_run(pairs, label, d->runs + cur_base, label + cur_base, d->runs + prev_base,
-                              label + prev_base, pn, i);
-            }
-        } else {
-            __syncthreads();
-        }
-    } else {
-        __syncthreads();
-    }
-    // Rows of a band that was not joined in LDS, one wave per row.
-    for (int r = max(first_row, 1) + wave; r < last_row; r += static_cast<int>(blockDim.x) / kWave) {
+    __syncthreads();
+    // Row by row (the band's first row against the band above; every row of
+    // a band not joined in LDS against the row above it), the whole
+    // workgroup on each pair of rows: both rows' runs and labels staged in
+    // LDS by coalesced loads when they fit, else read from global memory.
+    for (int r = first_row; r < last_row; ++r) {
         const int y = y0 + r;
         const int n = static_cast<int>(d->row_count[y]);
         const int pn = static_cast<int>(d->row_count[y - 1]);
         if (n == 0 || pn == 0) continue;
         const uint32_t cur_base = static_cast<uint32_t>(y) * row_cap;
         const uint32_t prev_base = static_cast<uint32_t>(y - 1) * row_cap;
-        for (int i = lane; i < n; i += kWave)
-            merge_run(pairs, label, d->runs + cur_base, label + cur_base, d->runs + prev_base, label + prev_base,
-                      pn, i);
+        if (n <= stage_cap && pn <= stage_cap) {
+            __syncthreads();  // the previous pair's readers are done
+            for (int i = t; i < n; i += blockDim.x) {
+                s_cur[i] = d->runs[cur_base + i];
+                s_cur_lab[i] = label[cur_base + i];
+            }
+            for (int i = t; i < pn; i += blockDim.x) {
+                s_prev[i] = d->runs[prev_base + i];
+                s_prev_lab[i] = label[prev_base + i];
+            }
+            __syncthreads();
+            for (int i = t; i < n; i += blockDim.x)
+                merge_run(pairs, label, s_cur, s_cur_lab, s_prev, s_prev_lab, pn, i);
+        } else {
+            for (int i = t; i < n; i += blockDim.x)
+                merge_run(pairs, label, d->runs + cur_base, label + cur_base, d->runs + prev_base,
+                          label + prev_base, pn, i);
+        }
     }
     __syncthreads();
     // Distinct pairs are gathered to the front of the table (each thread
@@ -2948,9 +2939,14 @@ int32_t binarize_wide_blocks() {
     return blocks;
 }
 
+// 1024 threads per workgroup when the launch fits in one round of them, or
+// when the class's band capacity leaves room for at most two workgroups per
+// CU (wide frames): two 1024-thread workgroups keep a CU's 32 wave slots
+// busy where two 256-thread ones would hold eight.
 template <int kChunkWords, int kBands>
 void launch_binarize(const Plan& plan, int cls, int blocks, size_t lds, hipStream_t stream) {
-    if (blocks <= plan.wide_blocks)
+    constexpr size_t kLdsPerCu = 160 * 1024;
+    if (blocks <= plan.wide_blocks || kLdsPerCu / (lds + 2048) <= 2)
         hipLaunchKernelGGL((k_binarize_runs<kChunkWords, kBands, 1024>), dim3(blocks), dim3(1024), lds, stream, plan,
                            cls);
     else
@@ -3000,7 +2996,15 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
                     launch_binarize<1, 1>(plan, c, blocks, lds, stream);
             }
             break;
-        case 3: hipLaunchKernelGGL(k_merge, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
+        case 3:
+            // One launch per width class present, staging its rows in LDS.
+            for (int c = 0; c < kWidthClasses; ++c) {
+                const int blocks = plan.blocks[SH_CLASS + c];
+                if (blocks == 0) continue;
+                const size_t lds = static_cast<size_t>(plan.class_merge_stage[c]) * 16;
+                hipLaunchKernelGGL(k_merge, dim3(blocks), dim3(256), lds, stream, plan, c);
+            }
+            break;
         case 4: {
             // One launch per width class present, with the class's row tables.
             static_assert(kWidthClasses == 3, "one k_stats instance per width class");

@@ -167,6 +167,7 @@ int detect_noalloc(const Params& p, int width, int height, const uint8_t* gray, 
     plan.nframes = 1;
     plan.work_cap = maxc;
     plan.class_band_runs[cls] = band_runs_for_width(width);
+    plan.class_merge_stage[cls] = merge_stage_for_width(width);
     plan.tune = default_tune();
     plan.wide_blocks = binarize_wide_blocks();
     plan.tune.streams = 1;

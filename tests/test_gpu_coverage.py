@@ -7,7 +7,7 @@ reach, each test asserting from the GPU's own buffers that its path ran:
     use 1024);
   - bands with more runs than the LDS band capacity of k_binarize_runs
     (joined row by row in global memory by k_merge), rows with more runs
-    than k_merge stages in LDS, bands with more band roots than k_stats'
+    than k_merge stages in LDS (8192 px wide), bands with more band roots than k_stats'
     walk list, and bands with more component roots than k_stats' root table
     (the global-atomic fallbacks), from 1-px stripes, isolated dots, a 1-px
     checkerboard and salt-and-pepper noise;
@@ -29,7 +29,7 @@ from parity import compare_frame, final_outputs, gpu_root_of, gpu_stage_dump, sa
 pytestmark = pytest.mark.gpu
 
 ROOT_FLAG = 0x80000000
-MERGE_STAGE = 760      # runs per row that k_merge stages in LDS
+MERGE_STAGE_MAX = 2050  # runs per row k_merge stages in LDS at most (width / 2 + 2 below)
 WALK_CAP = 512         # band roots k_stats walks from its LDS list
 ROOT_SLOTS = 256       # component roots per band in k_stats' LDS table
 
@@ -125,9 +125,20 @@ def test_stripes_1920x1080_merge_and_stats_overflow():
     (g, o, dets), = check_batch([img])
     c = branch_counts(g)
     assert c["nonlocal_bands"] > 0, c
-    assert c["max_row_runs"] > MERGE_STAGE, c
+    assert c["max_row_runs"] > 760, c
     assert c["max_band_roots"] > WALK_CAP and c["max_band_components"] > ROOT_SLOTS, c
     assert [d["id"] for d in dets] == [17]
+
+
+def test_stripes_8192_rows_beyond_merge_staging():
+    # 1-px stripes over 6000 px: 3000 runs per row, more than k_merge stages
+    # in LDS, so the boundary rows are joined from global memory.
+    img = tag_frame(8192, 256, 21, 600, 128, 120)
+    img[20:236, 1500:7500:2] = 20
+    (g, o, dets), = check_batch([img])
+    c = branch_counts(g)
+    assert c["max_row_runs"] > MERGE_STAGE_MAX and c["nonlocal_bands"] > 0, c
+    assert [d["id"] for d in dets] == [21]
 
 
 def test_dots_local_band_many_roots():
@@ -169,7 +180,7 @@ def test_salt_and_pepper(W, H, rows, black):
     (g, o, dets), = check_batch([img])
     c = branch_counts(g)
     if W == 4096:
-        assert c["nonlocal_bands"] > 0 and c["max_row_runs"] > MERGE_STAGE, c
+        assert c["nonlocal_bands"] > 0 and c["max_row_runs"] > 760, c
     else:
         assert c["max_band_components"] > ROOT_SLOTS, c
 
