@@ -432,7 +432,8 @@ hipError_t Engine::enqueue_lane(bool device_path, int lane) {
 // of these differs from the last upload, so repeated calls on the same
 // buffers launch the graphs alone.  The pinned staging is not written again
 // before the launch completes (callers wait for it).
-hipError_t Engine::upload_descriptors() {
+hipError_t Engine::upload_descriptors(bool* uploaded) {
+    *uploaded = false;
     const int nf = frame_count();
     bool changed = !up_valid_;
     for (int f = 0; f < nf && !changed; ++f) {
@@ -452,22 +453,31 @@ hipError_t Engine::upload_descriptors() {
         up_active_[f] = h_desc_dev_[f].active;
     }
     up_valid_ = true;
+    *uploaded = true;
     return hipSuccess;
 }
 
 hipError_t Engine::launch(bool device_path) {
+    bool uploaded = false;
     if (device_path) {
-        const hipError_t st = upload_descriptors();
+        const hipError_t st = upload_descriptors(&uploaded);
         if (st != hipSuccess) return st;
     }
     if (!two_lanes()) {
         hipGraphExec_t g = device_path ? graph_dev_ : graph_host_;
         return g ? hipGraphLaunch(g, stream_) : enqueue_lane(device_path, -1);
     }
-    // The second lane starts after everything queued on stream_ so far;
-    // stream_ waits for it before anything queued later.
-    hipError_t st = hipEventRecord(ev_fork_, stream_);
-    if (st == hipSuccess) st = hipStreamWaitEvent(stream2_, ev_fork_, 0);
+    // The second lane starts after the descriptor upload when this launch
+    // queued one on stream_ (a cross-stream wait costs ~35 us of host time,
+    // so it is skipped otherwise: every earlier launch of this engine has
+    // completed on both streams, or is one of back-to-back replays of the
+    // same frames, whose lanes share no buffer); stream_ waits for the
+    // second lane before anything queued later.
+    hipError_t st = hipSuccess;
+    if (uploaded) {
+        st = hipEventRecord(ev_fork_, stream_);
+        if (st == hipSuccess) st = hipStreamWaitEvent(stream2_, ev_fork_, 0);
+    }
     for (int l = 0; l < 2 && st == hipSuccess; ++l) {
         hipGraphExec_t g = device_path ? lane_graph_dev_[l] : lane_graph_host_[l];
         st = g ? hipGraphLaunch(g, l == 1 ? stream2_ : stream_) : enqueue_lane(device_path, l);

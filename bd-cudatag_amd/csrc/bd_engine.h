@@ -144,7 +144,9 @@ private:
     // One pipeline run on the staged frames, by graph where one was
     // captured; everything later queued on stream_ runs after it.
     hipError_t launch(bool device_path);
-    hipError_t upload_descriptors();
+    // Copies the device-path descriptors to the device when a frame pointer,
+    // stride or active flag changed; *uploaded says whether it queued a copy.
+    hipError_t upload_descriptors(bool* uploaded);
     bool set_tune_locked(const Tune& t, std::string* error);
     bool replay_locked(int reps, bool device_path, float* ms, std::string* error);
     void release();
