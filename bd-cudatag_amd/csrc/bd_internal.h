@@ -180,7 +180,7 @@ constexpr int kHeadSlots = 32;
 // decode waves per workgroup, the pixels per binarization item and the
 // pixel rows per binarization workgroup.
 struct Tune {
-    int32_t fit_grid = 1792;     // 7 resident 256-thread workgroups per CU (default_tune: the occupancy)
+    int32_t fit_grid = 2048;     // 8 resident 256-thread workgroups per CU (default_tune: the occupancy)
     int32_t decode_grid = 1024;  // 4 resident 512-thread workgroups per CU
     int32_t head_ranks = 2;
     int32_t streams = 2;

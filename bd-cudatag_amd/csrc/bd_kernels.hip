@@ -2369,7 +2369,7 @@ __device__ inline uint32_t work_item(const Plan& plan, uint32_t item, uint32_t n
 // 256 threads, or for a launch over one or two frames (one round of
 // candidates) 1024, four waves per edge in the refinement's rise loop.
 template <int kThreads>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7))) void k_fit(Plan plan) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kThreads == 256 ? 8 : 7))) void k_fit(Plan plan) {
     const uint32_t nhead = plan.work[kWorkHeadCount];
     const uint32_t total = nhead + plan.work[kWorkCount];
     for (uint32_t item = blockIdx.x; item < total; item = next_item(&plan.work[kWorkFitCursor])) {
