@@ -2628,7 +2628,7 @@ __device__ void decode_choose(const FrameDesc* d, int ci, bool have_quad, const 
 // workgroups take items by a fixed stride (no cursor).  The codebook (shared
 // by every frame of the batch) is copied to LDS once per workgroup.
 template <int kWaves>
-__global__ __launch_bounds__(kWaves * 64) void k_decode(Plan plan) {
+__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))) void k_decode(Plan plan) {
     constexpr int kPerBlock = kWaves >= 16 ? kWaves / 8 : 1;   // candidates per workgroup
     constexpr int kWavesPerCand = kWaves / kPerBlock;           // 4 or 8
     constexpr int kVariantsPerWave = 8 / kWavesPerCand;         // 2 or 1
