@@ -1206,8 +1206,10 @@ constexpr uint32_t kEmpty = 0xffffffffu;
 //     atomics per (band, root) (reference k_region_stats :562-586 does one
 //     atomic per run); the row tables are written out for k_fit's row
 //     extremes (1 << kRowLog2 slots per row).
-template <int kRowLog2>
-__global__ __launch_bounds__(256) void k_stats(Plan plan) {
+// kThreads: 256, or 512 for the 256-slot tables (frames over 2560 px), whose
+// 33 KB of LDS leaves four workgroups per CU.
+template <int kRowLog2, int kThreads>
+__global__ __launch_bounds__(kThreads) void k_stats(Plan plan) {
     constexpr int kRowSlots = 1 << kRowLog2;
     int ty;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_CLASS + (kRowLog2 - kRowExtLog2Min), blockIdx.x, &ty);
@@ -1345,7 +1347,7 @@ __global__ __launch_bounds__(256) void k_stats(Plan plan) {
     // The band roots (a few dozen per band) are gathered into an LDS list
     // and walk in one batch, so a band pays the walk's load latency once.
     // Band roots beyond the list's capacity walk where they are found.
-    constexpr uint32_t kWalkCap = kItems * 256;
+    constexpr uint32_t kWalkCap = kItems * kThreads;
     __shared__ uint32_t s_wslot[kWalkCap];
     for (uint32_t j0 = 0; j0 < nitems; j0 += blockDim.x) {
         const uint32_t j = j0 + static_cast<uint32_t>(t);
@@ -3009,11 +3011,11 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
             // One launch per width class present, with the class's row tables.
             static_assert(kWidthClasses == 3, "one k_stats instance per width class");
             if (plan.blocks[SH_CLASS + 0])
-                hipLaunchKernelGGL(k_stats<kRowExtLog2Min + 0>, dim3(plan.blocks[SH_CLASS + 0]), dim3(256), 0, stream, plan);
+                hipLaunchKernelGGL((k_stats<kRowExtLog2Min + 0, 256>), dim3(plan.blocks[SH_CLASS + 0]), dim3(256), 0, stream, plan);
             if (plan.blocks[SH_CLASS + 1])
-                hipLaunchKernelGGL(k_stats<kRowExtLog2Min + 1>, dim3(plan.blocks[SH_CLASS + 1]), dim3(256), 0, stream, plan);
+                hipLaunchKernelGGL((k_stats<kRowExtLog2Min + 1, 256>), dim3(plan.blocks[SH_CLASS + 1]), dim3(256), 0, stream, plan);
             if (plan.blocks[SH_CLASS + 2])
-                hipLaunchKernelGGL(k_stats<kRowExtLog2Min + 2>, dim3(plan.blocks[SH_CLASS + 2]), dim3(256), 0, stream, plan);
+                hipLaunchKernelGGL((k_stats<kRowExtLog2Min + 2, 512>), dim3(plan.blocks[SH_CLASS + 2]), dim3(512), 0, stream, plan);
             break;
         }
         case 5: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
