@@ -2757,11 +2757,15 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
         __syncthreads();
     }
     const int n = min(static_cast<int>(s_base), maxd);
+    // The entries' ids, loaded once (the loops below compare every pair).
+    __shared__ int s_lid[256];
+    for (int i = t; i < n; i += blockDim.x) s_lid[i] = d->dec[list[i]].id;
+    __syncthreads();
     // Any id shared by two entries?  If not, de-duplication keeps every entry.
     for (int i = t; i < n; i += blockDim.x) {
-        const int id = d->dec[list[i]].id;
+        const int id = s_lid[i];
         for (int j = 0; j < n; ++j) {
-            if (j != i && d->dec[list[j]].id == id) {
+            if (j != i && s_lid[j] == id) {
                 s_any_shared = 1;
                 break;
             }
@@ -2769,7 +2773,10 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
     }
     __syncthreads();
     if (!s_any_shared) {
-        for (int i = t; i < n; i += blockDim.x) k_src[i] = list[i];
+        for (int i = t; i < n; i += blockDim.x) {
+            k_src[i] = list[i];
+            k_id[i] = s_lid[i];
+        }
         if (t == 0) s_kept = n;
     } else {
         for (int i = 0; i < n; ++i) {
@@ -2812,12 +2819,13 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
     }
     __syncthreads();
     const int kept = s_kept;
-    // Stable placement by id (the reference's insertion sort).
+    // Stable placement by id (the reference's insertion sort); k_id holds
+    // the kept entries' ids.
     for (int i = t; i < kept; i += blockDim.x) {
-        const int id = d->dec[k_src[i]].id;
+        const int id = k_id[i];
         int rank = 0;
         for (int j = 0; j < kept; ++j) {
-            const int oj = d->dec[k_src[j]].id;
+            const int oj = k_id[j];
             rank += (oj < id) || (oj == id && j < i);
         }
         bd_detection o = d->dec[k_src[i]];
