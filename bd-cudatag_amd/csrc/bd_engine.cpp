@@ -5,8 +5,9 @@
 // stream, events around each frame, and a captured graph replayed per call
 // (H2D copy -> kernels -> D2H of counts and detections).  Differences: one
 // engine serves a list of frames (the batch API), the device-pointer path is
-// a graph too (its first node copies the frame descriptors), and the
-// homography is computed on the device inside the graph.
+// a graph too (a small k_bind launch before it rebinds the frame descriptors
+// when a pointer changed), and the homography is computed on the device
+// inside the graph.
 #include "bd_engine.h"
 
 #include <algorithm>
@@ -178,7 +179,6 @@ void Engine::release() {
     if (h_input_) (void)hipHostFree(h_input_);
     if (h_scal_) (void)hipHostFree(h_scal_);
     if (h_out_) (void)hipHostFree(h_out_);
-    if (h_desc_dev_) (void)hipHostFree(h_desc_dev_);
     graph_host_ = graph_dev_ = nullptr;
     ev_start_ = ev_stop_ = nullptr;
     ev_fork_ = ev_join_ = nullptr;
@@ -188,7 +188,7 @@ void Engine::release() {
     h_input_ = nullptr;
     h_scal_ = nullptr;
     h_out_ = nullptr;
-    h_desc_dev_ = nullptr;
+    h_desc_dev_.clear();
 }
 
 bool Engine::init(const Params& params, const std::vector<int>& widths,
@@ -279,12 +279,13 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     st = hipHostMalloc(reinterpret_cast<void**>(&h_input_), std::max<size_t>(input_bytes_, 1), hipHostMallocPortable);
     if (st == hipSuccess) st = hipHostMalloc(reinterpret_cast<void**>(&h_scal_), static_cast<size_t>(nf) * S_COUNT * 4, hipHostMallocPortable);
     if (st == hipSuccess) st = hipHostMalloc(reinterpret_cast<void**>(&h_out_), static_cast<size_t>(nf) * max_det_ * sizeof(bd_detection), hipHostMallocPortable);
-    if (st == hipSuccess) st = hipHostMalloc(reinterpret_cast<void**>(&h_desc_dev_), static_cast<size_t>(nf) * sizeof(FrameDesc), hipHostMallocPortable);
-    // k_finalize writes each frame's detection records straight into the
-    // pinned host array (only the records found cross PCIe; a copy node of
-    // the whole array moved nframes x max_detections x 84 bytes per call).
+    // k_finalize writes each frame's detection records and counts straight
+    // into the pinned host arrays (only the records found cross PCIe; copy
+    // nodes cost a blit kernel each, ~4 us at the end of every lane).
     bd_detection* out_dev = nullptr;
+    uint32_t* scal_dev = nullptr;
     if (st == hipSuccess) st = hipHostGetDevicePointer(reinterpret_cast<void**>(&out_dev), h_out_, 0);
+    if (st == hipSuccess) st = hipHostGetDevicePointer(reinterpret_cast<void**>(&scal_dev), h_scal_, 0);
     if (st != hipSuccess) {
         if (error) *error = hip_error("hipHostMalloc", st);
         return false;
@@ -298,6 +299,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         d.gray = d_input_ + input_off_[f];
         d.scal = d_scal_ + static_cast<size_t>(f) * S_COUNT;
         d.out = out_dev + static_cast<size_t>(f) * max_det_;
+        d.host_counts = scal_dev + static_cast<size_t>(f) * S_COUNT + S_COUNTS;
         prefix[SH_TILE_ROW * (nf + 1) + f + 1] = prefix[SH_TILE_ROW * (nf + 1) + f] + d.tile_h;
         prefix[SH_FRAME * (nf + 1) + f + 1] = f + 1;
         const int pairs = (d.tile_w + 1) / 2;
@@ -312,7 +314,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
             pp[f + 1] = pp[f] + (mine ? (d.tile_h + 1) / 2 : 0);
         }
     }
-    std::memcpy(h_desc_dev_, desc_.data(), desc_.size() * sizeof(FrameDesc));
+    h_desc_dev_ = desc_;
     // Per-lane prefix tables: the whole-batch table restricted to the lane's
     // frames and rebased to zero.
     std::vector<int32_t> lane_prefix[2];
@@ -421,17 +423,14 @@ hipError_t Engine::enqueue_lane(bool device_path, int lane) {
         st = hipMemcpyAsync(d_input_ + begin, h_input_ + begin, end - begin, hipMemcpyHostToDevice, s);
     }
     if (st == hipSuccess) st = launch_pipeline(plan, s);
-    if (st == hipSuccess)
-        st = hipMemcpyAsync(h_scal_ + static_cast<size_t>(first) * S_COUNT, d_scal_ + static_cast<size_t>(first) * S_COUNT,
-                            static_cast<size_t>(n) * S_COUNT * 4, hipMemcpyDeviceToHost, s);
     return st;
 }
 
 // The device path's descriptors change only in frame pointer, stride and
-// active flag; they are copied (whole, before the lanes fork) only when one
-// of these differs from the last upload, so repeated calls on the same
-// buffers launch the graphs alone.  The pinned staging is not written again
-// before the launch completes (callers wait for it).
+// active flag; they are rebound (by k_bind launches carrying them as kernel
+// arguments, before the lanes fork) only when one of these differs from the
+// last binding, so repeated calls on the same buffers launch the graphs
+// alone.
 hipError_t Engine::upload_descriptors(bool* uploaded) {
     *uploaded = false;
     const int nf = frame_count();
@@ -441,9 +440,19 @@ hipError_t Engine::upload_descriptors(bool* uploaded) {
         changed = d.gray != up_gray_[f] || d.stride != up_stride_[f] || d.active != up_active_[f];
     }
     if (!changed) return hipSuccess;
-    const hipError_t st = hipMemcpyAsync(d_desc_dev_, h_desc_dev_, static_cast<size_t>(nf) * sizeof(FrameDesc),
-                                         hipMemcpyHostToDevice, stream_);
-    if (st != hipSuccess) return st;
+    for (int first = 0; first < nf; first += kBindFrames) {
+        BindArgs args = {};
+        args.first = first;
+        args.count = std::min(kBindFrames, nf - first);
+        for (int k = 0; k < args.count; ++k) {
+            const FrameDesc& d = h_desc_dev_[first + k];
+            args.gray[k] = d.gray;
+            args.stride[k] = d.stride;
+            args.active[k] = d.active;
+        }
+        const hipError_t st = launch_bind(d_desc_dev_, args, stream_);
+        if (st != hipSuccess) return st;
+    }
     up_gray_.resize(nf);
     up_stride_.resize(nf);
     up_active_.resize(nf);

@@ -169,11 +169,11 @@ private:
     uint32_t* d_work_ = nullptr;        // batch candidate list (header, head and tail regions)
 
     uint8_t* h_input_ = nullptr;        // pinned staging
-    uint32_t* h_scal_ = nullptr;
-    bd_detection* h_out_ = nullptr;
-    FrameDesc* h_desc_dev_ = nullptr;   // pinned staging of the device-path descriptors
+    uint32_t* h_scal_ = nullptr;        // S_COUNT per frame; k_finalize writes the counts
+    bd_detection* h_out_ = nullptr;     // max_det records per frame, written by k_finalize
+    std::vector<FrameDesc> h_desc_dev_;  // host copy of the device-path descriptors
     // Frame pointer, stride and active flag of each device-path descriptor as
-    // last uploaded; a launch uploads the descriptors only when one changed.
+    // last bound; a launch rebinds the descriptors only when one changed.
     std::vector<const uint8_t*> up_gray_;
     std::vector<int32_t> up_stride_, up_active_;
     bool up_valid_ = false;

@@ -590,6 +590,9 @@ __device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, int
     const int y0 = ty * kTile;
     const int nrows = min(kTile, H - y0);
     const int t = threadIdx.x;
+    // Stage capture also keeps the buffers no later kernel reads: the pooled
+    // thresholds, the binary image and the runs and labels inside the band.
+    const bool capture = plan.capture != 0;
     int32_t* roi = d->row_roi + 4 * ty;
     const int row_on = roi[0], rs = roi[1], re = roi[2];
 
@@ -643,12 +646,14 @@ __device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, int
         }
         const int tv = (mx - mn >= floor_contrast) ? ((mn + mx + 1) >> 1) : ref1;
         thr[tx] = static_cast<uint8_t>(tv);
-        d->tile_thresh[static_cast<size_t>(ty) * tw + tx] = static_cast<uint8_t>(tv);
+        if (capture) d->tile_thresh[static_cast<size_t>(ty) * tw + tx] = static_cast<uint8_t>(tv);
     }
     if (!row_on) {
-        for (int i = t; i < nrows * w64; i += blockDim.x) {
-            const int r = i / w64;
-            d->bits[static_cast<size_t>(y0 + r) * w64 + (i - r * w64)] = ~0ull;
+        if (capture) {
+            for (int i = t; i < nrows * w64; i += blockDim.x) {
+                const int r = i / w64;
+                d->bits[static_cast<size_t>(y0 + r) * w64 + (i - r * w64)] = ~0ull;
+            }
         }
         if (t < nrows) d->row_count[y0 + t] = 0;
         return;
@@ -727,7 +732,7 @@ __device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, int
                 m = (m & ~(0xffffffffull << (32 * g))) | (static_cast<uint64_t>(m32) << (32 * g));
             }
             masks[r * w64 + w] = m;
-            d->bits[static_cast<size_t>(y) * w64 + w] = m;
+            if (capture) d->bits[static_cast<size_t>(y) * w64 + w] = m;
             // The word's run starts, counted here rather than in a pass over
             // the masks: the bit before the word is the pixel left of it,
             // thresholded by the same rule (white outside the row's ROI
@@ -939,7 +944,6 @@ __device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, int
     // end its last run's, as a row's runs are x-ordered), so k_stats folds a
     // few dozen items per row instead of every run.  A band root's own run
     // always starts a stretch (no earlier run of the row is in its set).
-    const bool capture = plan.capture != 0;
     for (int r = wave; r < nrows; r += nwaves) {
         const uint32_t rbase = so[r], rn = so[r + 1] - so[r];
         const uint32_t row_slot = static_cast<uint32_t>(y0 + r) * static_cast<uint32_t>(row_cap);
@@ -2716,7 +2720,7 @@ __device__ inline void centre_side(const float* c, float* side, float* cx, float
 // One workgroup per frame: decoded candidates in candidate order (the first
 // max_detections), de-duplicated as the reference's k_finalize (:1542-1601)
 // does it serially, then placed by a stable rank on id, each with its FP64
-// homography.
+// homography; the records and the frame's counts go to pinned host memory.
 __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
     const FrameDesc* d = plan.frames + blockIdx.x;
     if (!d->active) return;
@@ -2834,6 +2838,9 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
         d->out[rank] = o;
     }
     if (t == 0) d->scal[S_COUNTS + 1] = static_cast<uint32_t>(kept);
+    // The counts for the host (every other count is final: its kernels ran).
+    if (t < 5 && d->host_counts)
+        d->host_counts[t] = t == 1 ? static_cast<uint32_t>(kept) : d->scal[S_COUNTS + t];
 }
 
 // ---------------------------------------------------------------------------
@@ -2926,7 +2933,24 @@ __global__ __launch_bounds__(64) void k_publish(Plan plan, ResidentArgs a) {
     }
 }
 
+// Frame bindings of a device-path call (one wave): pointer, stride and
+// active flag of up to kBindFrames positions from the kernel arguments into
+// their descriptors.
+__global__ __launch_bounds__(64) void k_bind(FrameDesc* frames, BindArgs a) {
+    const int k = threadIdx.x;
+    if (k >= a.count) return;
+    FrameDesc* d = frames + a.first + k;
+    d->gray = a.gray[k];
+    d->stride = a.stride[k];
+    d->active = a.active[k];
+}
+
 }  // namespace
+
+hipError_t launch_bind(FrameDesc* frames, const BindArgs& args, hipStream_t stream) {
+    hipLaunchKernelGGL(k_bind, dim3(1), dim3(64), 0, stream, frames, args);
+    return hipGetLastError();
+}
 
 hipError_t launch_admit(const Plan& plan, const ResidentArgs& args, hipStream_t stream) {
     hipLaunchKernelGGL(k_admit, dim3(1), dim3(64), 0, stream, plan, args);

@@ -199,7 +199,8 @@ enum {
     BD_STAGE_HIST = 3,         /* u32[256 span + 64 mean] of the last frame */
     BD_STAGE_SCALARS = 4,      /* u32[16], see bd_internal.h */
     BD_STAGE_ROW_ROI = 5,      /* i32[4 * tile_h]: active, start, end, band joined locally */
-    BD_STAGE_BITS = 6,         /* u64[height * ceil(width/64)], bit x&63, 1 = white */
+    BD_STAGE_BITS = 6,         /* u64[height * ceil(width/64)], bit x&63, 1 = white
+                                  (written only under stage capture) */
     BD_STAGE_ROW_COUNT = 7,    /* u32[height]: black runs per row (maximal; 32-px fragments in row 0) */
     BD_STAGE_RUNS = 8,         /* u32[height * row_cap]: x0 | x1 << 16, bit 31 = band root
                                   (rows inside a band only under stage capture) */
@@ -211,15 +212,17 @@ enum {
     BD_STAGE_QUADS = 13,       /* f32[max_candidates * 8] */
     BD_STAGE_HULL_QUADS = 14,  /* f32[max_candidates * 8] */
     BD_STAGE_DECODE = 15,      /* bd_detection[max_candidates] per candidate, id -1 = none */
-    BD_STAGE_TILE_THRESH = 16  /* u8[tile_h * tile_w], pooled thresholds */
+    BD_STAGE_TILE_THRESH = 16  /* u8[tile_h * tile_w], pooled thresholds (written only
+                                  under stage capture) */
 };
 BD_API int64_t bd_batch_debug_copy(bd_batch* batch, int32_t index, int32_t stage, void* dst,
                                    int64_t capacity);
 /* Stage capture (no counterpart in the reference; for stage-by-stage
- * comparisons): with on != 0, later calls also write the runs and labels of
- * the rows inside each 8-row band, which the pipeline itself does not keep
- * (it keeps each band's first and last rows and its band roots).  Detections
- * are the same either way.  Waits for the batch's streams. */
+ * comparisons): with on != 0, later calls also write the binary image, the
+ * pooled thresholds and the runs and labels of the rows inside each 8-row
+ * band, which no later kernel reads (the pipeline keeps each band's first
+ * and last rows and its band roots).  Detections are the same either way.
+ * Waits for the batch's streams. */
 BD_API int32_t bd_batch_set_capture(bd_batch* batch, int32_t on);
 /* Geometry of frame `index`: out[0..7] = width, height, tile_w, tile_h,
  * words64, row_cap, max_candidates, max_detections. */
