@@ -142,8 +142,10 @@ private:
     // (Re)captures the graphs the current launch parameters use.
     void capture_all();
     // One pipeline run on the staged frames, by graph where one was
-    // captured; everything later queued on stream_ runs after it.
-    hipError_t launch(bool device_path);
+    // captured; with `join`, everything later queued on stream_ runs after
+    // it (without, stream2_'s lane ends on its own: the caller waits for
+    // both streams).
+    hipError_t launch(bool device_path, bool join = true);
     // Copies the device-path descriptors to the device when a frame pointer,
     // stride or active flag changed; *uploaded says whether it queued a copy.
     hipError_t upload_descriptors(bool* uploaded);
@@ -195,6 +197,7 @@ private:
     hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr;
+    hipEvent_t ev_stop2_ = nullptr;  // end of stream2_'s lane in a synchronous call
     hipEvent_t stage_events_[kStageCount + 1] = {};
     hipGraphExec_t graph_host_ = nullptr, graph_dev_ = nullptr;
     ResidentArgs resident_{};

@@ -423,8 +423,8 @@ def main():
             "workload": f"batch of {B} x {W}x{H} u8 frames, 8 tag36h11 tags each (seeded), "
                         "inputs resident in HBM, one bd_batch_detect_device call per step",
             "batch": B, "width": W, "height": H, "family": "tag36h11",
-            "launch": ("two captured hipGraphs per step, one per half of the batch, on two streams "
-                       "(joined by an event before the host waits)" if tune["streams"] == 2 else
+            "launch": ("two captured hipGraphs per step, one per part of the batch (34/30 frames), "
+                       "on two streams; the host waits for each stream's end event" if tune["streams"] == 2 else
                        "one captured hipGraph per step on one stream"),
         },
         "single_frame": single,

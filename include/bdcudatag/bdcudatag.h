@@ -253,8 +253,9 @@ typedef struct bd_tune {
     int32_t decode_grid; /* persistent decode workgroups, 64..16384 (default: as many as the
                             device's CUs hold at once) */
     int32_t head_ranks;  /* each frame's largest candidates scheduled first, 0..4 (default 2) */
-    int32_t streams;     /* 2: a batch of two or more frames runs as two concurrent halves of
-                            about equal pixel count, on two streams; 1: as one launch sequence
+    int32_t streams;     /* 2: a batch of two or more frames runs as two concurrent parts (the
+                            first about 2 Mpx larger, at most 1/16 of the batch, as its graph
+                            starts first), on two streams; 1: as one launch sequence
                             (default 2; 0 in bd_batch_set_tune selects the default; a one-frame
                             batch reports 1) */
     int32_t decode_waves; /* waves per decode workgroup: 4 (two variants per wave), 8 (one variant
