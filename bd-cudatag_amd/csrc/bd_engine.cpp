@@ -231,8 +231,8 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     // second, or as near as the frame sizes allow: the second lane's graph
     // starts about 20 us later (the host submits the first graph meanwhile),
     // and a synchronous call waits for the later of the two.  Measured on 64
-    // frames of 1280x800: 34/30 frames 0.399 ms per call against 0.405 for
-    // 32/32.
+    // frames of 1280x800 (three engines per split, calls interleaved): 34/30
+    // frames 0.394 ms per call, 33/31 0.396, 35/29 0.398, 32/32 0.415.
     int split = 0;
     if (nf >= 2) {
         int64_t total_px = 0, run = 0;

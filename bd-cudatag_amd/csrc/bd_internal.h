@@ -76,7 +76,7 @@ inline int band_runs_for_width(int width) {
 constexpr int kRowExtLog2Min = 6, kRowExtLog2Max = 8;
 // Pixels the first of two lanes holds beyond the second (about 20 us of the
 // device's batch throughput, the head start the first lane's graph has).
-constexpr int64_t kLaneLeadPx = 2 * 1280 * 800;
+constexpr int64_t kLaneLeadPx = 4 << 20;
 constexpr uint32_t kRowTableFull = 0xffffffffu;
 inline int row_ext_log2_for_width(int width) {
     int lg = kRowExtLog2Min;

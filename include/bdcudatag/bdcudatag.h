@@ -254,7 +254,7 @@ typedef struct bd_tune {
                             device's CUs hold at once) */
     int32_t head_ranks;  /* each frame's largest candidates scheduled first, 0..4 (default 2) */
     int32_t streams;     /* 2: a batch of two or more frames runs as two concurrent parts (the
-                            first about 2 Mpx larger, at most 1/16 of the batch, as its graph
+                            first about 4 Mpx larger, at most 1/16 of the batch, as its graph
                             starts first), on two streams; 1: as one launch sequence
                             (default 2; 0 in bd_batch_set_tune selects the default; a one-frame
                             batch reports 1) */
