@@ -298,6 +298,8 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         if (error) *error = hip_error("hipHostMalloc", st);
         return false;
     }
+    // Pass numbers start at 0 on both sides (k_finalize publishes 1 first).
+    std::memset(h_scal_, 0, static_cast<size_t>(nf) * S_COUNT * 4);
 
     desc_.assign(nf, FrameDesc{});
     std::vector<int32_t> prefix(static_cast<size_t>(SH_COUNT) * (nf + 1), 0);
@@ -412,6 +414,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         return false;
     }
     stats_.assign(nf, FrameStats{});
+    seq_before_.assign(nf, 0u);
 
     capture_all();
     return true;
@@ -577,9 +580,20 @@ int Engine::detect(const bd_frame* frames, bool device, bd_detection* out, int o
             }
         }
     }
-    // Two lanes end on their own streams and the host waits for both (a
-    // cross-stream join before the wait cost ~20 us per call); nothing is
-    // queued on either stream before both have ended.
+    // The previous call's events are re-recorded below.
+    const hipError_t prev = resolve_timing();
+    if (prev != hipSuccess) {
+        if (error) *error = hip_error("hipEventSynchronize", prev);
+        return -1;
+    }
+    for (int f = 0; f < nf; ++f)
+        seq_before_[f] = __atomic_load_n(h_scal_ + static_cast<size_t>(f) * S_COUNT + S_COUNTS + kHostSeq,
+                                         __ATOMIC_ACQUIRE);
+    // Two lanes end on their own streams (a cross-stream join cost ~20 us
+    // per call).  The host returns once every frame's k_finalize has
+    // published its pass number into pinned memory (the events' completion
+    // reaches the host ~10 us after the last kernel); nothing is queued on
+    // either stream before both lanes' kernels have ended.
     const bool split_wait = two_lanes();
     hipError_t st = hipEventRecord(ev_start_, stream_);
     if (st == hipSuccess) st = launch(device, !split_wait);
@@ -589,18 +603,9 @@ int Engine::detect(const bd_frame* frames, bool device, bd_detection* out, int o
         if (error) *error = hip_error("pipeline launch", st);
         return -1;
     }
-    hipError_t sync = hipEventSynchronize(ev_stop_);
-    if (sync == hipSuccess && split_wait) sync = hipEventSynchronize(ev_stop2_);
-    if (sync != hipSuccess) {
-        if (error) *error = hip_error("hipEventSynchronize", sync);
-        return -1;
-    }
-    float gpu_ms = 0.0f;
-    (void)hipEventElapsedTime(&gpu_ms, ev_start_, ev_stop_);
-    if (split_wait) {
-        float lane2_ms = 0.0f;
-        if (hipEventElapsedTime(&lane2_ms, ev_start_, ev_stop2_) == hipSuccess) gpu_ms = std::max(gpu_ms, lane2_ms);
-    }
+    if (!wait_passes(split_wait, error)) return -1;
+    timing_pending_ = true;
+    timing_split_ = split_wait;
     const auto end = std::chrono::steady_clock::now();
     const float total_ms = std::chrono::duration<float, std::milli>(end - start).count();
 
@@ -622,16 +627,63 @@ int Engine::detect(const bd_frame* frames, bool device, bd_detection* out, int o
         s.detections = found;
         s.quads = static_cast<int>(c[3]);
         s.roi_rows = std::min(static_cast<int>(c[4]) * kTile, h_[f]);
-        s.last_gpu_ms = gpu_ms;
         s.last_total_ms = total_ms;
-        s.average_gpu_ms = s.frames == 0 ? gpu_ms : 0.9f * s.average_gpu_ms + 0.1f * gpu_ms;
         ++s.frames;
     }
     return written_total;
 }
 
+bool Engine::wait_passes(bool split, std::string* error) {
+    const int nf = frame_count();
+    const auto seq_of = [this](int f) {
+        return __atomic_load_n(h_scal_ + static_cast<size_t>(f) * S_COUNT + S_COUNTS + kHostSeq, __ATOMIC_ACQUIRE);
+    };
+    int f = 0;
+    for (uint32_t spin = 1;; ++spin) {
+        while (f < nf && seq_of(f) != seq_before_[f]) ++f;
+        if (f == nf) return true;
+        // Every so often the streams are asked too: a pipeline that failed
+        // never publishes, and its error surfaces here.
+        if ((spin & 1023u) == 0) {
+            hipError_t q = hipEventQuery(ev_stop_);
+            if (q == hipSuccess && split) q = hipEventQuery(ev_stop2_);
+            if (q == hipSuccess) {
+                while (f < nf && seq_of(f) != seq_before_[f]) ++f;
+                if (f == nf) return true;
+                if (error) *error = "pipeline ended without publishing frame " + std::to_string(f);
+                return false;
+            }
+            if (q != hipErrorNotReady) {
+                if (error) *error = hip_error("hipEventQuery", q);
+                return false;
+            }
+        }
+        __builtin_ia32_pause();
+    }
+}
+
+hipError_t Engine::resolve_timing() const {
+    if (!timing_pending_) return hipSuccess;
+    timing_pending_ = false;
+    hipError_t st = hipEventSynchronize(ev_stop_);
+    if (st == hipSuccess && timing_split_) st = hipEventSynchronize(ev_stop2_);
+    if (st != hipSuccess) return st;
+    float gpu_ms = 0.0f;
+    (void)hipEventElapsedTime(&gpu_ms, ev_start_, ev_stop_);
+    if (timing_split_) {
+        float lane2_ms = 0.0f;
+        if (hipEventElapsedTime(&lane2_ms, ev_start_, ev_stop2_) == hipSuccess) gpu_ms = std::max(gpu_ms, lane2_ms);
+    }
+    for (FrameStats& s : stats_) {
+        s.average_gpu_ms = s.frames <= 1 ? gpu_ms : 0.9f * s.average_gpu_ms + 0.1f * gpu_ms;
+        s.last_gpu_ms = gpu_ms;
+    }
+    return hipSuccess;
+}
+
 void Engine::stats(int index, bd_stats* out) const {
     const std::lock_guard<std::mutex> lock(mutex_);
+    (void)resolve_timing();
     const FrameStats& s = stats_[index];
     out->frames = s.frames;
     out->last_gpu_ms = s.last_gpu_ms;
@@ -660,6 +712,7 @@ bool Engine::frame_info(int index, int32_t out[8]) const {
 
 int64_t Engine::debug_copy(int index, int stage, void* dst, int64_t capacity, std::string* error) {
     const std::lock_guard<std::mutex> lock(mutex_);
+    (void)resolve_timing();
     if (index < 0 || index >= frame_count()) {
         if (error) *error = "frame index out of range";
         return -1;
@@ -710,6 +763,7 @@ bool Engine::launch_async(const uint8_t* const* gray, const int32_t* stride, int
         if (error) *error = "invalid frame count";
         return false;
     }
+    (void)resolve_timing();
     for (int f = 0; f < nf; ++f) {
         h_desc_dev_[f].active = f < n ? 1 : 0;
         if (f < n) {
@@ -783,6 +837,7 @@ bool Engine::launch_resident(std::string* error) {
         if (error) *error = "resident launch not prepared";
         return false;
     }
+    (void)resolve_timing();
     hipError_t st = graph_resident_ ? hipGraphLaunch(graph_resident_, stream_) : enqueue_resident();
     if (st == hipSuccess) st = hipEventRecord(ev_stop_, stream_);
     if (st != hipSuccess) {
@@ -807,6 +862,7 @@ bool Engine::time_stages(int reps, float* ms, std::string* error) {
             }
         }
     }
+    (void)resolve_timing();
     for (int i = 0; i <= kStageCount; ++i) ms[i] = 0.0f;
     for (int r = 0; r < reps; ++r) {
         for (int stage = 0; stage < kStageCount; ++stage) {
@@ -846,6 +902,7 @@ bool Engine::replay_locked(int reps, bool device_path, float* ms, std::string* e
         if (error) *error = "invalid repetition count";
         return false;
     }
+    (void)resolve_timing();
     hipError_t st = hipEventRecord(ev_start_, stream_);
     for (int r = 0; r < reps && st == hipSuccess; ++r) st = launch(device_path);
     if (st == hipSuccess) st = hipEventRecord(ev_stop_, stream_);

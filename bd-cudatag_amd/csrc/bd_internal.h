@@ -96,6 +96,7 @@ enum Scalar : int {
     S_NCAND = 10,     // min(candidates, max_candidates)
     S_MAXRUNS = 11,   // maximal black runs in the frame
     S_DECODED = 12,   // candidates that decoded (before de-duplication)
+    S_SEQ = 13,       // pipeline passes completed (k_finalize), never cleared
     S_COUNT = 16
 };
 
@@ -147,8 +148,12 @@ struct FrameDesc {
     float* hull_quads;    // max_candidates * 8
     bd_detection* dec;    // max_candidates, per-candidate decode result
     bd_detection* out;    // max_detections, final detections (pinned host memory)
-    uint32_t* host_counts;  // the five counts from S_COUNTS (pinned host memory), or null
+    // Pinned host memory, or null: the five counts from S_COUNTS, then at
+    // kHostSeq the frame's completed-pass number, stored last with system
+    // scope release (the host polls it instead of waiting on an event).
+    uint32_t* host_counts;
 };
+constexpr int kHostSeq = 5;
 
 // Prefix tables over frames: prefix[k][f] = first block of frame f for
 // kernel shape k; prefix[k][nframes] = total blocks.

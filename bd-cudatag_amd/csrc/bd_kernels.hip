@@ -2839,8 +2839,18 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
     }
     if (t == 0) d->scal[S_COUNTS + 1] = static_cast<uint32_t>(kept);
     // The counts for the host (every other count is final: its kernels ran).
-    if (t < 5 && d->host_counts)
-        d->host_counts[t] = t == 1 ? static_cast<uint32_t>(kept) : d->scal[S_COUNTS + t];
+    if (!d->host_counts) return;
+    if (t < 5) d->host_counts[t] = t == 1 ? static_cast<uint32_t>(kept) : d->scal[S_COUNTS + t];
+    // Then the pass number, after every wave's records and counts: each wave
+    // waits for its stores and writes them back (system-scope release), the
+    // barrier orders those before the one flag store.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    __syncthreads();
+    if (t == 0) {
+        const uint32_t seq = d->scal[S_SEQ] + 1u;
+        d->scal[S_SEQ] = seq;
+        __hip_atomic_store(&d->host_counts[kHostSeq], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 // ---------------------------------------------------------------------------
