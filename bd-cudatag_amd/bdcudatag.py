@@ -25,7 +25,7 @@ STAGES = {
     "bits": 6, "row_count": 7, "runs": 8, "labels": 9, "area": 10, "bbox": 11,
     "candidates": 12, "quads": 13, "hull_quads": 14, "decode": 15, "tile_thresh": 16,
 }
-KERNELS = ["tile_stats", "frame_ref", "binarize_runs", "merge", "stats", "collect",
+KERNELS = ["tile_stats", "binarize_runs", "merge", "stats", "collect",
            "select", "fit", "decode", "finalize"]
 
 

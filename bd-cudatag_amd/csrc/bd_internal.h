@@ -253,9 +253,9 @@ struct ResidentArgs {
     uint32_t max_polls;        // and at most this many polls of the ring
 };
 
-// Kernels in pipeline order: tile_stats, frame_ref, binarize_runs, merge,
-// stats, collect, select, fit, decode, finalize.
-constexpr int kStageCount = 10;
+// Kernels in pipeline order: tile_stats, binarize_runs, merge, stats,
+// collect, select, fit, decode, finalize.
+constexpr int kStageCount = 9;
 
 // Default launch parameters: persistent grids of one full round of the
 // workgroups the device's CUs hold for the compiled k_fit and k_decode.

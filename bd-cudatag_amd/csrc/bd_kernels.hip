@@ -1,10 +1,10 @@
 // HIP kernels of the detection pipeline for gfx950 (CDNA4, wave64).
 //
-// Ten kernels, each with a grid that spans every frame of the plan:
+// Nine kernels, each with a grid that spans every frame of the plan:
 //   k_tile_stats      8x8 tile min/max/mean, histograms, active extents   (a)
-//   k_frame_ref       quartile span + Otsu split, row ROI                 (a)
-//   k_binarize_runs   pooled threshold, 64-px bit packing, black runs per
-//                     row, band-local union-find in LDS                    (b)
+//   k_binarize_runs   quartile span + Otsu split and the band's ROI row
+//                     (a), pooled threshold, 64-px bit packing, black runs
+//                     per row, band-local union-find in LDS               (b)
 //   k_merge           8-connected union across band boundaries            (b)
 //   k_stats           band roots resolved to component roots, root labels,
 //                     component area and bounding box, per-row component
@@ -199,12 +199,25 @@ __host__ __device__ inline int tile_rows_per_wg(int tile_w) {
 // (k_tile_stats, k_tile_histograms) plus the per-row active extents that
 // k_build_row_roi (:282-357) needs.
 __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
+    // The batch's candidate list starts empty (block 0 exists in every
+    // launch, whichever frames are active).
+    if (blockIdx.x == 0 && threadIdx.x < kWorkItems) plan.work[threadIdx.x] = 0;
     int blk;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_TSTAT, blockIdx.x, &blk);
     if (!d->active) return;
     const int W = d->width, H = d->height, stride = d->stride, tw = d->tile_w, th = d->tile_h;
     const uint8_t* gray = d->gray;
     const int t = threadIdx.x;
+    // The counts later kernels accumulate start at zero (the histograms and
+    // the active-row extremes this kernel accumulates were cleared by the
+    // previous pass's k_finalize, or at creation).
+    if (blk == 0 && t < 4) d->scal[S_COUNTS + t] = 0;
+    if (blk == 0 && t == 4) {
+        d->scal[S_CAND_ACC] = 0;
+        d->scal[S_NCAND] = 0;
+        d->scal[S_MAXRUNS] = 0;
+        d->scal[S_DECODED] = 0;
+    }
     const int pairs = (tw + 1) >> 1;
     const int R = tile_rows_per_wg(tw);
     const int ty0 = blk * R;
@@ -353,45 +366,59 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
     }
 }
 
-// One workgroup per frame: the noise-floor quartile and the Otsu split
-// (reference gpu_detector.cu:174-230), the ROI rows (:282-357), and the
-// reset of the accumulators the next frame will use.
-__global__ __launch_bounds__(256) void k_frame_ref(Plan plan) {
-    // The batch's candidate list starts empty (block 0 exists in every
-    // launch, whichever frames are active).
-    if (blockIdx.x == 0 && threadIdx.x < kWorkItems) plan.work[threadIdx.x] = 0;
-    const FrameDesc* d = plan.frames + blockIdx.x;
-    if (!d->active) return;
-    const int t = threadIdx.x;
-    const int tw = d->tile_w, th = d->tile_h;
-    __shared__ uint32_t span[256];
-    __shared__ uint32_t scan[256];
-    __shared__ uint32_t mean[64];
-    __shared__ int s_top, s_bottom;
-    __shared__ uint32_t s_ref0;
-    __shared__ int s_otsu;
+// The frame's references and active tile rows, for one workgroup of
+// k_binarize_runs (every band workgroup derives them from the 1.3 KB of
+// histograms k_tile_stats left, instead of a one-workgroup kernel between
+// the two): the noise-floor quartile and the Otsu split (reference
+// gpu_detector.cu:174-230) and the first and last active tile rows
+// (:282-300).  Called by one whole wave (lanes 0-63).
+struct FrameRef {
+    int ref0;    // lower-quartile tile span (k_tile_reference)
+    int ref1;    // Otsu split of the tile means, scaled to a pixel level
+    int top;     // first active tile row (-1 when none)
+    int bottom;  // last active tile row (-1 when none)
+};
 
-    span[t] = d->hist[t];
-    if (t < 64) mean[t] = d->hist[256 + t];
-    if (t == 0) {
-        s_top = static_cast<int>(d->scal[S_TOP]);
-        s_bottom = static_cast<int>(d->scal[S_BOTTOM1]) - 1;
-        s_ref0 = 0;
-        s_otsu = 0;
-    }
-    scan[t] = span[t];
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const uint32_t v = t >= off ? scan[t - off] : 0u;
-        __syncthreads();
-        scan[t] += v;
-        __syncthreads();
-    }
-    // First bin whose cumulative count exceeds total/4 (k_tile_reference :184-198).
-    const uint32_t total_span = scan[255];
-    if (total_span > 0) {
-        const uint32_t target = total_span / 4;
-        if (scan[t] > target && (t == 0 || scan[t - 1] <= target)) s_ref0 = static_cast<uint32_t>(t);
+// The inputs, loaded first so that their latency overlaps other loads:
+// four span bins and one mean bin per lane, the active-row extremes.
+struct RefInputs {
+    uint4 s4;
+    uint32_t m;
+    uint32_t top, bottom1;
+};
+
+__device__ inline RefInputs load_ref_inputs(const FrameDesc* d, int lane) {
+    RefInputs r;
+    r.s4 = *reinterpret_cast<const uint4*>(d->hist + 4 * lane);
+    r.m = d->hist[256 + lane];
+    r.top = d->scal[S_TOP];
+    r.bottom1 = d->scal[S_BOTTOM1];
+    return r;
+}
+
+__device__ inline FrameRef frame_reference(const FrameDesc* d, const RefInputs& in, int lane) {
+    FrameRef fr;
+    fr.top = static_cast<int>(in.top);
+    fr.bottom = static_cast<int>(in.bottom1) - 1;
+    // Quartile: the first span bin whose cumulative count exceeds total/4
+    // (:184-198); four bins per lane, a wave scan of their sums.
+    const uint4 s4 = in.s4;
+    const uint32_t m = in.m;
+    uint32_t total;
+    uint32_t cum = wave_excl_scan(s4.x + s4.y + s4.z + s4.w, lane, &total);
+    fr.ref0 = 0;
+    if (total > 0) {
+        const uint32_t target = total / 4;
+        const uint32_t bins[4] = {s4.x, s4.y, s4.z, s4.w};
+        int first = 4;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            cum += bins[j];  // cumulative count through bin 4 * lane + j
+            if (first == 4 && cum > target) first = j;
+        }
+        const uint64_t crossing = __ballot(first < 4);
+        const int src = __ffsll(static_cast<long long>(crossing)) - 1;
+        fr.ref0 = static_cast<int>(wave_lane(static_cast<uint32_t>(4 * lane + first), src));
     }
 
     // Otsu over the 64 mean bins (:200-229).  The reference accumulates the
@@ -399,57 +426,57 @@ __global__ __launch_bounds__(256) void k_frame_ref(Plan plan) {
     // sum is an integer below 2^24 those FP32 sums are exact, so each bin's
     // terms equal the integer prefix sums converted once and the 64 bins can
     // be evaluated in parallel.  Larger frames take the serial path.
-    const uint64_t tiles = static_cast<uint64_t>(tw) * th;
+    int otsu = 0;
+    const uint64_t tiles = static_cast<uint64_t>(d->tile_w) * d->tile_h;
     if (tiles * 63u < (1u << 24)) {
-        if (t < kWave) {
-            const uint32_t m = mean[t];
-            uint32_t tw_total, ts_total;
-            const uint32_t wex = wave_excl_scan(m, t, &tw_total);
-            const uint32_t sex = wave_excl_scan(static_cast<uint32_t>(t) * m, t, &ts_total);
-            const float total = static_cast<float>(tw_total);
-            const float sum = static_cast<float>(ts_total);
-            float between = -1.0f;
-            bool valid = false;
-            if (total > 0.0f) {
-                const float weight_below = static_cast<float>(wex + m);
-                const float weight_above = total - weight_below;
-                if (weight_below > 0.0f && weight_above > 0.0f) {
-                    const float sum_below = static_cast<float>(sex + static_cast<uint32_t>(t) * m);
-                    const float mean_below = sum_below / weight_below;
-                    const float mean_above = (sum - sum_below) / weight_above;
-                    const float delta = mean_below - mean_above;
-                    between = weight_below * weight_above * delta * delta;
-                    valid = true;
-                }
+        uint32_t tw_total, ts_total;
+        const uint32_t wex = wave_excl_scan(m, lane, &tw_total);
+        const uint32_t sex = wave_excl_scan(static_cast<uint32_t>(lane) * m, lane, &ts_total);
+        const float total_w = static_cast<float>(tw_total);
+        const float sum = static_cast<float>(ts_total);
+        float between = -1.0f;
+        bool valid = false;
+        if (total_w > 0.0f) {
+            const float weight_below = static_cast<float>(wex + m);
+            const float weight_above = total_w - weight_below;
+            if (weight_below > 0.0f && weight_above > 0.0f) {
+                const float sum_below = static_cast<float>(sex + static_cast<uint32_t>(lane) * m);
+                const float mean_below = sum_below / weight_below;
+                const float mean_above = (sum - sum_below) / weight_above;
+                const float delta = mean_below - mean_above;
+                between = weight_below * weight_above * delta * delta;
+                valid = true;
             }
-            // First bin with the largest between-class variance.
-            float best = valid ? between : -1.0f;
-            int idx = valid ? t : kWave;
-            for (int off = kWave / 2; off > 0; off >>= 1) {
-                const float ob = __shfl_xor(best, off, kWave);
-                const int oi = __shfl_xor(idx, off, kWave);
-                if (ob > best || (ob == best && oi < idx)) {
-                    best = ob;
-                    idx = oi;
-                }
-            }
-            if (t == 0) s_otsu = idx < kWave ? idx : 0;
         }
-    } else if (t == 0) {
-        float total = 0.0f, sum = 0.0f;
+        // First bin with the largest between-class variance.
+        float best = valid ? between : -1.0f;
+        int idx = valid ? lane : kWave;
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            const float ob = __shfl_xor(best, off, kWave);
+            const int oi = __shfl_xor(idx, off, kWave);
+            if (ob > best || (ob == best && oi < idx)) {
+                best = ob;
+                idx = oi;
+            }
+        }
+        otsu = idx < kWave ? idx : 0;
+    } else {
+        // Every lane runs the serial loop on the same values (uniform).
+        float total_w = 0.0f, sum = 0.0f;
         for (int i = 0; i < 64; ++i) {
-            total += static_cast<float>(mean[i]);
-            sum += static_cast<float>(i) * static_cast<float>(mean[i]);
+            const uint32_t mi = wave_lane(m, i);
+            total_w += static_cast<float>(mi);
+            sum += static_cast<float>(i) * static_cast<float>(mi);
         }
-        int otsu = 0;
-        if (total > 0.0f) {
+        if (total_w > 0.0f) {
             float weight_below = 0.0f, sum_below = 0.0f, best = -1.0f;
             for (int i = 0; i < 64; ++i) {
-                weight_below += static_cast<float>(mean[i]);
+                const uint32_t mi = wave_lane(m, i);
+                weight_below += static_cast<float>(mi);
                 if (weight_below <= 0.0f) continue;
-                const float weight_above = total - weight_below;
+                const float weight_above = total_w - weight_below;
                 if (weight_above <= 0.0f) break;
-                sum_below += static_cast<float>(i) * static_cast<float>(mean[i]);
+                sum_below += static_cast<float>(i) * static_cast<float>(mi);
                 const float mean_below = sum_below / weight_below;
                 const float mean_above = (sum - sum_below) / weight_above;
                 const float delta = mean_below - mean_above;
@@ -460,51 +487,19 @@ __global__ __launch_bounds__(256) void k_frame_ref(Plan plan) {
                 }
             }
         }
-        s_otsu = otsu;
     }
-    __syncthreads();
+    fr.ref1 = otsu * 4 + 2;
+    return fr;
+}
 
-    // ROI rows: every tile row between the first and last active row (+-1) is
-    // searched, between its own active extent +-4 tiles, or whole when it has
-    // none (k_build_row_roi :314-356).
-    const int top = s_top, bottom = s_bottom;
-    for (int ty = t; ty < th; ty += blockDim.x) {
-        int32_t* roi = d->row_roi + 4 * ty;
-        roi[3] = 0;
-        if (bottom < 0 || ty < top - 1 || ty > bottom + 1) {
-            roi[0] = 0;
-            roi[1] = 0;
-            roi[2] = -1;
-            continue;
-        }
-        const int lo = d->trow_lo[ty], hi = d->trow_hi[ty];
-        roi[0] = 1;
-        if (hi < 0) {
-            roi[1] = 0;
-            roi[2] = tw - 1;
-        } else {
-            roi[1] = max(0, lo - 4);
-            roi[2] = min(tw - 1, hi + 4);
-        }
-    }
-    if (t == 0) {
-        uint32_t* s = d->scal;
-        s[S_REF0] = s_ref0;
-        s[S_REF1] = static_cast<uint32_t>(s_otsu * 4 + 2);
-        s[S_COUNTS + 0] = 0;
-        s[S_COUNTS + 1] = 0;
-        s[S_COUNTS + 2] = 0;
-        s[S_COUNTS + 3] = 0;
-        s[S_COUNTS + 4] = bottom < 0 ? 0u : static_cast<uint32_t>(min(bottom + 1, th - 1) - max(top - 1, 0) + 1);
-        s[S_CAND_ACC] = 0;
-        s[S_NCAND] = 0;
-        s[S_MAXRUNS] = 0;
-        s[S_DECODED] = 0;
-        s[S_TOP] = 0xffffffffu;
-        s[S_BOTTOM1] = 0;
-    }
-    d->hist[t] = 0;
-    if (t < 64) d->hist[256 + t] = 0;
+// ROI of tile row ty, whose active extent is [lo, hi] (hi < 0: none):
+// every tile row between the first and last active row (+-1) is searched,
+// between its own active extent +-4 tiles, or whole when it has none
+// (k_build_row_roi :314-356).  Returns active, start, end.
+__device__ inline int3 tile_row_roi(const FrameRef& fr, int ty, int lo, int hi, int tile_w) {
+    if (fr.bottom < 0 || ty < fr.top - 1 || ty > fr.bottom + 1) return make_int3(0, 0, -1);
+    if (hi < 0) return make_int3(1, 0, tile_w - 1);
+    return make_int3(1, max(0, lo - 4), min(tile_w - 1, hi + 4));
 }
 
 // ---------------------------------------------------------------------------
@@ -584,7 +579,8 @@ __device__ inline uint32_t bytes_above(uint32_t x, uint32_t t) {
 //     join all of its rows.
 // Band ty of frame d (every branch below is uniform across the workgroup).
 template <int kChunkWords>
-__device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, int ty) {
+__device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, FrameRef* s_fr, bool ref_here,
+                              bool frame_scalars, int ty) {
     const int W = d->width, H = d->height, tw = d->tile_w, th = d->tile_h;
     const int w64 = d->words64, row_cap = d->row_cap, stride = d->stride;
     const int y0 = ty * kTile;
@@ -594,7 +590,12 @@ __device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, int
     // thresholds, the binary image and the runs and labels inside the band.
     const bool capture = plan.capture != 0;
     int32_t* roi = d->row_roi + 4 * ty;
-    const int row_on = roi[0], rs = roi[1], re = roi[2];
+    // The frame's references (first band of the workgroup, wave 0): their
+    // inputs are loaded here, computed after the staging loads below.
+    const bool ref_wave = ref_here && t < kWave;
+    RefInputs ref_in;
+    if (ref_wave) ref_in = load_ref_inputs(d, t);
+    const int row_lo = d->trow_lo[ty], row_hi = d->trow_hi[ty];
 
     __shared__ uint8_t thr[1024];
     __shared__ uint32_t s_off[kTile + 1];
@@ -620,9 +621,6 @@ __device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, int
 
     // Pooled 3x3 thresholds for this tile row (k_tile_threshold): the three
     // tile rows are staged in LDS (padded with neutral values), then pooled.
-    const int ref0 = static_cast<int>(d->scal[S_REF0]);
-    const int ref1 = static_cast<int>(d->scal[S_REF1]);
-    const int floor_contrast = max(kMinTileContrast, (ref0 * 3) / 2);
     for (int i = t; i < 3 * (tw + 2); i += blockDim.x) {
         const int dy = i / (tw + 2) - 1;
         const int xx = i % (tw + 2) - 1;
@@ -635,7 +633,29 @@ __device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, int
         pool_min[i] = mn;
         pool_max[i] = mx;
     }
+    if (ref_wave) {
+        const FrameRef r = frame_reference(d, ref_in, t);
+        if (t == 0) {
+            *s_fr = r;
+            // The frame's scalars, once (its first workgroup): the references
+            // and the ROI tile-row count (a statistic).
+            if (frame_scalars) {
+                d->scal[S_REF0] = static_cast<uint32_t>(r.ref0);
+                d->scal[S_REF1] = static_cast<uint32_t>(r.ref1);
+                d->scal[S_COUNTS + 4] = r.bottom < 0 ? 0u
+                    : static_cast<uint32_t>(min(r.bottom + 1, th - 1) - max(r.top - 1, 0) + 1);
+            }
+        }
+    }
     __syncthreads();
+    const FrameRef fr = *s_fr;
+    const int3 roi3 = tile_row_roi(fr, ty, row_lo, row_hi, tw);
+    const int row_on = roi3.x, rs = roi3.y, re = roi3.z;
+    // The row's ROI for the later kernels; column 3 (band joined locally)
+    // is set once the band's runs are counted.
+    if (t == 0) *reinterpret_cast<int4*>(roi) = make_int4(row_on, rs, re, 0);
+    const int ref1 = fr.ref1;
+    const int floor_contrast = max(kMinTileContrast, (fr.ref0 * 3) / 2);
     for (int tx = t; tx < tw; tx += blockDim.x) {
         int mn = 255, mx = 0;
         for (int dy = 0; dy < 3; ++dy) {
@@ -1009,12 +1029,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
     const FrameDesc* d =
         plan.frames + frame_of(plan, (kBands == 1 ? SH_CLASS : SH_CLASS_PAIR) + cls, blockIdx.x, &unit);
     if (!d->active) return;
+    __shared__ FrameRef s_fr;
 #pragma unroll 1
     for (int b = 0; b < kBands; ++b) {
         const int ty = unit * kBands + b;
         if (ty >= d->tile_h) break;
         if (b > 0) __syncthreads();  // the previous band's LDS reads are done
-        binarize_band<kChunkWords>(plan, cls, d, ty);
+        binarize_band<kChunkWords>(plan, cls, d, &s_fr, b == 0, unit == 0 && b == 0, ty);
     }
 }
 
@@ -1509,31 +1530,39 @@ __global__ __launch_bounds__(kThreads) void k_stats(Plan plan) {
 // (~area << 32 | root slot): ascending keys are largest first, ties by
 // raster order (reference k_collect_candidates :591-638, which visits every
 // run and keeps the roots).
+// The sort key of component root `idx` when it passes the tag-border
+// filter (reference k_collect_candidates :600-636).
+__device__ inline bool candidate_key(const FrameDesc* d, uint32_t idx, uint64_t* key) {
+    const uint32_t row_cap = static_cast<uint32_t>(d->row_cap);
+    const int W = d->width, H = d->height;
+    const int y = static_cast<int>(idx / row_cap);
+    const int a = static_cast<int>(d->area[idx]);
+    if (a < d->min_cluster) return false;
+    const uint4 bb = *reinterpret_cast<const uint4*>(d->bbox + 4 * static_cast<size_t>(idx));
+    const int x0 = static_cast<int>(bb.x), y0 = y;
+    const int x1 = static_cast<int>(bb.z), y1 = static_cast<int>(bb.w);
+    const int bw = x1 - x0 + 1;
+    const int bh = y1 - y0 + 1;
+    if (bw < d->min_tag || bh < d->min_tag) return false;
+    if (x0 <= 0 || y0 <= 0 || x1 >= W - 1 || y1 >= H - 1) return false;
+    if (bw > 6 * bh || bh > 6 * bw) return false;
+    const float fill = static_cast<float>(a) / static_cast<float>(bw * bh);
+    if (fill < 0.18f || fill > 0.95f) return false;
+    *key = (static_cast<uint64_t>(~static_cast<uint32_t>(a)) << 32) | idx;
+    return true;
+}
+
 __global__ __launch_bounds__(256) void k_collect(Plan plan) {
     int ty;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
     if (!d->active) return;
-    const int W = d->width, H = d->height;
-    const uint32_t row_cap = static_cast<uint32_t>(d->row_cap);
-    const uint32_t base = static_cast<uint32_t>(ty * kTile) * row_cap;
+    const uint32_t base = static_cast<uint32_t>(ty * kTile) * static_cast<uint32_t>(d->row_cap);
     const uint32_t n = d->band_nroots[ty];
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const uint32_t idx = d->roots[base + i];
-        const int y = static_cast<int>(idx / row_cap);
-        const int a = static_cast<int>(d->area[idx]);
-        if (a < d->min_cluster) continue;
-        const uint4 bb = *reinterpret_cast<const uint4*>(d->bbox + 4 * static_cast<size_t>(idx));
-        const int x0 = static_cast<int>(bb.x), y0 = y;
-        const int x1 = static_cast<int>(bb.z), y1 = static_cast<int>(bb.w);
-        const int bw = x1 - x0 + 1;
-        const int bh = y1 - y0 + 1;
-        if (bw < d->min_tag || bh < d->min_tag) continue;
-        if (x0 <= 0 || y0 <= 0 || x1 >= W - 1 || y1 >= H - 1) continue;
-        if (bw > 6 * bh || bh > 6 * bw) continue;
-        const float fill = static_cast<float>(a) / static_cast<float>(bw * bh);
-        if (fill < 0.18f || fill > 0.95f) continue;
+        uint64_t key;
+        if (!candidate_key(d, d->roots[base + i], &key)) continue;
         const uint32_t pos = atomicAdd(&d->scal[S_CAND_ACC], 1u);
-        d->cand_keys[pos] = (static_cast<uint64_t>(~static_cast<uint32_t>(a)) << 32) | idx;
+        d->cand_keys[pos] = key;
     }
 }
 
@@ -1567,9 +1596,9 @@ __global__ __launch_bounds__(256) void k_select(Plan plan) {
     __shared__ uint32_t hist[256];
     __shared__ uint64_t s_prefix, s_mask;
     __shared__ uint32_t s_rank, s_fill;
+    const int t = threadIdx.x;
     const uint32_t n = d->scal[S_CAND_ACC];
     const uint32_t nsel = min(n, static_cast<uint32_t>(d->max_candidates));
-    const int t = threadIdx.x;
 
     uint32_t m = n;
     if (n > static_cast<uint32_t>(kSortCap)) {
@@ -2725,6 +2754,13 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
     const FrameDesc* d = plan.frames + blockIdx.x;
     if (!d->active) return;
     const int t = threadIdx.x;
+    // k_tile_stats' accumulators (histograms, active-row extremes) start
+    // cleared for the frame's next pass; k_binarize_runs has read them.
+    for (int i = t; i < 320; i += blockDim.x) d->hist[i] = 0;
+    if (t == 0) {
+        d->scal[S_TOP] = 0xffffffffu;
+        d->scal[S_BOTTOM1] = 0;
+    }
     const int maxd = d->max_detections;
     const int ncand = static_cast<int>(d->scal[S_NCAND]);
     __shared__ int list[256];
@@ -3021,8 +3057,7 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
     const int nf = plan.nframes;
     switch (stage) {
         case 0: hipLaunchKernelGGL(k_tile_stats, dim3(plan.blocks[SH_TSTAT]), dim3(256), 0, stream, plan); break;
-        case 1: hipLaunchKernelGGL(k_frame_ref, dim3(nf), dim3(256), 0, stream, plan); break;
-        case 2:
+        case 1:
             // One launch per width class present, with the class's band capacity.
             for (int c = 0; c < kWidthClasses; ++c) {
                 const bool pairs = plan.tune.band_rows == 16;
@@ -3040,7 +3075,7 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
                     launch_binarize<1, 1>(plan, c, blocks, lds, stream);
             }
             break;
-        case 3:
+        case 2:
             // One launch per width class present, staging its rows in LDS.
             for (int c = 0; c < kWidthClasses; ++c) {
                 const int blocks = plan.blocks[SH_CLASS + c];
@@ -3049,7 +3084,7 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
                 hipLaunchKernelGGL(k_merge, dim3(blocks), dim3(256), lds, stream, plan, c);
             }
             break;
-        case 4: {
+        case 3: {
             // One launch per width class present, with the class's row tables.
             static_assert(kWidthClasses == 3, "one k_stats instance per width class");
             if (plan.blocks[SH_CLASS + 0])
@@ -3060,16 +3095,16 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
                 hipLaunchKernelGGL((k_stats<kRowExtLog2Min + 2, 512>), dim3(plan.blocks[SH_CLASS + 2]), dim3(512), 0, stream, plan);
             break;
         }
-        case 5: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
-        case 6: hipLaunchKernelGGL(k_select, dim3(nf), dim3(256), 0, stream, plan); break;
-        case 7:
+        case 4: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
+        case 5: hipLaunchKernelGGL(k_select, dim3(nf), dim3(256), 0, stream, plan); break;
+        case 6:
             if (plan.nframes <= 2 && plan.wide_blocks >= 2)
                 hipLaunchKernelGGL(k_fit<1024>, dim3(min(plan.work_cap, plan.wide_blocks / 2)), dim3(1024), 0, stream,
                                    plan);
             else
                 hipLaunchKernelGGL(k_fit<256>, dim3(min(plan.work_cap, plan.tune.fit_grid)), dim3(256), 0, stream, plan);
             break;
-        case 8: {
+        case 7: {
             const int w = plan.tune.decode_waves;
             const int per = w >= 16 ? w / 8 : 1;
             const dim3 grid(min((plan.work_cap + per - 1) / per, plan.tune.decode_grid));
@@ -3078,7 +3113,7 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
             else hipLaunchKernelGGL(k_decode<8>, grid, dim3(512), 0, stream, plan);
             break;
         }
-        case 9: hipLaunchKernelGGL(k_finalize, dim3(nf), dim3(256), 0, stream, plan); break;
+        case 8: hipLaunchKernelGGL(k_finalize, dim3(nf), dim3(256), 0, stream, plan); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

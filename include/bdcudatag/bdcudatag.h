@@ -233,9 +233,9 @@ BD_API bd_batch* bd_detector_batch(bd_detector* detector);
 /*
  * Benchmark hooks on the frames staged by the last host call.
  * bd_batch_time_stages: runs the kernels `reps` times with HIP events between
- * consecutive kernels on the detector's stream; ms[0..9] = mean time per
- * kernel (tile_stats, frame_ref, binarize_runs, merge, stats, collect,
- * select, fit, decode, finalize), ms[10] = mean whole pipeline.
+ * consecutive kernels on the detector's stream; ms[0..8] = mean time per
+ * kernel (tile_stats, binarize_runs, merge, stats, collect, select, fit,
+ * decode, finalize), ms[9] = mean whole pipeline.
  * bd_batch_replay: replays the captured graph `reps` times back to back
  * without host synchronisation and returns the total device time in *ms;
  * device_path = 0 replays the host path (H2D of the staged frames, kernels,
