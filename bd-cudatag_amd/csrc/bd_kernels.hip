@@ -1697,36 +1697,18 @@ __device__ inline uint32_t frame_span(int width, int height, int stride) {
     return static_cast<uint32_t>(height - 1) * static_cast<uint32_t>(stride) + static_cast<uint32_t>(width);
 }
 
-// Pixels o and o + 1 of a frame (low byte first), in two steps so that the
-// loads of many pairs are all in flight before any is used: pair_load
-// issues one 8-byte load, unconditionally, from the 4-byte-aligned position
-// at or below pixel o clamped into the frame (frames span at least 256
-// bytes); pair_value extracts the pair from it or, where the clamp moved the
-// load (the frame's first or last few bytes), reads the two bytes instead.
-// Positions count from the aligned address base - mis.
-__device__ inline uint32_t pair_aligned(const gbyte* image, uint32_t o, uint32_t span, uint32_t* clamped) {
+// Pixels o and o + 1 of a frame (low byte first): one 8-byte load from the
+// 4-byte-aligned address at or below pixel o (one request for the pair
+// instead of two), or two byte loads where that load would start before the
+// frame's first byte or end past its last (`span`).
+__device__ inline uint32_t pixel_pair(const gbyte* image, uint32_t o, uint32_t span) {
     const uint32_t mis = static_cast<uint32_t>((uintptr_t)image) & 3u;
-    const uint32_t a = (o + mis) & ~3u;
-    const uint32_t lo = (mis + 3u) & ~3u;           // first aligned position inside the frame
-    const uint32_t hi = (span + mis - 8u) & ~3u;    // last one whose 8 bytes end inside it
-    *clamped = min(max(a, lo), hi);
-    return a;
-}
-
-__device__ inline uint64_t pair_load(const gbyte* image, uint32_t o, uint32_t span) {
-    const uint32_t mis = static_cast<uint32_t>((uintptr_t)image) & 3u;
-    uint32_t ac;
-    (void)pair_aligned(image, o, span, &ac);
-    return *reinterpret_cast<const gword64*>(image + (ac - mis));
-}
-
-__device__ inline uint32_t pair_value(const gbyte* image, uint32_t o, uint32_t span, uint64_t w) {
-    const uint32_t mis = static_cast<uint32_t>((uintptr_t)image) & 3u;
-    uint32_t ac;
-    const uint32_t a = pair_aligned(image, o, span, &ac);
-    if (ac == a)
+    const uint32_t a = (o + mis) & ~3u;  // aligned position, counted from base - mis
+    if (a >= mis && a - mis + 8u <= span) {
+        const uint64_t w = *reinterpret_cast<const gword64*>(image + (a - mis));
         return __builtin_amdgcn_alignbyte(static_cast<uint32_t>(w >> 32), static_cast<uint32_t>(w), (o + mis) & 3u) &
                0xffffu;
+    }
     return static_cast<uint32_t>(image[o]) | (static_cast<uint32_t>(image[o + 1]) << 8);
 }
 
@@ -1742,11 +1724,18 @@ __device__ inline float sample_bilinear(const gbyte* image, int width, int heigh
     const float fy = y - static_cast<float>(iy);
     const uint32_t o = static_cast<uint32_t>(iy) * static_cast<uint32_t>(stride) + static_cast<uint32_t>(ix);
     const uint32_t span = frame_span(width, height, stride);
-    const uint64_t wt = pair_load(image, o, span), wb = pair_load(image, o + stride, span);
-    const uint32_t top = pair_value(image, o, span, wt), bottom = pair_value(image, o + stride, span, wb);
+    const uint32_t top = pixel_pair(image, o, span), bottom = pixel_pair(image, o + stride, span);
     const float a = static_cast<float>(top & 0xffu), b = static_cast<float>(top >> 8);
     const float c = static_cast<float>(bottom & 0xffu), e = static_cast<float>(bottom >> 8);
     return (a * (1.0f - fx) + b * fx) * (1.0f - fy) + (c * (1.0f - fx) + e * fx) * fy;
+}
+
+__device__ inline float directional_rise(const gbyte* image, int width, int height, int stride,
+                                         float x, float y, float dx, float dy) {
+    const float ahead = sample_bilinear(image, width, height, stride, x + dx * 0.7f, y + dy * 0.7f);
+    const float behind = sample_bilinear(image, width, height, stride, x - dx * 0.7f, y - dy * 0.7f);
+    if (ahead < 0.0f || behind < 0.0f) return 0.0f;
+    return ahead - behind;
 }
 
 // sample_bilinear split into address and value halves so the loads of
@@ -1772,38 +1761,6 @@ __device__ inline Tap make_tap(int width, int height, int stride, float x, float
 __device__ inline float tap_value(const Tap& t, float a, float b, float c, float e) {
     if (!t.ok) return -1.0f;
     return (a * (1.0f - t.fx) + b * t.fx) * (1.0f - t.fy) + (c * (1.0f - t.fx) + e * t.fx) * t.fy;
-}
-
-// A tap's 2x2 neighbourhood: its two pixel pairs' loads (in bounds whether
-// or not the tap is inside the frame: an outside tap points at pixel 0).
-struct TapLoad {
-    uint64_t top, bottom;
-};
-
-__device__ inline TapLoad tap_load(const gbyte* image, const Tap& t, int stride, uint32_t span) {
-    return TapLoad{pair_load(image, t.off, span), pair_load(image, t.off + static_cast<uint32_t>(stride), span)};
-}
-
-__device__ inline float tap_sample(const gbyte* image, const Tap& t, int stride, uint32_t span, const TapLoad& l) {
-    const uint32_t top = pair_value(image, t.off, span, l.top);
-    const uint32_t bottom = pair_value(image, t.off + static_cast<uint32_t>(stride), span, l.bottom);
-    return tap_value(t, static_cast<float>(top & 0xffu), static_cast<float>(top >> 8),
-                     static_cast<float>(bottom & 0xffu), static_cast<float>(bottom >> 8));
-}
-
-// The rise across an edge at (x, y) along (dx, dy): bilinear samples 0.7 px
-// ahead and behind (both taps' loads issued together), 0 where either falls
-// outside the frame; the same values as sample_bilinear gives.
-__device__ inline float directional_rise(const gbyte* image, int width, int height, int stride,
-                                         float x, float y, float dx, float dy) {
-    const uint32_t span = frame_span(width, height, stride);
-    const Tap ta = make_tap(width, height, stride, x + dx * 0.7f, y + dy * 0.7f);
-    const Tap tb = make_tap(width, height, stride, x - dx * 0.7f, y - dy * 0.7f);
-    const TapLoad la = tap_load(image, ta, stride, span), lb = tap_load(image, tb, stride, span);
-    const float ahead = tap_sample(image, ta, stride, span, la);
-    const float behind = tap_sample(image, tb, stride, span, lb);
-    if (ahead < 0.0f || behind < 0.0f) return 0.0f;
-    return ahead - behind;
 }
 
 __device__ inline float hyp(float x, float y) { return sqrtf(x * x + y * y); }
@@ -2223,19 +2180,15 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
             if (g.ok) {
                 // Every (sample, probe) rise of this edge, one per lane per
                 // round (its four pixel-pair loads issued together); more per
-                // round measured slower or no faster: the scattered loads of
-                // one candidate queue behind each other in its CU's address
-                // unit, and the larger code and registers cost more than the
-                // extra loads in flight gain.
-                constexpr int kRiseUnroll = 1;
-                constexpr int kRiseStride = kSub * kWave;
+                // round measured slower: the larger code and registers cost
+                // more than the extra loads in flight gain.
                 const int total = g.samples * g.probes;
-                for (int it0 = sub * kWave + lane; it0 < total; it0 += kRiseStride * kRiseUnroll) {
-                    Tap ta[kRiseUnroll], tb[kRiseUnroll];
-                    int slot[kRiseUnroll];
+                for (int it0 = sub * kWave + lane; it0 < total; it0 += kSub * kWave) {
+                    Tap ta[1], tb[1];
+                    int slot[1];
 #pragma unroll
-                    for (int u = 0; u < kRiseUnroll; ++u) {
-                        const int it = it0 + u * kRiseStride;
+                    for (int u = 0; u < 1; ++u) {
+                        const int it = it0 + u * kWave;
                         const bool live = it < total;
                         const int s = live ? it / g.probes : 0;
                         const int i = live ? it - s * g.probes : 0;
@@ -2252,19 +2205,33 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
                         tb[u].ok = tb[u].ok && live;
                     }
                     // Each tap's 2x2 neighbourhood as two horizontal pixel
-                    // pairs (one request each), every load issued before any
-                    // value is taken.
-                    TapLoad la[kRiseUnroll], lb[kRiseUnroll];
+                    // pairs (one request each).
+                    uint32_t pa[1][2], pb[1][2];
 #pragma unroll
-                    for (int u = 0; u < kRiseUnroll; ++u) {
-                        la[u] = tap_load(gray, ta[u], stride, span);
-                        lb[u] = tap_load(gray, tb[u], stride, span);
+                    for (int u = 0; u < 1; ++u) {
+                        const uint32_t p = ta[u].off, q = tb[u].off;
+                        pa[u][0] = ta[u].ok ? pixel_pair(gray, p, span) : 0u;
+                        pa[u][1] = ta[u].ok ? pixel_pair(gray, p + stride, span) : 0u;
+                        pb[u][0] = tb[u].ok ? pixel_pair(gray, q, span) : 0u;
+                        pb[u][1] = tb[u].ok ? pixel_pair(gray, q + stride, span) : 0u;
+                    }
+                    float va[1][4], vb[1][4];
+#pragma unroll
+                    for (int u = 0; u < 1; ++u) {
+                        va[u][0] = static_cast<float>(pa[u][0] & 0xffu);
+                        va[u][1] = static_cast<float>(pa[u][0] >> 8);
+                        va[u][2] = static_cast<float>(pa[u][1] & 0xffu);
+                        va[u][3] = static_cast<float>(pa[u][1] >> 8);
+                        vb[u][0] = static_cast<float>(pb[u][0] & 0xffu);
+                        vb[u][1] = static_cast<float>(pb[u][0] >> 8);
+                        vb[u][2] = static_cast<float>(pb[u][1] & 0xffu);
+                        vb[u][3] = static_cast<float>(pb[u][1] >> 8);
                     }
 #pragma unroll
-                    for (int u = 0; u < kRiseUnroll; ++u) {
-                        if (it0 + u * kRiseStride >= total) continue;
-                        const float ahead = tap_sample(gray, ta[u], stride, span, la[u]);
-                        const float behind = tap_sample(gray, tb[u], stride, span, lb[u]);
+                    for (int u = 0; u < 1; ++u) {
+                        if (it0 + u * kWave >= total) continue;
+                        const float ahead = tap_value(ta[u], va[u][0], va[u][1], va[u][2], va[u][3]);
+                        const float behind = tap_value(tb[u], vb[u][0], vb[u][1], vb[u][2], vb[u][3]);
                         (&mags[edge][0][0])[slot[u]] = (ahead < 0.0f || behind < 0.0f) ? 0.0f : ahead - behind;
                     }
                 }
