@@ -312,10 +312,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         d.host_counts = scal_dev + static_cast<size_t>(f) * S_COUNT + S_COUNTS;
         prefix[SH_TILE_ROW * (nf + 1) + f + 1] = prefix[SH_TILE_ROW * (nf + 1) + f] + d.tile_h;
         prefix[SH_FRAME * (nf + 1) + f + 1] = f + 1;
-        const int pairs = (d.tile_w + 1) / 2;
-        const int rows_per_wg = pairs >= 256 ? 1 : 256 / pairs;
-        prefix[SH_TSTAT * (nf + 1) + f + 1] =
-            prefix[SH_TSTAT * (nf + 1) + f] + (d.tile_h + rows_per_wg - 1) / rows_per_wg;
+        prefix[SH_TSTAT * (nf + 1) + f + 1] = prefix[SH_TSTAT * (nf + 1) + f] + tile_stat_blocks(d.tile_w, d.tile_h);
         for (int c = 0; c < kWidthClasses; ++c) {
             const bool mine = d.row_ext_log2 == kRowExtLog2Min + c;
             int32_t* pc = prefix.data() + static_cast<size_t>(SH_CLASS + c) * (nf + 1);
@@ -349,7 +346,12 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     if (st == hipSuccess) st = hipMemcpy(d_codes_, codes, static_cast<size_t>(code_count) * 8, hipMemcpyHostToDevice);
     // Accumulators start cleared; afterwards each consumer kernel clears them.
     for (int f = 0; f < nf && st == hipSuccess; ++f) {
-        st = hipMemset(desc_[f].hist, 0, 320 * 4);
+        // Each tile row's active extent starts empty: (tile_w, -1).
+        const std::vector<int32_t> lo(static_cast<size_t>(desc_[f].tile_h), desc_[f].tile_w);
+        const std::vector<int32_t> hi(static_cast<size_t>(desc_[f].tile_h), -1);
+        st = hipMemcpy(desc_[f].trow_lo, lo.data(), lo.size() * 4, hipMemcpyHostToDevice);
+        if (st == hipSuccess) st = hipMemcpy(desc_[f].trow_hi, hi.data(), hi.size() * 4, hipMemcpyHostToDevice);
+        if (st == hipSuccess) st = hipMemset(desc_[f].hist, 0, 320 * 4);
         if (st == hipSuccess) st = hipMemset(desc_[f].dec, 0xff, static_cast<size_t>(maxc[f]) * sizeof(bd_detection));
     }
     if (st == hipSuccess) {

@@ -84,6 +84,14 @@ inline int row_ext_log2_for_width(int width) {
     return lg;
 }
 
+// k_tile_stats: 16x8-pixel blocks per workgroup (one per thread), numbered
+// across each frame's tile rows.
+constexpr int kTileStatBlocks = 256;
+inline int tile_stat_blocks(int tile_w, int tile_h) {
+    const int pairs = (tile_w + 1) / 2;
+    return (tile_h * pairs + kTileStatBlocks - 1) / kTileStatBlocks;
+}
+
 // Per-frame scalar block (u32 each).  Accumulators are cleared by the kernel
 // that consumes them, so a replayed graph needs no memset nodes.
 enum Scalar : int {

@@ -186,18 +186,16 @@ __device__ inline void acc_word(uint32_t w, uint32_t& mn, uint32_t& mx, uint32_t
     sum = __builtin_amdgcn_sad_u8(w, 0u, sum);
 }
 
-// Tile rows per k_tile_stats workgroup: as many as fit 256 threads at one
-// 16-pixel-wide column pair per thread (shared with the host's grid sizing).
-__host__ __device__ inline int tile_rows_per_wg(int tile_w) {
-    const int pairs = (tile_w + 1) >> 1;
-    return pairs >= 256 ? 1 : 256 / pairs;
-}
-
-// Each thread owns a 16x8 pixel block (two tiles) read as 16-byte rows, so a
-// wave reads 1 KiB contiguous per pixel row; a workgroup covers
-// tile_rows_per_wg() tile rows.  reference gpu_detector.cu:104-166
-// (k_tile_stats, k_tile_histograms) plus the per-row active extents that
-// k_build_row_roi (:282-357) needs.
+// Each thread owns a 16x8 pixel block (two tiles of one tile row) read as
+// eight 16-byte rows issued together, so a wave reads 1 KiB contiguous per
+// pixel row.  A frame's blocks are numbered tile row by tile row and each
+// workgroup takes 256 consecutive ones: every thread has a block but the
+// frame's last workgroup's tail, and the 64-frame 1280x800 batch is 2048
+// workgroups, one round of resident ones.  A tile row may straddle two
+// workgroups, so its active extent is combined by global atomics (k_finalize
+// of the previous pass, or creation, resets it).  reference
+// gpu_detector.cu:104-166 (k_tile_stats, k_tile_histograms) plus the per-row
+// active extents that k_build_row_roi (:282-357) needs.
 __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
     // The batch's candidate list starts empty (block 0 exists in every
     // launch, whichever frames are active).
@@ -209,7 +207,7 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
     const uint8_t* gray = d->gray;
     const int t = threadIdx.x;
     // The counts later kernels accumulate start at zero (the histograms and
-    // the active-row extremes this kernel accumulates were cleared by the
+    // the active extents this kernel accumulates were cleared by the
     // previous pass's k_finalize, or at creation).
     if (blk == 0 && t < 4) d->scal[S_COUNTS + t] = 0;
     if (blk == 0 && t == 4) {
@@ -219,17 +217,22 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
         d->scal[S_DECODED] = 0;
     }
     const int pairs = (tw + 1) >> 1;
-    const int R = tile_rows_per_wg(tw);
-    const int ty0 = blk * R;
-    __shared__ uint32_t hs[256 + 64];
-    __shared__ int lo_s[256], hi_s[256];
-    // The workgroup's tiles (rows ty0 .. ty0 + R - 1, contiguous in the tile
-    // arrays) are staged here and written with coalesced word stores.
-    constexpr int kStage = 1024;
+    const int g0 = blk * kTileStatBlocks;
+    const int g1 = min(g0 + kTileStatBlocks, th * pairs);
+    const int ty0 = g0 / pairs;             // first tile row of the workgroup
+    const int nrows = (g1 - 1) / pairs - ty0 + 1;
+    // Histograms, one copy per wave: neighbouring tiles often share a bin,
+    // and atomics to one LDS address serialise.
+    constexpr int kWaves = kTileStatBlocks / kWave;
+    __shared__ uint32_t hs[kWaves][256 + 64];
+    __shared__ int lo_s[kTileStatBlocks], hi_s[kTileStatBlocks];
+    // The workgroup's tiles (contiguous in the tile arrays) are staged here
+    // and written with coalesced word stores.
+    constexpr int kStage = 2 * kTileStatBlocks;
     __shared__ __align__(4) uint8_t s_min[kStage], s_max[kStage], s_mean[kStage];
     __shared__ int s_top, s_bottom;
-    for (int i = t; i < 320; i += blockDim.x) hs[i] = 0;
-    if (t < R) {
+    for (int i = t; i < kWaves * 320; i += blockDim.x) (&hs[0][0])[i] = 0;
+    if (t < nrows) {
         lo_s[t] = tw;
         hi_s[t] = -1;
     }
@@ -237,108 +240,110 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
         s_top = INT32_MAX;
         s_bottom = -1;
     }
+    const int first_tile = ty0 * tw + 2 * (g0 - ty0 * pairs);
     __syncthreads();
 
-    const int sub = pairs >= 256 ? 0 : t / pairs;
-    const int p0 = pairs >= 256 ? t : t - sub * pairs;
-    const int pstep = pairs >= 256 ? 256 : pairs;
-    const int ty = ty0 + sub;
-    if (sub < R && ty < th) {
+    uint32_t* my_hs = hs[t / kWave];
+    const int g = g0 + t;
+    if (g < g1) {
+        const int ty = g / pairs;
+        const int p = g - ty * pairs;
         const int y0 = ty * kTile;
         const int y1 = min(y0 + kTile, H);
         const int rows = y1 - y0;
         const bool vec = ((reinterpret_cast<uintptr_t>(gray) | static_cast<uintptr_t>(stride)) & 15u) == 0;
-        for (int p = p0; p < pairs; p += pstep) {
-            const int x0 = p * 16;
-            uint32_t mnA = 255, mxA = 0, sumA = 0, mnB = 255, mxB = 0, sumB = 0;
-            const int nA = min(8, W - x0);
-            const int nB = min(8, W - x0 - 8);
-            if (vec && x0 + 16 <= W && rows == kTile) {
-                // A whole 16x8 block: all eight row loads in flight before
-                // any is used.
-                const gbyte* src = frame_bytes(gray) + static_cast<size_t>(y0) * stride + x0;
-                uint4 v[kTile];
+        const int x0 = p * 16;
+        uint32_t mnA = 255, mxA = 0, sumA = 0, mnB = 255, mxB = 0, sumB = 0;
+        const int nA = min(8, W - x0);
+        const int nB = min(8, W - x0 - 8);
+        if (vec && x0 + 16 <= W && rows == kTile) {
+            // A whole 16x8 block: all eight row loads in flight before
+            // any is used.
+            const gbyte* src = frame_bytes(gray) + static_cast<size_t>(y0) * stride + x0;
+            uint4 v[kTile];
 #pragma unroll
-                for (int r = 0; r < kTile; ++r) v[r] = load16(src, static_cast<size_t>(r) * stride);
+            for (int r = 0; r < kTile; ++r) v[r] = load16(src, static_cast<size_t>(r) * stride);
 #pragma unroll
-                for (int r = 0; r < kTile; ++r) {
-                    acc_word(v[r].x, mnA, mxA, sumA);
-                    acc_word(v[r].y, mnA, mxA, sumA);
-                    acc_word(v[r].z, mnB, mxB, sumB);
-                    acc_word(v[r].w, mnB, mxB, sumB);
+            for (int r = 0; r < kTile; ++r) {
+                acc_word(v[r].x, mnA, mxA, sumA);
+                acc_word(v[r].y, mnA, mxA, sumA);
+                acc_word(v[r].z, mnB, mxB, sumB);
+                acc_word(v[r].w, mnB, mxB, sumB);
+            }
+        } else if (vec && x0 + 16 <= W) {
+            for (int y = y0; y < y1; ++y) {
+                const uint4 v = load16(frame_bytes(gray), static_cast<size_t>(y) * stride + x0);
+                acc_word(v.x, mnA, mxA, sumA);
+                acc_word(v.y, mnA, mxA, sumA);
+                acc_word(v.z, mnB, mxB, sumB);
+                acc_word(v.w, mnB, mxB, sumB);
+            }
+        } else {
+            for (int y = y0; y < y1; ++y) {
+                const uint8_t* row = gray + static_cast<size_t>(y) * stride + x0;
+                for (int x = 0; x < nA; ++x) {
+                    const uint32_t v = row[x];
+                    mnA = min(mnA, v);
+                    mxA = max(mxA, v);
+                    sumA += v;
                 }
-            } else if (vec && x0 + 16 <= W) {
-                for (int y = y0; y < y1; ++y) {
-                    const uint4 v = load16(frame_bytes(gray), static_cast<size_t>(y) * stride + x0);
-                    acc_word(v.x, mnA, mxA, sumA);
-                    acc_word(v.y, mnA, mxA, sumA);
-                    acc_word(v.z, mnB, mxB, sumB);
-                    acc_word(v.w, mnB, mxB, sumB);
-                }
-            } else {
-                for (int y = y0; y < y1; ++y) {
-                    const uint8_t* row = gray + static_cast<size_t>(y) * stride + x0;
-                    for (int x = 0; x < nA; ++x) {
-                        const uint32_t v = row[x];
-                        mnA = min(mnA, v);
-                        mxA = max(mxA, v);
-                        sumA += v;
-                    }
-                    for (int x = 0; x < nB; ++x) {
-                        const uint32_t v = row[8 + x];
-                        mnB = min(mnB, v);
-                        mxB = max(mxB, v);
-                        sumB += v;
-                    }
+                for (int x = 0; x < nB; ++x) {
+                    const uint32_t v = row[8 + x];
+                    mnB = min(mnB, v);
+                    mxB = max(mxB, v);
+                    sumB += v;
                 }
             }
-            const int ta = 2 * p;
-            const uint32_t cA = static_cast<uint32_t>(nA * rows);
-            const uint32_t meanA = (sumA + cA / 2) / cA;
-            const int ia = sub * tw + ta;  // staging index (R * tw <= kStage)
-            s_min[ia] = static_cast<uint8_t>(mnA);
-            s_max[ia] = static_cast<uint8_t>(mxA);
-            s_mean[ia] = static_cast<uint8_t>(meanA);
-            atomicAdd(&hs[(mxA - mnA) & 0xffu], 1u);
-            atomicAdd(&hs[256 + (meanA >> 2)], 1u);
-            int lo = tw, hi = -1;
-            if (static_cast<int>(mxA - mnA) >= kMinTileContrast) lo = hi = ta;
-            if (nB > 0) {
-                const uint32_t cB = static_cast<uint32_t>(nB * rows);
-                const uint32_t meanB = (sumB + cB / 2) / cB;
-                s_min[ia + 1] = static_cast<uint8_t>(mnB);
-                s_max[ia + 1] = static_cast<uint8_t>(mxB);
-                s_mean[ia + 1] = static_cast<uint8_t>(meanB);
-                atomicAdd(&hs[(mxB - mnB) & 0xffu], 1u);
-                atomicAdd(&hs[256 + (meanB >> 2)], 1u);
-                if (static_cast<int>(mxB - mnB) >= kMinTileContrast) {
-                    lo = min(lo, ta + 1);
-                    hi = ta + 1;
-                }
+        }
+        const int ta = 2 * p;
+        const uint32_t cA = static_cast<uint32_t>(nA * rows);
+        const uint32_t meanA = (sumA + cA / 2) / cA;
+        const int ia = ty * tw + ta - first_tile;  // staging index (< kStage)
+        s_min[ia] = static_cast<uint8_t>(mnA);
+        s_max[ia] = static_cast<uint8_t>(mxA);
+        s_mean[ia] = static_cast<uint8_t>(meanA);
+        atomicAdd(&my_hs[(mxA - mnA) & 0xffu], 1u);
+        atomicAdd(&my_hs[256 + (meanA >> 2)], 1u);
+        int lo = tw, hi = -1;
+        if (static_cast<int>(mxA - mnA) >= kMinTileContrast) lo = hi = ta;
+        if (nB > 0) {
+            const uint32_t cB = static_cast<uint32_t>(nB * rows);
+            const uint32_t meanB = (sumB + cB / 2) / cB;
+            s_min[ia + 1] = static_cast<uint8_t>(mnB);
+            s_max[ia + 1] = static_cast<uint8_t>(mxB);
+            s_mean[ia + 1] = static_cast<uint8_t>(meanB);
+            atomicAdd(&my_hs[(mxB - mnB) & 0xffu], 1u);
+            atomicAdd(&my_hs[256 + (meanB >> 2)], 1u);
+            if (static_cast<int>(mxB - mnB) >= kMinTileContrast) {
+                lo = min(lo, ta + 1);
+                hi = ta + 1;
             }
-            if (hi >= 0) {
-                atomicMin(&lo_s[sub], lo);
-                atomicMax(&hi_s[sub], hi);
-            }
+        }
+        if (hi >= 0) {
+            atomicMin(&lo_s[ty - ty0], lo);
+            atomicMax(&hi_s[ty - ty0], hi);
         }
     }
     __syncthreads();
     for (int i = t; i < 320; i += blockDim.x) {
-        if (hs[i]) atomicAdd(&d->hist[i], hs[i]);
+        uint32_t n = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) n += hs[w][i];
+        if (n) atomicAdd(&d->hist[i], n);
     }
-    if (t < R && ty0 + t < th) {
+    if (t < nrows && hi_s[t] >= 0) {
         const int row = ty0 + t;
-        d->trow_lo[row] = lo_s[t];
-        d->trow_hi[row] = hi_s[t];
-        if (hi_s[t] >= 0) {
-            atomicMin(&s_top, row);
-            atomicMax(&s_bottom, row);
-        }
+        atomicMin(&d->trow_lo[row], lo_s[t]);
+        atomicMax(&d->trow_hi[row], hi_s[t]);
+        atomicMin(&s_top, row);
+        atomicMax(&s_bottom, row);
     }
     // The staged tiles: bytes [first, first + count) of each tile array, as
     // aligned words with the unaligned ends byte by byte.
-    const int rows_here = min(R, th - ty0);
-    const int first = ty0 * tw, count = rows_here * tw;
+    const int last_ty = (g1 - 1) / pairs;
+    const int last_p = g1 - 1 - last_ty * pairs;
+    const int first = first_tile;
+    const int count = last_ty * tw + min(2 * last_p + 2, tw) - first;
     const int a0 = min((first + 3) & ~3, first + count), a1 = max(a0, (first + count) & ~3);
     for (int i = t; i < (a1 - a0) / 4; i += blockDim.x) {
         const int o = a0 - first + 4 * i;
@@ -2749,9 +2754,14 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
     const FrameDesc* d = plan.frames + blockIdx.x;
     if (!d->active) return;
     const int t = threadIdx.x;
-    // k_tile_stats' accumulators (histograms, active-row extremes) start
-    // cleared for the frame's next pass; k_binarize_runs has read them.
+    // k_tile_stats' accumulators (histograms, active tile rows and each
+    // tile row's active extent) start cleared for the frame's next pass;
+    // k_binarize_runs has read them.
     for (int i = t; i < 320; i += blockDim.x) d->hist[i] = 0;
+    for (int i = t; i < d->tile_h; i += blockDim.x) {
+        d->trow_lo[i] = d->tile_w;
+        d->trow_hi[i] = -1;
+    }
     if (t == 0) {
         d->scal[S_TOP] = 0xffffffffu;
         d->scal[S_BOTTOM1] = 0;

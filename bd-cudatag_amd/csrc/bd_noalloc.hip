@@ -83,6 +83,10 @@ __global__ __launch_bounds__(256) void k_noalloc_setup(SetupArgs a) {
     if (t < kWorkItems) a.work[t] = 0u;
     if (t < S_COUNT) a.desc.scal[t] = t == S_TOP ? 0xffffffffu : 0u;
     for (int i = t; i < 320; i += blockDim.x) a.desc.hist[i] = 0u;
+    for (int i = t; i < a.desc.tile_h; i += blockDim.x) {
+        a.desc.trow_lo[i] = a.desc.tile_w;
+        a.desc.trow_hi[i] = -1;
+    }
     for (int i = t; i < a.desc.max_candidates; i += blockDim.x) a.desc.dec[i].id = -1;
     const uint64_t* src = a.family == 0 ? bd_codes_tag36h11 : bd_codes_tag16h5;
     for (int i = t; i < a.desc.code_count; i += blockDim.x) a.codes[i] = src[i];
@@ -147,9 +151,7 @@ int detect_noalloc(const Params& p, int width, int height, const uint8_t* gray, 
     a.family = p.family;
     // One frame: every prefix table is {0, blocks of the frame}; only the
     // frame's own width class has blocks.
-    const int pairs = (a.desc.tile_w + 1) / 2;
-    const int rows_per_wg = pairs >= 256 ? 1 : 256 / pairs;
-    int32_t blocks[SH_COUNT] = {a.desc.tile_h, 1, (a.desc.tile_h + rows_per_wg - 1) / rows_per_wg};
+    int32_t blocks[SH_COUNT] = {a.desc.tile_h, 1, tile_stat_blocks(a.desc.tile_w, a.desc.tile_h)};
     const int cls = row_ext_log2 - kRowExtLog2Min;
     for (int c = 0; c < kWidthClasses; ++c) {
         blocks[SH_CLASS + c] = c == cls ? a.desc.tile_h : 0;
