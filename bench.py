@@ -423,8 +423,10 @@ def main():
                         "inputs resident in HBM, one bd_batch_detect_device call per step",
             "batch": B, "width": W, "height": H, "family": "tag36h11",
             "launch": ("two captured hipGraphs per step, one per part of the batch (34/30 frames), "
-                       "on two streams; the host waits for each stream's end event" if tune["streams"] == 2 else
-                       "one captured hipGraph per step on one stream"),
+                       "on two streams; the host polls each frame's pass number, published in pinned "
+                       "memory by the last kernel" if tune["streams"] == 2 else
+                       "one captured hipGraph per step on one stream; the host polls each frame's pass "
+                       "number, published in pinned memory by the last kernel"),
         },
         "single_frame": single,
         "batched_vs_single": round(batched_mpix / single["mpix_s"], 2) if single else None,
