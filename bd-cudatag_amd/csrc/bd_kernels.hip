@@ -1075,14 +1075,16 @@ __device__ inline void uf_union(uint32_t* label, uint32_t a, uint32_t b) {
 // Plan::class_merge_stage runs per row (rows with more runs are read from
 // global memory).
 constexpr int kPairSlots = 1024;
+constexpr int kMergeStageNarrow = 256;
 
-// Records the (smaller, larger) label pair in the LDS hash set, or joins
-// the pair directly when 16 probes find no room.
+// Records the (smaller, larger) label pair in the LDS hash set of kSlots
+// entries, or joins the pair directly when 16 probes find no room.
+template <int kSlots>
 __device__ inline void merge_pair(unsigned long long* pairs, uint32_t* label, uint32_t a, uint32_t b) {
     const unsigned long long key = (static_cast<unsigned long long>(min(a, b)) << 32) | max(a, b);
     const uint32_t h = static_cast<uint32_t>((key * 0x9e3779b97f4a7c15ull) >> 54);
     for (int probe = 0; probe < 16; ++probe) {
-        const int e = static_cast<int>((h + static_cast<uint32_t>(probe)) & (kPairSlots - 1));
+        const int e = static_cast<int>((h + static_cast<uint32_t>(probe)) & (kSlots - 1));
         unsigned long long old = pairs[e];
         if (old == ~0ull) old = atomicCAS(&pairs[e], ~0ull, key);
         if (old == ~0ull || old == key) return;
@@ -1094,7 +1096,7 @@ __device__ inline void merge_pair(unsigned long long* pairs, uint32_t* label, ui
 // the row above (prev, prev_lab, pn runs): 8-connectivity, runs touching
 // [x0 - 1, x1 + 1] (reference k_merge_runs :509-557).  The arrays are LDS
 // copies or the global rows.
-template <class RunPtr, class LabPtr>
+template <int kSlots, class RunPtr, class LabPtr>
 __device__ inline void merge_run(unsigned long long* pairs, uint32_t* label, RunPtr cur, LabPtr cur_lab,
                                  RunPtr prev, LabPtr prev_lab, int pn, int i) {
     const uint32_t run = cur[i];
@@ -1117,7 +1119,7 @@ __device__ inline void merge_run(unsigned long long* pairs, uint32_t* label, Run
         const uint32_t b = prev_lab[k];
         if (b == a || b == last_b) continue;
         last_b = b;
-        merge_pair(pairs, label, a, b);
+        merge_pair<kSlots>(pairs, label, a, b);
     }
 }
 
@@ -1133,7 +1135,14 @@ __device__ inline void merge_run(unsigned long long* pairs, uint32_t* label, Run
 // runs' roots, so the pairs join the same components whenever they are
 // read.  Runs of one row never need same-row joins: they are maximal, or in
 // row 0 they are fragments the reference leaves unjoined.
-__global__ __launch_bounds__(256) void k_merge(Plan plan, int cls) {
+// kThreads 128 with a 512-entry pair table and rows of up to kMergeStageNarrow
+// runs staged serves the narrow width class in launches over more than two
+// frames: 16 resident workgroups per CU instead of 8 (a band's ~140 runs per
+// row at about one per thread), so the 64-frame 1280x800 batch needs fewer
+// rounds of this latency-bound chain (52 -> 38 us); a single frame's bands
+// all fit in one round of the 256-thread form.
+template <int kThreads, int kSlots>
+__global__ __launch_bounds__(kThreads) void k_merge(Plan plan, int cls) {
     int ty;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_CLASS + cls, blockIdx.x, &ty);
     if (!d->active) return;
@@ -1147,14 +1156,15 @@ __global__ __launch_bounds__(256) void k_merge(Plan plan, int cls) {
     const int last_row = local ? 1 : nrows;  // exclusive, band-relative
     if (first_row >= last_row) return;
     const int t = threadIdx.x;
-    __shared__ unsigned long long pairs[kPairSlots];
+    __shared__ unsigned long long pairs[kSlots];
     extern __shared__ uint32_t merge_stage[];
-    const int stage_cap = plan.class_merge_stage[cls];
+    const int stage_cap = kThreads == 256 ? plan.class_merge_stage[cls]
+                                          : min(plan.class_merge_stage[cls], kMergeStageNarrow);
     uint32_t* s_cur = merge_stage;
     uint32_t* s_cur_lab = merge_stage + stage_cap;
     uint32_t* s_prev = merge_stage + 2 * stage_cap;
     uint32_t* s_prev_lab = merge_stage + 3 * stage_cap;
-    for (int e = t; e < kPairSlots; e += blockDim.x) pairs[e] = ~0ull;
+    for (int e = t; e < kSlots; e += blockDim.x) pairs[e] = ~0ull;
     const uint32_t row_cap = static_cast<uint32_t>(d->row_cap);
     uint32_t* label = d->label;
     __syncthreads();
@@ -1181,11 +1191,11 @@ __global__ __launch_bounds__(256) void k_merge(Plan plan, int cls) {
             }
             __syncthreads();
             for (int i = t; i < n; i += blockDim.x)
-                merge_run(pairs, label, s_cur, s_cur_lab, s_prev, s_prev_lab, pn, i);
+                merge_run<kSlots>(pairs, label, s_cur, s_cur_lab, s_prev, s_prev_lab, pn, i);
         } else {
             for (int i = t; i < n; i += blockDim.x)
-                merge_run(pairs, label, d->runs + cur_base, label + cur_base, d->runs + prev_base,
-                          label + prev_base, pn, i);
+                merge_run<kSlots>(pairs, label, d->runs + cur_base, label + cur_base, d->runs + prev_base,
+                                  label + prev_base, pn, i);
         }
     }
     __syncthreads();
@@ -1195,9 +1205,9 @@ __global__ __launch_bounds__(256) void k_merge(Plan plan, int cls) {
     __shared__ uint32_t s_npairs;
     if (t == 0) s_npairs = 0;
     __syncthreads();
-    unsigned long long mine[kPairSlots / 256];
+    unsigned long long mine[kSlots / kThreads];
     int nmine = 0;
-    for (int e = t; e < kPairSlots; e += blockDim.x) {
+    for (int e = t; e < kSlots; e += blockDim.x) {
         const unsigned long long key = pairs[e];
         if (key != ~0ull) mine[nmine++] = key;
     }
@@ -3085,8 +3095,13 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
             for (int c = 0; c < kWidthClasses; ++c) {
                 const int blocks = plan.blocks[SH_CLASS + c];
                 if (blocks == 0) continue;
-                const size_t lds = static_cast<size_t>(plan.class_merge_stage[c]) * 16;
-                hipLaunchKernelGGL(k_merge, dim3(blocks), dim3(256), lds, stream, plan, c);
+                if (c == 0 && nf > 2) {
+                    const size_t lds = static_cast<size_t>(std::min(plan.class_merge_stage[c], kMergeStageNarrow)) * 16;
+                    hipLaunchKernelGGL((k_merge<128, kPairSlots / 2>), dim3(blocks), dim3(128), lds, stream, plan, c);
+                } else {
+                    const size_t lds = static_cast<size_t>(plan.class_merge_stage[c]) * 16;
+                    hipLaunchKernelGGL((k_merge<256, kPairSlots>), dim3(blocks), dim3(256), lds, stream, plan, c);
+                }
             }
             break;
         case 3: {

@@ -107,6 +107,25 @@ def test_batch_of_eight_1280x800():
     assert sum(len(d) for _, _, d in dumps) >= 40
 
 
+def test_batch_of_six_merge_rows_beyond_narrow_staging():
+    # Six frames: each lane holds three, so k_merge runs its 128-thread form
+    # (rows of up to 256 runs staged in LDS).  One frame has 1-px dots on
+    # every band's first row (about 450 runs there) while its bands stay
+    # joined in LDS, so the boundary rows exceed the staging and are read
+    # from global memory.
+    imgs = [scenes.scene(0, 1280, 800, 80 + i)[0] for i in range(5)]
+    dots = imgs[0].copy()
+    for y in range(80, 720, 8):
+        dots[y, 200:1100:2] = 20
+    imgs.append(dots)
+    dumps = check_batch(imgs)
+    g = dumps[-1][0]
+    rc, roi = g["row_count"], g["row_roi"]
+    wide_local = [b for b in range(roi.shape[0]) if roi[b, 0] and roi[b, 3] and int(rc[8 * b]) > 256]
+    assert len(wide_local) > 10, [int(rc[8 * b]) for b in range(roi.shape[0])]
+    assert sum(len(d) for _, _, d in dumps) >= 30
+
+
 def test_stripes_1280x800_band_overflow():
     # 1-px stripes: 450 runs per row, 3600 per band, above the 2048 runs a
     # 1280-px band joins in LDS.
