@@ -1562,7 +1562,9 @@ __device__ inline bool candidate_key(const FrameDesc* d, uint32_t idx, uint64_t*
     return true;
 }
 
-__global__ __launch_bounds__(256) void k_collect(Plan plan) {
+// One wave per band: a band lists a few dozen component roots, and 64-thread
+// workgroups put every band of the 64-frame batch in one round.
+__global__ __launch_bounds__(64) void k_collect(Plan plan) {
     int ty;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
     if (!d->active) return;
@@ -3108,14 +3110,21 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
             // One launch per width class present, with the class's row tables.
             static_assert(kWidthClasses == 3, "one k_stats instance per width class");
             if (plan.blocks[SH_CLASS + 0])
-                hipLaunchKernelGGL((k_stats<kRowExtLog2Min + 0, 256>), dim3(plan.blocks[SH_CLASS + 0]), dim3(256), 0, stream, plan);
+            {
+                // 128 threads for launches over more than two frames: more
+                // resident workgroups of this latency-bound kernel per CU.
+                if (nf > 2)
+                    hipLaunchKernelGGL((k_stats<kRowExtLog2Min + 0, 128>), dim3(plan.blocks[SH_CLASS + 0]), dim3(128), 0, stream, plan);
+                else
+                    hipLaunchKernelGGL((k_stats<kRowExtLog2Min + 0, 256>), dim3(plan.blocks[SH_CLASS + 0]), dim3(256), 0, stream, plan);
+            }
             if (plan.blocks[SH_CLASS + 1])
                 hipLaunchKernelGGL((k_stats<kRowExtLog2Min + 1, 256>), dim3(plan.blocks[SH_CLASS + 1]), dim3(256), 0, stream, plan);
             if (plan.blocks[SH_CLASS + 2])
                 hipLaunchKernelGGL((k_stats<kRowExtLog2Min + 2, 512>), dim3(plan.blocks[SH_CLASS + 2]), dim3(512), 0, stream, plan);
             break;
         }
-        case 4: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_TILE_ROW]), dim3(256), 0, stream, plan); break;
+        case 4: hipLaunchKernelGGL(k_collect, dim3(plan.blocks[SH_TILE_ROW]), dim3(64), 0, stream, plan); break;
         case 5: hipLaunchKernelGGL(k_select, dim3(nf), dim3(256), 0, stream, plan); break;
         case 6:
             if (plan.nframes <= 2 && plan.wide_blocks >= 2)
