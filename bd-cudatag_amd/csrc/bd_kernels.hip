@@ -1659,11 +1659,25 @@ __global__ __launch_bounds__(256) void k_select(Plan plan) {
     } else {
         for (uint32_t i = t; i < n; i += blockDim.x) keys[i] = d->cand_keys[i];
     }
-    int p2 = 1;
-    while (p2 < static_cast<int>(m)) p2 <<= 1;
-    for (int i = static_cast<int>(m) + t; i < p2; i += blockDim.x) keys[i] = ~0ull;
-    __syncthreads();
-    if (m > 1) bitonic_sort_lds(keys, p2);
+    if (m <= blockDim.x) {
+        // A key per thread (the usual case: a frame keeps a few hundred
+        // candidates at most): each key's rank is the number of smaller
+        // keys, all distinct (the low word is the root's slot), so one pass
+        // and one barrier place them.
+        __syncthreads();
+        const uint64_t mine = static_cast<uint32_t>(t) < m ? keys[t] : ~0ull;
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < m; ++j) rank += keys[j] < mine ? 1u : 0u;
+        __syncthreads();
+        if (static_cast<uint32_t>(t) < m) keys[rank] = mine;
+        __syncthreads();
+    } else {
+        int p2 = 1;
+        while (p2 < static_cast<int>(m)) p2 <<= 1;
+        for (int i = static_cast<int>(m) + t; i < p2; i += blockDim.x) keys[i] = ~0ull;
+        __syncthreads();
+        bitonic_sort_lds(keys, p2);
+    }
     // The frame's largest candidates go to the head of the batch list, so
     // that the longest fits of the batch start first: its first head_ranks
     // and every other one of at least head_area pixels (a prefix of the
