@@ -408,6 +408,11 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
     if (st == hipSuccess && lanes_ == 2) st = hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming);
     if (st == hipSuccess && lanes_ == 2) st = hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming);
     if (st == hipSuccess && lanes_ == 2) st = hipEventCreate(&ev_stop2_);
+    // Device wall-clock rate (kHz) for the pass stamps.
+    int dev = 0, khz = 0;
+    if (st == hipSuccess && hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess)
+        clock_khz_ = static_cast<double>(khz);
     if (st == hipSuccess) st = hipEventCreate(&ev_start_);
     if (st == hipSuccess) st = hipEventCreate(&ev_stop_);
     if (st == hipSuccess) st = hipDeviceSynchronize();
@@ -582,34 +587,34 @@ int Engine::detect(const bd_frame* frames, bool device, bd_detection* out, int o
             }
         }
     }
-    // The previous call's events are re-recorded below.
-    const hipError_t prev = resolve_timing();
-    if (prev != hipSuccess) {
-        if (error) *error = hip_error("hipEventSynchronize", prev);
-        return -1;
-    }
     for (int f = 0; f < nf; ++f)
         seq_before_[f] = __atomic_load_n(h_scal_ + static_cast<size_t>(f) * S_COUNT + S_COUNTS + kHostSeq,
                                          __ATOMIC_ACQUIRE);
     // Two lanes end on their own streams (a cross-stream join cost ~20 us
     // per call).  The host returns once every frame's k_finalize has
-    // published its pass number into pinned memory (the events' completion
-    // reaches the host ~10 us after the last kernel); nothing is queued on
-    // either stream before both lanes' kernels have ended.
+    // published its pass number into pinned memory (an event's completion
+    // reaches a waiting host ~10 us after the last kernel); nothing is
+    // queued on either stream before both lanes' kernels have ended.  The
+    // pass is timed by the device clock stamps k_tile_stats and k_finalize
+    // leave beside the counts, so the call records no event.
     const bool split_wait = two_lanes();
-    hipError_t st = hipEventRecord(ev_start_, stream_);
-    if (st == hipSuccess) st = launch(device, !split_wait);
-    if (st == hipSuccess && split_wait) st = hipEventRecord(ev_stop2_, stream2_);
-    if (st == hipSuccess) st = hipEventRecord(ev_stop_, stream_);
+    const hipError_t st = launch(device, !split_wait);
     if (st != hipSuccess) {
         if (error) *error = hip_error("pipeline launch", st);
         return -1;
     }
     if (!wait_passes(split_wait, error)) return -1;
-    timing_pending_ = true;
-    timing_split_ = split_wait;
     const auto end = std::chrono::steady_clock::now();
     const float total_ms = std::chrono::duration<float, std::milli>(end - start).count();
+    uint64_t t_first = ~0ull, t_last = 0;
+    for (int f = 0; f < nf; ++f) {
+        const uint32_t* c = h_scal_ + static_cast<size_t>(f) * S_COUNT + S_COUNTS;
+        t_first = std::min(t_first, *reinterpret_cast<const uint64_t*>(c + kHostStart));
+        t_last = std::max(t_last, *reinterpret_cast<const uint64_t*>(c + kHostEnd));
+    }
+    const float gpu_ms = t_last > t_first && clock_khz_ > 0
+                             ? static_cast<float>(static_cast<double>(t_last - t_first) / clock_khz_)
+                             : 0.0f;
 
     // finish() of gpu_detector.cu:2100-2127, per frame.
     int written_total = 0;
@@ -629,7 +634,9 @@ int Engine::detect(const bd_frame* frames, bool device, bd_detection* out, int o
         s.detections = found;
         s.quads = static_cast<int>(c[3]);
         s.roi_rows = std::min(static_cast<int>(c[4]) * kTile, h_[f]);
+        s.last_gpu_ms = gpu_ms;
         s.last_total_ms = total_ms;
+        s.average_gpu_ms = s.frames == 0 ? gpu_ms : 0.9f * s.average_gpu_ms + 0.1f * gpu_ms;
         ++s.frames;
     }
     return written_total;
@@ -647,8 +654,8 @@ bool Engine::wait_passes(bool split, std::string* error) {
         // Every so often the streams are asked too: a pipeline that failed
         // never publishes, and its error surfaces here.
         if ((spin & 1023u) == 0) {
-            hipError_t q = hipEventQuery(ev_stop_);
-            if (q == hipSuccess && split) q = hipEventQuery(ev_stop2_);
+            hipError_t q = hipStreamQuery(stream_);
+            if (q == hipSuccess && split) q = hipStreamQuery(stream2_);
             if (q == hipSuccess) {
                 while (f < nf && seq_of(f) != seq_before_[f]) ++f;
                 if (f == nf) return true;
@@ -656,7 +663,7 @@ bool Engine::wait_passes(bool split, std::string* error) {
                 return false;
             }
             if (q != hipErrorNotReady) {
-                if (error) *error = hip_error("hipEventQuery", q);
+                if (error) *error = hip_error("hipStreamQuery", q);
                 return false;
             }
         }
@@ -664,28 +671,8 @@ bool Engine::wait_passes(bool split, std::string* error) {
     }
 }
 
-hipError_t Engine::resolve_timing() const {
-    if (!timing_pending_) return hipSuccess;
-    timing_pending_ = false;
-    hipError_t st = hipEventSynchronize(ev_stop_);
-    if (st == hipSuccess && timing_split_) st = hipEventSynchronize(ev_stop2_);
-    if (st != hipSuccess) return st;
-    float gpu_ms = 0.0f;
-    (void)hipEventElapsedTime(&gpu_ms, ev_start_, ev_stop_);
-    if (timing_split_) {
-        float lane2_ms = 0.0f;
-        if (hipEventElapsedTime(&lane2_ms, ev_start_, ev_stop2_) == hipSuccess) gpu_ms = std::max(gpu_ms, lane2_ms);
-    }
-    for (FrameStats& s : stats_) {
-        s.average_gpu_ms = s.frames <= 1 ? gpu_ms : 0.9f * s.average_gpu_ms + 0.1f * gpu_ms;
-        s.last_gpu_ms = gpu_ms;
-    }
-    return hipSuccess;
-}
-
 void Engine::stats(int index, bd_stats* out) const {
     const std::lock_guard<std::mutex> lock(mutex_);
-    (void)resolve_timing();
     const FrameStats& s = stats_[index];
     out->frames = s.frames;
     out->last_gpu_ms = s.last_gpu_ms;
@@ -714,7 +701,6 @@ bool Engine::frame_info(int index, int32_t out[8]) const {
 
 int64_t Engine::debug_copy(int index, int stage, void* dst, int64_t capacity, std::string* error) {
     const std::lock_guard<std::mutex> lock(mutex_);
-    (void)resolve_timing();
     if (index < 0 || index >= frame_count()) {
         if (error) *error = "frame index out of range";
         return -1;
@@ -765,7 +751,6 @@ bool Engine::launch_async(const uint8_t* const* gray, const int32_t* stride, int
         if (error) *error = "invalid frame count";
         return false;
     }
-    (void)resolve_timing();
     for (int f = 0; f < nf; ++f) {
         h_desc_dev_[f].active = f < n ? 1 : 0;
         if (f < n) {
@@ -839,7 +824,6 @@ bool Engine::launch_resident(std::string* error) {
         if (error) *error = "resident launch not prepared";
         return false;
     }
-    (void)resolve_timing();
     hipError_t st = graph_resident_ ? hipGraphLaunch(graph_resident_, stream_) : enqueue_resident();
     if (st == hipSuccess) st = hipEventRecord(ev_stop_, stream_);
     if (st != hipSuccess) {
@@ -864,7 +848,6 @@ bool Engine::time_stages(int reps, float* ms, std::string* error) {
             }
         }
     }
-    (void)resolve_timing();
     for (int i = 0; i <= kStageCount; ++i) ms[i] = 0.0f;
     for (int r = 0; r < reps; ++r) {
         for (int stage = 0; stage < kStageCount; ++stage) {
@@ -904,7 +887,6 @@ bool Engine::replay_locked(int reps, bool device_path, float* ms, std::string* e
         if (error) *error = "invalid repetition count";
         return false;
     }
-    (void)resolve_timing();
     hipError_t st = hipEventRecord(ev_start_, stream_);
     for (int r = 0; r < reps && st == hipSuccess; ++r) st = launch(device_path);
     if (st == hipSuccess) st = hipEventRecord(ev_stop_, stream_);

@@ -152,11 +152,6 @@ private:
     // Waits until every frame's k_finalize has published its pass number
     // (pinned host memory, polled), or reports why it never will.
     bool wait_passes(bool split, std::string* error);
-    // Folds the GPU time of the last synchronous call into the frame
-    // statistics once its events have completed (they are waited for here:
-    // detect() returns on the published pass numbers, before the events'
-    // own completion reaches the host).  Call before re-recording them.
-    hipError_t resolve_timing() const;
     bool set_tune_locked(const Tune& t, std::string* error);
     bool replay_locked(int reps, bool device_path, float* ms, std::string* error);
     void release();
@@ -213,10 +208,9 @@ private:
     hipGraphExec_t graph_resident_ = nullptr;
     hipError_t enqueue_resident();
 
-    mutable std::vector<FrameStats> stats_;
+    std::vector<FrameStats> stats_;
     std::vector<uint32_t> seq_before_;   // per frame: pass number before the current call
-    mutable bool timing_pending_ = false;
-    bool timing_split_ = false;
+    double clock_khz_ = 0.0;             // device wall clock (s_memrealtime) rate
     int async_frames_ = 0;
     mutable std::mutex mutex_;
 };

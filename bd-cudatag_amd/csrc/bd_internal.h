@@ -158,10 +158,14 @@ struct FrameDesc {
     bd_detection* out;    // max_detections, final detections (pinned host memory)
     // Pinned host memory, or null: the five counts from S_COUNTS, then at
     // kHostSeq the frame's completed-pass number, stored last with system
-    // scope release (the host polls it instead of waiting on an event).
+    // scope release (the host polls it instead of waiting on an event); the
+    // device wall clock (s_memrealtime) at the pass's start (k_tile_stats)
+    // and end (k_finalize) as u64 words at kHostStart and kHostEnd.
     uint32_t* host_counts;
 };
 constexpr int kHostSeq = 5;
+constexpr int kHostEnd = 6;    // u64 at host_counts[6..7] (8-byte aligned)
+constexpr int kHostStart = 8;  // u64 at host_counts[8..9]
 
 // Prefix tables over frames: prefix[k][f] = first block of frame f for
 // kernel shape k; prefix[k][nframes] = total blocks.

@@ -210,6 +210,9 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
     // the active extents this kernel accumulates were cleared by the
     // previous pass's k_finalize, or at creation).
     if (blk == 0 && t < 4) d->scal[S_COUNTS + t] = 0;
+    // The pass's start on the device clock, for the host's timing.
+    if (blk == 0 && t == 5 && d->host_counts)
+        *reinterpret_cast<uint64_t*>(d->host_counts + kHostStart) = __builtin_amdgcn_s_memrealtime();
     if (blk == 0 && t == 4) {
         d->scal[S_CAND_ACC] = 0;
         d->scal[S_NCAND] = 0;
@@ -2908,6 +2911,7 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
     // The counts for the host (every other count is final: its kernels ran).
     if (!d->host_counts) return;
     if (t < 5) d->host_counts[t] = t == 1 ? static_cast<uint32_t>(kept) : d->scal[S_COUNTS + t];
+    if (t == 5) *reinterpret_cast<uint64_t*>(d->host_counts + kHostEnd) = __builtin_amdgcn_s_memrealtime();
     // Then the pass number, after every wave's records and counts: each wave
     // waits for its stores and writes them back (system-scope release), the
     // barrier orders those before the one flag store.
