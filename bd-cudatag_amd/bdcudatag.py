@@ -185,6 +185,25 @@ def detection_to_dict(d: Detection) -> dict:
     }
 
 
+# bd_detection as a numpy record (84 bytes, the C layout).
+DETECTION_DTYPE = np.dtype([("id", "<i4"), ("hamming", "<i4"), ("decision_margin", "<f4"),
+                            ("corners", "<f4", (4, 2)), ("reserved", "<i4"), ("homography", "<f4", (9,))])
+assert DETECTION_DTYPE.itemsize == ctypes.sizeof(Detection)
+
+
+def detections_from(buf, first: int, n: int) -> list:
+    """Records first .. first + n - 1 of a ctypes Detection array as dicts,
+    from one numpy copy of those records (the dicts' arrays are views of
+    the copy, so the buffer may be reused at once)."""
+    if n <= 0:
+        return []
+    recs = np.frombuffer(buf, dtype=DETECTION_DTYPE, count=first + n)[first:].copy()
+    ids, hams, margins = recs["id"].tolist(), recs["hamming"].tolist(), recs["decision_margin"].tolist()
+    corners, homs = recs["corners"], recs["homography"]
+    return [{"id": ids[k], "hamming": hams[k], "decision_margin": margins[k],
+             "corners": corners[k], "homography": homs[k]} for k in range(n)]
+
+
 def _stats_dict(s: Stats) -> dict:
     return {k: getattr(s, k) for k, _ in Stats._fields_}
 
@@ -251,8 +270,7 @@ class Batch:
         rc = fn(self.handle, arr, n, out, md, counts)
         if rc < 0:
             raise RuntimeError(f"batch detect failed ({rc}): {last_error()}")
-        return [[detection_to_dict(out[i * md + k]) for k in range(min(counts[i], md))]
-                for i in range(n)]
+        return [detections_from(out, i * md, min(counts[i], md)) for i in range(n)]
 
     def detect(self, images):
         return self._run(images, False)
@@ -406,13 +424,13 @@ class Detector:
                                            self.max_detections)
         if rc < 0:
             raise RuntimeError(f"bd_detector_detect_host failed ({rc}): {last_error()}")
-        return [detection_to_dict(self.out[i]) for i in range(min(rc, self.max_detections))]
+        return detections_from(self.out, 0, min(rc, self.max_detections))
 
     def detect_device(self, ptr: int, stride: int):
         rc = lib().bd_detector_detect_device(self.handle, ptr, stride, self.out, self.max_detections)
         if rc < 0:
             raise RuntimeError(f"bd_detector_detect_device failed ({rc}): {last_error()}")
-        return [detection_to_dict(self.out[i]) for i in range(min(rc, self.max_detections))]
+        return detections_from(self.out, 0, min(rc, self.max_detections))
 
     def stats(self) -> dict:
         s = Stats()
