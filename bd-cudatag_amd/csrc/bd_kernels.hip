@@ -1079,6 +1079,10 @@ __device__ inline void uf_union(uint32_t* label, uint32_t a, uint32_t b) {
 // global memory).
 constexpr int kPairSlots = 1024;
 constexpr int kMergeStageNarrow = 256;
+// Rows staged by the 256-thread form: up to 768 runs (noisy 4096-px rows
+// hold about 500, at most ~620), 20 KB of LDS with the pair table, seven
+// resident workgroups per CU; longer rows are read in place.
+constexpr int kMergeStageWide = 768;
 
 // Records the (smaller, larger) label pair in the LDS hash set of kSlots
 // entries, or joins the pair directly when 16 probes find no room.
@@ -1161,8 +1165,7 @@ __global__ __launch_bounds__(kThreads) void k_merge(Plan plan, int cls) {
     const int t = threadIdx.x;
     __shared__ unsigned long long pairs[kSlots];
     extern __shared__ uint32_t merge_stage[];
-    const int stage_cap = kThreads == 256 ? plan.class_merge_stage[cls]
-                                          : min(plan.class_merge_stage[cls], kMergeStageNarrow);
+    const int stage_cap = min(plan.class_merge_stage[cls], kThreads == 256 ? kMergeStageWide : kMergeStageNarrow);
     uint32_t* s_cur = merge_stage;
     uint32_t* s_cur_lab = merge_stage + stage_cap;
     uint32_t* s_prev = merge_stage + 2 * stage_cap;
@@ -3119,7 +3122,7 @@ hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
                     const size_t lds = static_cast<size_t>(std::min(plan.class_merge_stage[c], kMergeStageNarrow)) * 16;
                     hipLaunchKernelGGL((k_merge<128, kPairSlots / 2>), dim3(blocks), dim3(128), lds, stream, plan, c);
                 } else {
-                    const size_t lds = static_cast<size_t>(plan.class_merge_stage[c]) * 16;
+                    const size_t lds = static_cast<size_t>(std::min(plan.class_merge_stage[c], kMergeStageWide)) * 16;
                     hipLaunchKernelGGL((k_merge<256, kPairSlots>), dim3(blocks), dim3(256), lds, stream, plan, c);
                 }
             }

@@ -29,7 +29,7 @@ from parity import compare_frame, final_outputs, gpu_root_of, gpu_stage_dump, sa
 pytestmark = pytest.mark.gpu
 
 ROOT_FLAG = 0x80000000
-MERGE_STAGE_MAX = 2050  # runs per row k_merge stages in LDS at most (width / 2 + 2 below)
+MERGE_STAGE_MAX = 768   # runs per row k_merge stages in LDS at most (256 in its 128-thread form)
 WALK_CAP = 512         # band roots k_stats walks from its LDS list
 ROOT_SLOTS = 256       # component roots per band in k_stats' LDS table
 
