@@ -2062,6 +2062,8 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
     if (t < kFitDirections) {
         float best = -FLT_MAX;
         uint32_t point = 0xffffffffu;
+        // Unrolled so that the parts' LDS reads are issued together.
+#pragma unroll
         for (int p = 0; p < kParts; ++p) {
             if (pdot[p][t] > best) {
                 best = pdot[p][t];
@@ -2295,6 +2297,7 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
                     // LDS read.
                     float best_mag = 0.0f;
                     int best = -1;
+#pragma unroll 4
                     for (int q = 0; q < g.probes; q += 4) {
                         const float4 v = *reinterpret_cast<const float4*>(mg + q);
                         const float vv[4] = {v.x, v.y, v.z, v.w};
@@ -2317,6 +2320,7 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
                         float valley = FLT_MAX;
                         float right = mg[best - 1], here = mg[best - 2];
                         int nearest = -1;
+#pragma unroll 4
                         for (int i = best - 2; i >= 1; --i) {
                             const float left = mg[i - 1];
                             valley = fminf(valley, right);
