@@ -1153,13 +1153,18 @@ __global__ __launch_bounds__(kThreads) void k_merge(Plan plan, int cls) {
     int ty;
     const FrameDesc* d = plan.frames + frame_of(plan, SH_CLASS + cls, blockIdx.x, &ty);
     if (!d->active) return;
-    const int32_t* roi = d->row_roi + 4 * ty;
-    if (!roi[0]) return;
     const int H = d->height;
     const int y0 = ty * kTile;
     const int nrows = min(kTile, H - y0);
-    const bool local = roi[3] != 0;
     const int first_row = ty == 0 ? 1 : 0;
+    // The band's ROI word and its first row pair's run counts, loaded
+    // together (the counts do not depend on the ROI; frames have at least
+    // two rows).
+    const int4 roi = *reinterpret_cast<const int4*>(d->row_roi + 4 * ty);
+    const int n_first = static_cast<int>(d->row_count[y0 + first_row]);
+    const int pn_first = static_cast<int>(d->row_count[y0 + first_row - 1]);
+    if (!roi.x) return;
+    const bool local = roi.w != 0;
     const int last_row = local ? 1 : nrows;  // exclusive, band-relative
     if (first_row >= last_row) return;
     const int t = threadIdx.x;
@@ -1180,8 +1185,8 @@ __global__ __launch_bounds__(kThreads) void k_merge(Plan plan, int cls) {
     // LDS by coalesced loads when they fit, else read from global memory.
     for (int r = first_row; r < last_row; ++r) {
         const int y = y0 + r;
-        const int n = static_cast<int>(d->row_count[y]);
-        const int pn = static_cast<int>(d->row_count[y - 1]);
+        const int n = r == first_row ? n_first : static_cast<int>(d->row_count[y]);
+        const int pn = r == first_row ? pn_first : static_cast<int>(d->row_count[y - 1]);
         if (n == 0 || pn == 0) continue;
         const uint32_t cur_base = static_cast<uint32_t>(y) * row_cap;
         const uint32_t prev_base = static_cast<uint32_t>(y - 1) * row_cap;
@@ -1734,52 +1739,29 @@ __device__ inline uint32_t frame_span(int width, int height, int stride) {
     return static_cast<uint32_t>(height - 1) * static_cast<uint32_t>(stride) + static_cast<uint32_t>(width);
 }
 
-// Pixels o and o + 1 of a frame (low byte first): one 8-byte load from the
-// 4-byte-aligned address at or below pixel o (one request for the pair
-// instead of two), or two byte loads where that load would start before the
-// frame's first byte or end past its last (`span`).
+// Pixels o and o + 1 of a frame (low byte first) from one 8-byte load with
+// no branch, so that several pairs' loads can be in flight together.  The
+// window is the 4-byte-aligned one at or below pixel o, moved down one word
+// where its second word would start past the frame's last byte: both of its
+// words then hold a frame byte (bytes of those words outside the frame lie
+// in the same page as a frame byte and are discarded), and the pair sits at
+// byte 0-6 of it (frames are at least 16x16, so the moved window still
+// starts at or after the frame's first word).  `span` is the frame's byte
+// extent from its first pixel through its last.
 __device__ inline uint32_t pixel_pair(const gbyte* image, uint32_t o, uint32_t span) {
-    const uint32_t mis = static_cast<uint32_t>((uintptr_t)image) & 3u;
-    const uint32_t a = (o + mis) & ~3u;  // aligned position, counted from base - mis
-    if (a >= mis && a - mis + 8u <= span) {
-        const uint64_t w = *reinterpret_cast<const gword64*>(image + (a - mis));
-        return __builtin_amdgcn_alignbyte(static_cast<uint32_t>(w >> 32), static_cast<uint32_t>(w), (o + mis) & 3u) &
-               0xffffu;
-    }
-    return static_cast<uint32_t>(image[o]) | (static_cast<uint32_t>(image[o + 1]) << 8);
-}
-
-__device__ inline float sample_bilinear(const gbyte* image, int width, int height, int stride,
-                                        float x, float y) {
-    if (!(x >= 0.0f && y >= 0.0f && x < static_cast<float>(width - 1) &&
-          y < static_cast<float>(height - 1))) {
-        return -1.0f;
-    }
-    const int ix = static_cast<int>(x);
-    const int iy = static_cast<int>(y);
-    const float fx = x - static_cast<float>(ix);
-    const float fy = y - static_cast<float>(iy);
-    const uint32_t o = static_cast<uint32_t>(iy) * static_cast<uint32_t>(stride) + static_cast<uint32_t>(ix);
-    const uint32_t span = frame_span(width, height, stride);
-    const uint32_t top = pixel_pair(image, o, span), bottom = pixel_pair(image, o + stride, span);
-    const float a = static_cast<float>(top & 0xffu), b = static_cast<float>(top >> 8);
-    const float c = static_cast<float>(bottom & 0xffu), e = static_cast<float>(bottom >> 8);
-    return (a * (1.0f - fx) + b * fx) * (1.0f - fy) + (c * (1.0f - fx) + e * fx) * fy;
-}
-
-__device__ inline float directional_rise(const gbyte* image, int width, int height, int stride,
-                                         float x, float y, float dx, float dy) {
-    const float ahead = sample_bilinear(image, width, height, stride, x + dx * 0.7f, y + dy * 0.7f);
-    const float behind = sample_bilinear(image, width, height, stride, x - dx * 0.7f, y - dy * 0.7f);
-    if (ahead < 0.0f || behind < 0.0f) return 0.0f;
-    return ahead - behind;
+    const uint32_t mis = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(image)) & 3u;
+    const gbyte* base = image - mis;  // the frame's first word
+    const uint32_t p = o + mis;       // pixel o, counted from base
+    const uint32_t at = min(p & ~3u, ((span - 1u + mis) & ~3u) - 4u);
+    const uint64_t w = *reinterpret_cast<const gword64*>(base + at);
+    return static_cast<uint32_t>(w >> (8u * (p - at))) & 0xffffu;
 }
 
 // sample_bilinear split into address and value halves so the loads of
-// several samples can be in flight at once; tap_value evaluates the same
-// expression as sample_bilinear.
+// several samples can be in flight at once; tap_value evaluates the
+// reference's bilinear expression.
 struct Tap {
-    uint32_t off;  // byte offset of the top-left neighbour
+    uint32_t off;  // byte offset of the top-left neighbour (0 when outside)
     float fx, fy;
     bool ok;
 };
@@ -1795,9 +1777,32 @@ __device__ inline Tap make_tap(int width, int height, int stride, float x, float
     return t;
 }
 
+// The value is computed whether or not the tap is inside and then selected
+// (a branch here would let the compiler sink the tap's loads into it, one
+// round trip per tap).
 __device__ inline float tap_value(const Tap& t, float a, float b, float c, float e) {
-    if (!t.ok) return -1.0f;
-    return (a * (1.0f - t.fx) + b * t.fx) * (1.0f - t.fy) + (c * (1.0f - t.fx) + e * t.fx) * t.fy;
+    const float v = (a * (1.0f - t.fx) + b * t.fx) * (1.0f - t.fy) + (c * (1.0f - t.fx) + e * t.fx) * t.fy;
+    return t.ok ? v : -1.0f;
+}
+
+// reference sample_bilinear: -1 outside [0, W-1) x [0, H-1).  Both pixel
+// pairs are loaded whether or not the point is inside (a point outside
+// loads the frame's first pixels), so the loads carry no branch.
+__device__ inline float sample_bilinear(const gbyte* image, int width, int height, int stride,
+                                        float x, float y) {
+    const Tap tp = make_tap(width, height, stride, x, y);
+    const uint32_t span = frame_span(width, height, stride);
+    const uint32_t top = pixel_pair(image, tp.off, span), bottom = pixel_pair(image, tp.off + stride, span);
+    return tap_value(tp, static_cast<float>(top & 0xffu), static_cast<float>(top >> 8),
+                     static_cast<float>(bottom & 0xffu), static_cast<float>(bottom >> 8));
+}
+
+__device__ inline float directional_rise(const gbyte* image, int width, int height, int stride,
+                                         float x, float y, float dx, float dy) {
+    const float ahead = sample_bilinear(image, width, height, stride, x + dx * 0.7f, y + dy * 0.7f);
+    const float behind = sample_bilinear(image, width, height, stride, x - dx * 0.7f, y - dy * 0.7f);
+    if (ahead < 0.0f || behind < 0.0f) return 0.0f;
+    return ahead - behind;
 }
 
 __device__ inline float hyp(float x, float y) { return sqrtf(x * x + y * y); }
@@ -2248,11 +2253,14 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
                     uint32_t pa[1][2], pb[1][2];
 #pragma unroll
                     for (int u = 0; u < 1; ++u) {
+                        // Loaded whatever the taps' bounds (a tap outside
+                        // the frame reads its first pixels): no branch
+                        // between the four loads.
                         const uint32_t p = ta[u].off, q = tb[u].off;
-                        pa[u][0] = ta[u].ok ? pixel_pair(gray, p, span) : 0u;
-                        pa[u][1] = ta[u].ok ? pixel_pair(gray, p + stride, span) : 0u;
-                        pb[u][0] = tb[u].ok ? pixel_pair(gray, q, span) : 0u;
-                        pb[u][1] = tb[u].ok ? pixel_pair(gray, q + stride, span) : 0u;
+                        pa[u][0] = pixel_pair(gray, p, span);
+                        pa[u][1] = pixel_pair(gray, p + stride, span);
+                        pb[u][0] = pixel_pair(gray, q, span);
+                        pb[u][1] = pixel_pair(gray, q + stride, span);
                     }
                     float va[1][4], vb[1][4];
 #pragma unroll
@@ -2440,7 +2448,7 @@ __device__ inline uint32_t work_item(const Plan& plan, uint32_t item, uint32_t n
 // 256 threads, or for a launch over one or two frames (one round of
 // candidates) 1024, four waves per edge in the refinement's rise loop.
 template <int kThreads>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kThreads == 256 ? 8 : 7))) void k_fit(Plan plan) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kThreads == 256 ? 8 : 4))) void k_fit(Plan plan) {
     const uint32_t nhead = plan.work[kWorkHeadCount];
     const uint32_t total = nhead + plan.work[kWorkCount];
     for (uint32_t item = blockIdx.x; item < total; item = next_item(&plan.work[kWorkFitCursor])) {
