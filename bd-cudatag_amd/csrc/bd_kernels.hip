@@ -1839,48 +1839,69 @@ __device__ inline bool intersect_lines(const float* li, const float* lj, float* 
     return true;
 }
 
-// reference apply_refined_lines (gpu_detector.cu:1021-1076).
-__device__ float apply_refined_lines(float quad[4][2], const float lines[4][4], const int have[4]) {
-    int line_count = 0;
-    for (int i = 0; i < 4; ++i) line_count += have[i] ? 1 : 0;
-    if (line_count < 3) return 0.0f;
-    float refined[4][2];
-    for (int i = 0; i < 4; ++i) {
-        refined[i][0] = quad[i][0];
-        refined[i][1] = quad[i][1];
+__device__ inline float lane_f(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+// reference apply_refined_lines (gpu_detector.cu:1021-1076) on lanes 0-3 of
+// one wave (the whole wave calls it; lane i refines corner (i + 1) & 3 from
+// lines i and i + 1, the reference's loop step i), reading the quad, lines
+// and line flags from LDS and writing the refined quad back when it is
+// accepted.  Every value is the one the reference's loop computes: the
+// extremes are folded in the loop's order and the area terms summed in
+// corner order, from lane reads.  Returns the largest corner move (0 when
+// the refinement is rejected).
+__device__ float apply_refined_lines(float (*quad)[2], const float (*lines)[4], const int* have, int lane) {
+    const int i = lane & 3, j = (i + 1) & 3;
+    const float qix = quad[i][0], qiy = quad[i][1], qjx = quad[j][0], qjy = quad[j][1];
+    float li[4], lj[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        li[k] = lines[i][k];
+        lj[k] = lines[j][k];
     }
+    const bool hi = have[i] != 0, hj = have[j] != 0;
+    if (__popcll(__ballot(lane < 4 && hi)) < 3) return 0.0f;
+    const float side = hyp(qjx - qix, qjy - qiy);
     float extent = 0.0f;
-    for (int i = 0; i < 4; ++i) {
-        const int j = (i + 1) & 3;
-        extent = fmaxf(extent, hyp(quad[j][0] - quad[i][0], quad[j][1] - quad[i][1]));
-    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) extent = fmaxf(extent, lane_f(side, k));
     const float max_shift = fminf(fmaxf(0.15f * extent, 4.0f), 16.0f);
-    float moved = 0.0f;
-    for (int i = 0; i < 4; ++i) {
-        const int j = (i + 1) & 3;
-        if (!have[i] || !have[j]) continue;
-        float x, y;
-        if (!intersect_lines(lines[i], lines[j], &x, &y)) continue;
-        const float dx = x - quad[j][0];
-        const float dy = y - quad[j][1];
-        if (dx * dx + dy * dy > max_shift * max_shift) continue;
-        refined[j][0] = x;
-        refined[j][1] = y;
-        moved = fmaxf(moved, sqrtf(dx * dx + dy * dy));
+    float rx = qjx, ry = qjy, mv = 0.0f;  // corner j, refined or kept
+    float x, y;
+    if (hi && hj && intersect_lines(li, lj, &x, &y)) {
+        const float dx = x - qjx;
+        const float dy = y - qjy;
+        if (!(dx * dx + dy * dy > max_shift * max_shift)) {
+            rx = x;
+            ry = y;
+            mv = sqrtf(dx * dx + dy * dy);
+        }
     }
-    for (int i = 0; i < 4; ++i) {
-        const int j = (i + 1) & 3;
-        if (hyp(refined[j][0] - refined[i][0], refined[j][1] - refined[i][1]) < 3.0f) return 0.0f;
+    // Corner c is refined by lane (c + 3) & 3.
+    float cx[4], cy[4];
+    float moved = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        cx[c] = lane_f(rx, (c + 3) & 3);
+        cy[c] = lane_f(ry, (c + 3) & 3);
+        moved = fmaxf(moved, lane_f(mv, c));
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int d = (c + 1) & 3;
+        if (hyp(cx[d] - cx[c], cy[d] - cy[c]) < 3.0f) return 0.0f;
     }
     float area2 = 0.0f;
-    for (int i = 0; i < 4; ++i) {
-        const int j = (i + 1) & 3;
-        area2 += refined[i][0] * refined[j][1] - refined[j][0] * refined[i][1];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int d = (c + 1) & 3;
+        area2 += cx[c] * cy[d] - cx[d] * cy[c];
     }
     if (area2 <= 0.0f) return 0.0f;
-    for (int i = 0; i < 4; ++i) {
-        quad[i][0] = refined[i][0];
-        quad[i][1] = refined[i][1];
+    if (lane < 4) {
+        quad[j][0] = rx;
+        quad[j][1] = ry;
     }
     return moved;
 }
@@ -2296,27 +2317,47 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
                 int used = 0;
                 // Moments about the edge midpoint (see the oracle's note).
                 const float rx = g.ax + 0.5f * g.ex, ry = g.ay + 0.5f * g.ey;
+                // First strictly largest rise (:934-945): four lanes per
+                // sample (lane = sample + 16 * part), each the first strict
+                // maximum of one quarter of the probes (four per LDS read),
+                // folded into the sample's lane in probe order.
+                float best_mag = 0.0f;
+                int best = -1;
+                {
+                    static_assert(kWave == 4 * kMaxSamples, "four quarters of the probes per sample");
+                    const int ls = lane & (kMaxSamples - 1), part = lane / kMaxSamples;
+                    if (g.ok && ls < g.samples) {
+                        const float* mg = mags[edge][ls];
+                        const int quarter = ((g.probes + 15) >> 4) << 2;
+                        const int q0 = part * quarter, q1 = min(g.probes, q0 + quarter);
+#pragma unroll 4
+                        for (int q = q0; q < q1; q += 4) {
+                            const float4 v = *reinterpret_cast<const float4*>(mg + q);
+                            const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                if (q + c < q1 && vv[c] > best_mag) {
+                                    best_mag = vv[c];
+                                    best = q + c;
+                                }
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int k = 1; k < kWave / kMaxSamples; ++k) {
+                        const float m = __shfl(best_mag, lane + kMaxSamples * k);
+                        const int b = __shfl(best, lane + kMaxSamples * k);
+                        if (m > best_mag) {
+                            best_mag = m;
+                            best = b;
+                        }
+                    }
+                }
                 if (g.ok && lane < g.samples) {
                     const float* mg = mags[edge][lane];
                     const float alpha = static_cast<float>(lane + 1) / static_cast<float>(g.samples + 1);
                     const float px = g.ax + alpha * g.ex;
                     const float py = g.ay + alpha * g.ey;
-                    // First strictly largest rise (:934-945), four probes per
-                    // LDS read.
-                    float best_mag = 0.0f;
-                    int best = -1;
-#pragma unroll 4
-                    for (int q = 0; q < g.probes; q += 4) {
-                        const float4 v = *reinterpret_cast<const float4*>(mg + q);
-                        const float vv[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) {
-                            if (q + c < g.probes && vv[c] > best_mag) {
-                                best_mag = vv[c];
-                                best = q + c;
-                            }
-                        }
-                    }
                     // The nearest-peak rule (:952-966): the first i (in
                     // increasing order) below best - 1 that is a local
                     // maximum of at least 0.7 best_mag with a valley under
@@ -2395,21 +2436,9 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
                 }
             }
             __syncthreads();
-            if (t == 0) {
-                float q[4][2];
-                float lines[4][4];
-                int have[4];
-                for (int i = 0; i < 4; ++i) {
-                    q[i][0] = fit_quad[i][0];
-                    q[i][1] = fit_quad[i][1];
-                    have[i] = edge_have[i];
-                    for (int k = 0; k < 4; ++k) lines[i][k] = edge_lines[i][k];
-                }
-                refine_moved = apply_refined_lines(q, lines, have);
-                for (int i = 0; i < 4; ++i) {
-                    fit_quad[i][0] = q[i][0];
-                    fit_quad[i][1] = q[i][1];
-                }
+            if (wave == 0) {
+                const float moved = apply_refined_lines(fit_quad, edge_lines, edge_have, lane);
+                if (lane == 0) refine_moved = moved;
             }
             __syncthreads();
             if (refine_moved < 0.15f) break;
