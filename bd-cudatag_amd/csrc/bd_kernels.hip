@@ -74,9 +74,16 @@ __device__ inline uint4 load16(const gbyte* image, size_t o) {
 }
 
 // Frame of a block for one grid shape (prefix tables are tiny: one entry
-// per frame).
+// per frame).  The tables do not change while a kernel runs, so they are
+// read as constant memory: scalar loads through the scalar cache for the
+// search's dependent steps.
+typedef const int32_t __attribute__((address_space(4))) kint32;
+// Frame descriptors as the pipeline kernels read them: constant memory too
+// (only the host and the admission kernels, in earlier launches, write
+// them), so the uniform descriptor fields come through scalar loads.
+typedef const FrameDesc __attribute__((address_space(4))) KFrame;
 __device__ inline int frame_of(const Plan& plan, int shape, int block, int* local) {
-    const int32_t* p = plan.prefix + shape * (plan.nframes + 1);
+    const kint32* p = (const kint32*)(plan.prefix + shape * (plan.nframes + 1));
     int lo = 0, hi = plan.nframes - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -201,7 +208,7 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
     // launch, whichever frames are active).
     if (blockIdx.x == 0 && threadIdx.x < kWorkItems) plan.work[threadIdx.x] = 0;
     int blk;
-    const FrameDesc* d = plan.frames + frame_of(plan, SH_TSTAT, blockIdx.x, &blk);
+    const KFrame* d = (const KFrame*)plan.frames + frame_of(plan, SH_TSTAT, blockIdx.x, &blk);
     if (!d->active) return;
     const int W = d->width, H = d->height, stride = d->stride, tw = d->tile_w, th = d->tile_h;
     const uint8_t* gray = d->gray;
@@ -395,7 +402,7 @@ struct RefInputs {
     uint32_t top, bottom1;
 };
 
-__device__ inline RefInputs load_ref_inputs(const FrameDesc* d, int lane) {
+__device__ inline RefInputs load_ref_inputs(const KFrame* d, int lane) {
     RefInputs r;
     r.s4 = *reinterpret_cast<const uint4*>(d->hist + 4 * lane);
     r.m = d->hist[256 + lane];
@@ -404,7 +411,7 @@ __device__ inline RefInputs load_ref_inputs(const FrameDesc* d, int lane) {
     return r;
 }
 
-__device__ inline FrameRef frame_reference(const FrameDesc* d, const RefInputs& in, int lane) {
+__device__ inline FrameRef frame_reference(const KFrame* d, const RefInputs& in, int lane) {
     FrameRef fr;
     fr.top = static_cast<int>(in.top);
     fr.bottom = static_cast<int>(in.bottom1) - 1;
@@ -582,7 +589,7 @@ __device__ inline uint32_t bytes_above(uint32_t x, uint32_t t) {
 //     join all of its rows.
 // Band ty of frame d (every branch below is uniform across the workgroup).
 template <int kChunkWords>
-__device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, FrameRef* s_fr, bool ref_here,
+__device__ void binarize_band(const Plan& plan, int cls, const KFrame* d, FrameRef* s_fr, bool ref_here,
                               bool frame_scalars, int ty) {
     const int W = d->width, H = d->height, tw = d->tile_w, th = d->tile_h;
     const int w64 = d->words64, row_cap = d->row_cap, stride = d->stride;
@@ -1029,8 +1036,8 @@ __device__ void binarize_band(const Plan& plan, int cls, const FrameDesc* d, Fra
 template <int kChunkWords, int kBands, int kThreads>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_binarize_runs(Plan plan, int cls) {
     int unit;
-    const FrameDesc* d =
-        plan.frames + frame_of(plan, (kBands == 1 ? SH_CLASS : SH_CLASS_PAIR) + cls, blockIdx.x, &unit);
+    const KFrame* d =
+        (const KFrame*)plan.frames + frame_of(plan, (kBands == 1 ? SH_CLASS : SH_CLASS_PAIR) + cls, blockIdx.x, &unit);
     if (!d->active) return;
     __shared__ FrameRef s_fr;
 #pragma unroll 1
@@ -1151,7 +1158,7 @@ __device__ inline void merge_run(unsigned long long* pairs, uint32_t* label, Run
 template <int kThreads, int kSlots>
 __global__ __launch_bounds__(kThreads) void k_merge(Plan plan, int cls) {
     int ty;
-    const FrameDesc* d = plan.frames + frame_of(plan, SH_CLASS + cls, blockIdx.x, &ty);
+    const KFrame* d = (const KFrame*)plan.frames + frame_of(plan, SH_CLASS + cls, blockIdx.x, &ty);
     if (!d->active) return;
     const int H = d->height;
     const int y0 = ty * kTile;
@@ -1258,7 +1265,7 @@ template <int kRowLog2, int kThreads>
 __global__ __launch_bounds__(kThreads) void k_stats(Plan plan) {
     constexpr int kRowSlots = 1 << kRowLog2;
     int ty;
-    const FrameDesc* d = plan.frames + frame_of(plan, SH_CLASS + (kRowLog2 - kRowExtLog2Min), blockIdx.x, &ty);
+    const KFrame* d = (const KFrame*)plan.frames + frame_of(plan, SH_CLASS + (kRowLog2 - kRowExtLog2Min), blockIdx.x, &ty);
     if (!d->active) return;
     const int32_t* roi = d->row_roi + 4 * ty;
     if (!roi[0]) {
@@ -1553,7 +1560,7 @@ __global__ __launch_bounds__(kThreads) void k_stats(Plan plan) {
 // run and keeps the roots).
 // The sort key of component root `idx` when it passes the tag-border
 // filter (reference k_collect_candidates :600-636).
-__device__ inline bool candidate_key(const FrameDesc* d, uint32_t idx, uint64_t* key) {
+__device__ inline bool candidate_key(const KFrame* d, uint32_t idx, uint64_t* key) {
     const uint32_t row_cap = static_cast<uint32_t>(d->row_cap);
     const int W = d->width, H = d->height;
     const int y = static_cast<int>(idx / row_cap);
@@ -1577,7 +1584,7 @@ __device__ inline bool candidate_key(const FrameDesc* d, uint32_t idx, uint64_t*
 // workgroups put every band of the 64-frame batch in one round.
 __global__ __launch_bounds__(64) void k_collect(Plan plan) {
     int ty;
-    const FrameDesc* d = plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
+    const KFrame* d = (const KFrame*)plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
     if (!d->active) return;
     const uint32_t base = static_cast<uint32_t>(ty * kTile) * static_cast<uint32_t>(d->row_cap);
     const uint32_t n = d->band_nroots[ty];
@@ -1613,7 +1620,7 @@ __device__ inline void bitonic_sort_lds(uint64_t* keys, int n) {
 // order fixed to (area desc, root slot asc)).  Lists longer than the LDS
 // buffer first find the key of rank max_candidates by an 8-bit radix select.
 __global__ __launch_bounds__(256) void k_select(Plan plan) {
-    const FrameDesc* d = plan.frames + blockIdx.x;
+    const KFrame* d = (const KFrame*)plan.frames + blockIdx.x;
     if (!d->active) return;
     __shared__ uint64_t keys[kSortCap];
     __shared__ uint32_t hist[256];
@@ -1955,7 +1962,7 @@ __device__ inline int fit_row(int s, int kept, int rows, int by0) {
 //     the peak rules and the parabola, and the moments are summed in the
 //     reference's 32-lane shuffle-down order (fit_edge_line_warp :881-1016).
 template <int kThreads>
-__device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
+__device__ void fit_candidate(const KFrame* d, int ci, int t) {
     // kThreads / 32 slices of the boundary in the direction probe; kThreads
     // / 256 waves per edge in the refinement's rise loop.
     constexpr int kParts = kThreads / 32, kSub = kThreads / 256;
@@ -1968,7 +1975,7 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
     }
 
     __shared__ float pdot[kParts][kFitDirections];
-    __shared__ uint32_t ppoint[kParts][kFitDirections];
+    __shared__ int prow[kParts][kFitDirections];
     __shared__ uint32_t extreme[kFitDirections];
     __shared__ uint32_t hull_pts[kFitDirections];
     // One LDS buffer holds the boundary points (until the hull is built) and
@@ -2057,43 +2064,60 @@ __device__ void fit_candidate(const FrameDesc* d, int ci, int t) {
     __syncthreads();
 
     {
-        // 8 consecutive slices of the boundary (multiples of four points,
-        // read four at a time) x 32 directions; slices are merged in order,
-        // so the result is still the first maximum in boundary order.
+        // kParts consecutive slices of the boundary rows x 32 directions.
+        // Only one side of each row is scanned: projections are rounded
+        // monotonically, so for ux <= 0 a row's left point projects at least
+        // as far as its right point (and comes first in boundary order), and
+        // for ux > 0 the right point projects at least as far.  The slices
+        // are merged in order, so each direction keeps the first row holding
+        // the largest projection; for ux > 0 that row's left point replaces
+        // the right one when it projects as far, being first.  The result is
+        // the first maximum in boundary order, as in :1132-1170.
         const int dir = t & 31, part = t >> 5;
-        const int chunk = (((points + kParts - 1) / kParts) + 3) & ~3;
+        const int chunk = (((kept + kParts - 1) / kParts) + 1) & ~1;  // rows, even
         const int begin = part * chunk;
-        const int end = min(points, begin + chunk);
+        const int end = min(kept, begin + chunk);
         const float ux = c_dir_cos[dir], uy = c_dir_sin[dir];
+        const bool right = ux > 0.0f;
         float best = -FLT_MAX;
-        uint32_t bp = 0xffffffffu;
-        for (int i = begin; i < end; i += 4) {
-            const uint4 q = *reinterpret_cast<const uint4*>(boundary + i);
-            const uint32_t pv[4] = {q.x, q.y, q.z, q.w};
+        int brow = -1;
+        for (int r = begin; r < end; r += 2) {
+            const uint4 q = *reinterpret_cast<const uint4*>(boundary + 2 * r);
+            const uint32_t pv[2] = {right ? q.y : q.x, right ? q.w : q.z};
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
+            for (int c = 0; c < 2; ++c) {
                 const uint32_t p = pv[c];
-                if (i + c >= end || p == 0xffffffffu) continue;
+                if (r + c >= end || p == 0xffffffffu) continue;
                 const float dd = static_cast<float>(p & 0xffffu) * ux + static_cast<float>(p >> 16) * uy;
                 if (dd > best) {
                     best = dd;
-                    bp = p;
+                    brow = r + c;
                 }
             }
         }
         pdot[part][dir] = best;
-        ppoint[part][dir] = bp;
+        prow[part][dir] = brow;
     }
     __syncthreads();
     if (t < kFitDirections) {
         float best = -FLT_MAX;
-        uint32_t point = 0xffffffffu;
+        int brow = -1;
         // Unrolled so that the parts' LDS reads are issued together.
 #pragma unroll
         for (int p = 0; p < kParts; ++p) {
             if (pdot[p][t] > best) {
                 best = pdot[p][t];
-                point = ppoint[p][t];
+                brow = prow[p][t];
+            }
+        }
+        uint32_t point = 0xffffffffu;
+        if (brow >= 0) {
+            const float ux = c_dir_cos[t], uy = c_dir_sin[t];
+            const uint32_t left = boundary[2 * brow];
+            point = left;
+            if (ux > 0.0f) {
+                const float dl = static_cast<float>(left & 0xffffu) * ux + static_cast<float>(left >> 16) * uy;
+                if (!(dl == best)) point = boundary[2 * brow + 1];
             }
         }
         extreme[t] = point;
@@ -2488,7 +2512,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kThrea
         // them in registers across it and halve occupancy).
         int t = threadIdx.x;
         asm volatile("" : "+v"(t));
-        fit_candidate<kThreads>(plan.frames + (code >> 10), static_cast<int>(code & 1023u), t);
+        fit_candidate<kThreads>((const KFrame*)plan.frames + (code >> 10), static_cast<int>(code & 1023u), t);
     }
 }
 
@@ -2570,7 +2594,7 @@ struct VariantResult {
 // matched with one lane per group of codewords.  `diff` is the wave's own
 // 64-entry LDS row (`reused`: the wave's previous variant read it).  Lane 0
 // writes the result.
-__device__ void decode_variant(const FrameDesc* d, int ci, int v, int lane, float* diff_row, bool reused,
+__device__ void decode_variant(const KFrame* d, int ci, int v, int lane, float* diff_row, bool reused,
                                const uint64_t* codes, int ncodes, VariantResult* out) {
     const int dim = d->dim;
     const int cells = dim * dim;
@@ -2699,7 +2723,7 @@ __device__ void decode_variant(const FrameDesc* d, int ci, int v, int lane, floa
 // The reference's preference among the 8 variants (refined first, highest
 // score, first on ties; k_decode_quads :1438-1537), written to the
 // candidate's decode record.
-__device__ void decode_choose(const FrameDesc* d, int ci, bool have_quad, const VariantResult* r) {
+__device__ void decode_choose(const KFrame* d, int ci, bool have_quad, const VariantResult* r) {
     bd_detection* res = d->dec + ci;
     int best = -1;
     float best_score = -1.0f;
@@ -2756,7 +2780,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
         const bool have = item < total;
         // Uniform across the wave: scalar registers for the descriptor.
         const uint32_t code = have ? __builtin_amdgcn_readfirstlane(work_item(plan, item, nhead)) : 0u;
-        const FrameDesc* d = plan.frames + (code >> 10);
+        const KFrame* d = (const KFrame*)plan.frames + (code >> 10);
         const int ci = static_cast<int>(code & 1023u);
         // A candidate without a quad decodes to nothing.
         const bool have_quad = have && d->quads[8 * static_cast<size_t>(ci)] == d->quads[8 * static_cast<size_t>(ci)];
@@ -2824,7 +2848,7 @@ __device__ inline void centre_side(const float* c, float* side, float* cx, float
 // does it serially, then placed by a stable rank on id, each with its FP64
 // homography; the records and the frame's counts go to pinned host memory.
 __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
-    const FrameDesc* d = plan.frames + blockIdx.x;
+    const KFrame* d = (const KFrame*)plan.frames + blockIdx.x;
     if (!d->active) return;
     const int t = threadIdx.x;
     // k_tile_stats' accumulators (histograms, active tile rows and each
@@ -3037,7 +3061,7 @@ __global__ __launch_bounds__(64) void k_publish(Plan plan, ResidentArgs a) {
     const uint32_t words = sizeof(bd_detection) / sizeof(uint32_t);
     static_assert(sizeof(bd_detection) % sizeof(uint32_t) == 0, "records copy as words");
     for (uint32_t k = 0; k < count; ++k) {
-        const FrameDesc* d = plan.frames + k;
+        const KFrame* d = (const KFrame*)plan.frames + k;
         const uint32_t t = a.lane_tickets[k];
         const uint32_t slot = t % static_cast<uint32_t>(a.depth);
         const int n = min(static_cast<int>(d->scal[S_COUNTS + 1]), d->max_detections);
