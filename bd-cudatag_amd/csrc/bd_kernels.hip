@@ -82,6 +82,14 @@ typedef const int32_t __attribute__((address_space(4))) kint32;
 // (only the host and the admission kernels, in earlier launches, write
 // them), so the uniform descriptor fields come through scalar loads.
 typedef const FrameDesc __attribute__((address_space(4))) KFrame;
+
+// A uniform read of data that no workgroup of the running kernel writes
+// (earlier launches wrote it): a constant-memory load, so a scalar load
+// through the scalar cache.
+template <class T>
+__device__ inline T kld(const T* p) {
+    return *(const T __attribute__((address_space(4)))*)p;
+}
 __device__ inline int frame_of(const Plan& plan, int shape, int block, int* local) {
     const kint32* p = (const kint32*)(plan.prefix + shape * (plan.nframes + 1));
     int lo = 0, hi = plan.nframes - 1;
@@ -605,7 +613,7 @@ __device__ void binarize_band(const Plan& plan, int cls, const KFrame* d, FrameR
     const bool ref_wave = ref_here && t < kWave;
     RefInputs ref_in;
     if (ref_wave) ref_in = load_ref_inputs(d, t);
-    const int row_lo = d->trow_lo[ty], row_hi = d->trow_hi[ty];
+    const int row_lo = kld(d->trow_lo + ty), row_hi = kld(d->trow_hi + ty);
 
     __shared__ uint8_t thr[1024];
     __shared__ uint32_t s_off[kTile + 1];
@@ -1167,11 +1175,11 @@ __global__ __launch_bounds__(kThreads) void k_merge(Plan plan, int cls) {
     // The band's ROI word and its first row pair's run counts, loaded
     // together (the counts do not depend on the ROI; frames have at least
     // two rows).
-    const int4 roi = *reinterpret_cast<const int4*>(d->row_roi + 4 * ty);
-    const int n_first = static_cast<int>(d->row_count[y0 + first_row]);
-    const int pn_first = static_cast<int>(d->row_count[y0 + first_row - 1]);
-    if (!roi.x) return;
-    const bool local = roi.w != 0;
+    const int roi_on = kld(d->row_roi + 4 * ty), roi_local = kld(d->row_roi + 4 * ty + 3);
+    const int n_first = static_cast<int>(kld(d->row_count + y0 + first_row));
+    const int pn_first = static_cast<int>(kld(d->row_count + y0 + first_row - 1));
+    if (!roi_on) return;
+    const bool local = roi_local != 0;
     const int last_row = local ? 1 : nrows;  // exclusive, band-relative
     if (first_row >= last_row) return;
     const int t = threadIdx.x;
@@ -1192,8 +1200,8 @@ __global__ __launch_bounds__(kThreads) void k_merge(Plan plan, int cls) {
     // LDS by coalesced loads when they fit, else read from global memory.
     for (int r = first_row; r < last_row; ++r) {
         const int y = y0 + r;
-        const int n = r == first_row ? n_first : static_cast<int>(d->row_count[y]);
-        const int pn = r == first_row ? pn_first : static_cast<int>(d->row_count[y - 1]);
+        const int n = r == first_row ? n_first : static_cast<int>(kld(d->row_count + y));
+        const int pn = r == first_row ? pn_first : static_cast<int>(kld(d->row_count + y - 1));
         if (n == 0 || pn == 0) continue;
         const uint32_t cur_base = static_cast<uint32_t>(y) * row_cap;
         const uint32_t prev_base = static_cast<uint32_t>(y - 1) * row_cap;
@@ -1267,12 +1275,11 @@ __global__ __launch_bounds__(kThreads) void k_stats(Plan plan) {
     int ty;
     const KFrame* d = (const KFrame*)plan.frames + frame_of(plan, SH_CLASS + (kRowLog2 - kRowExtLog2Min), blockIdx.x, &ty);
     if (!d->active) return;
-    const int32_t* roi = d->row_roi + 4 * ty;
-    if (!roi[0]) {
+    if (!kld(d->row_roi + 4 * ty)) {
         if (threadIdx.x == 0) d->band_nroots[ty] = 0;  // an inactive band has no runs
         return;
     }
-    const bool local = roi[3] != 0;
+    const bool local = kld(d->row_roi + 4 * ty + 3) != 0;
     const int H = d->height;
     const int y0 = ty * kTile;
     const int nrows = min(kTile, H - y0);
@@ -1587,7 +1594,7 @@ __global__ __launch_bounds__(64) void k_collect(Plan plan) {
     const KFrame* d = (const KFrame*)plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
     if (!d->active) return;
     const uint32_t base = static_cast<uint32_t>(ty * kTile) * static_cast<uint32_t>(d->row_cap);
-    const uint32_t n = d->band_nroots[ty];
+    const uint32_t n = kld(d->band_nroots + ty);
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         uint64_t key;
         if (!candidate_key(d, d->roots[base + i], &key)) continue;
@@ -1627,7 +1634,7 @@ __global__ __launch_bounds__(256) void k_select(Plan plan) {
     __shared__ uint64_t s_prefix, s_mask;
     __shared__ uint32_t s_rank, s_fill;
     const int t = threadIdx.x;
-    const uint32_t n = d->scal[S_CAND_ACC];
+    const uint32_t n = kld(d->scal + S_CAND_ACC);
     const uint32_t nsel = min(n, static_cast<uint32_t>(d->max_candidates));
 
     uint32_t m = n;
@@ -2493,7 +2500,7 @@ __device__ inline uint32_t next_item(uint32_t* cursor) {
 // largest candidates) first, then the tail region.
 __device__ inline uint32_t work_item(const Plan& plan, uint32_t item, uint32_t nhead) {
     const uint32_t* head = plan.work + kWorkItems;
-    return item < nhead ? head[item] : head[kHeadSlots * plan.nframes + (item - nhead)];
+    return kld(item < nhead ? head + item : head + kHeadSlots * plan.nframes + (item - nhead));
 }
 
 // Persistent workgroups over the batch's candidate list (k_select appends
@@ -2502,8 +2509,8 @@ __device__ inline uint32_t work_item(const Plan& plan, uint32_t item, uint32_t n
 // candidates) 1024, four waves per edge in the refinement's rise loop.
 template <int kThreads>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kThreads == 256 ? 8 : 4))) void k_fit(Plan plan) {
-    const uint32_t nhead = plan.work[kWorkHeadCount];
-    const uint32_t total = nhead + plan.work[kWorkCount];
+    const uint32_t nhead = kld(plan.work + kWorkHeadCount);
+    const uint32_t total = nhead + kld(plan.work + kWorkCount);
     for (uint32_t item = blockIdx.x; item < total; item = next_item(&plan.work[kWorkFitCursor])) {
         // Uniform across the workgroup: scalar registers for the descriptor.
         const uint32_t code = __builtin_amdgcn_readfirstlane(work_item(plan, item, nhead));
@@ -2603,8 +2610,8 @@ __device__ void decode_variant(const KFrame* d, int ci, int v, int lane, float* 
     float corners[4][2];
     for (int k = 0; k < 4; ++k) {
         const int src = (rot + k) & 3;
-        corners[k][0] = base[2 * src];
-        corners[k][1] = base[2 * src + 1];
+        corners[k][0] = kld(base + 2 * src);
+        corners[k][1] = kld(base + 2 * src + 1);
     }
     float h[9];
     bool ok = homography_from_unit_quad(corners, h);
@@ -2764,14 +2771,15 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
     constexpr int kPerBlock = kWaves >= 16 ? kWaves / 8 : 1;   // candidates per workgroup
     constexpr int kWavesPerCand = kWaves / kPerBlock;           // 4 or 8
     constexpr int kVariantsPerWave = 8 / kWavesPerCand;         // 2 or 1
-    const uint32_t nhead = plan.work[kWorkHeadCount];
-    const uint32_t total = nhead + plan.work[kWorkCount];
+    const uint32_t nhead = kld(plan.work + kWorkHeadCount);
+    const uint32_t total = nhead + kld(plan.work + kWorkCount);
     if (blockIdx.x * kPerBlock >= total) return;
     __shared__ uint64_t s_codes[kMaxCodes];
     __shared__ float s_diff[kWaves][kWave];
     __shared__ VariantResult s_res[kPerBlock][8];
-    const int ncodes = min(plan.frames[0].code_count, kMaxCodes);
-    for (int i = threadIdx.x; i < ncodes; i += blockDim.x) s_codes[i] = plan.frames[0].codes[i];
+    const KFrame* d0 = (const KFrame*)plan.frames;
+    const int ncodes = min(d0->code_count, kMaxCodes);
+    for (int i = threadIdx.x; i < ncodes; i += blockDim.x) s_codes[i] = d0->codes[i];
     __syncthreads();
     const int wave = static_cast<int>(threadIdx.x) / kWave, lane = static_cast<int>(threadIdx.x) % kWave;
     const int slot = wave / kWavesPerCand, wslot = wave % kWavesPerCand;
@@ -2783,7 +2791,8 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
         const KFrame* d = (const KFrame*)plan.frames + (code >> 10);
         const int ci = static_cast<int>(code & 1023u);
         // A candidate without a quad decodes to nothing.
-        const bool have_quad = have && d->quads[8 * static_cast<size_t>(ci)] == d->quads[8 * static_cast<size_t>(ci)];
+        const float q0 = kld(d->quads + 8 * static_cast<size_t>(ci));
+        const bool have_quad = have && q0 == q0;
         if (have_quad) {
 #pragma unroll
             for (int k = 0; k < kVariantsPerWave; ++k) {
@@ -2864,7 +2873,7 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
         d->scal[S_BOTTOM1] = 0;
     }
     const int maxd = d->max_detections;
-    const int ncand = static_cast<int>(d->scal[S_NCAND]);
+    const int ncand = static_cast<int>(kld(d->scal + S_NCAND));
     __shared__ int list[256];
     __shared__ int k_src[256];      // candidate index of each kept entry
     __shared__ float k_cx[256], k_cy[256], k_margin[256];
