@@ -2018,24 +2018,36 @@ __device__ void fit_candidate(const KFrame* d, int ci, int t) {
         const int y = fit_row(s, kept, rows, by0);
         uint32_t lo = 0xffffffffu, hi = 0u;  // min x0, max x1 + 1 (0: none)
         const uint32_t entries = d->row_ext_full[y];
-        if (entries != kRowTableFull) {
-            // The row's packed entries, four per step (two 16-byte loads in
-            // flight; slots past `entries` hold no entry of this row).
-            const uint64_t* tab = d->row_ext + (static_cast<size_t>(y) << d->row_ext_log2);
-            for (uint32_t k = 0; k < entries; k += 4) {
-                const ulonglong2 e01 = *reinterpret_cast<const ulonglong2*>(tab + k);
-                const ulonglong2 e23 = *reinterpret_cast<const ulonglong2*>(tab + k + 2);
-                const uint64_t ev[4] = {e01.x, e01.y, e23.x, e23.y};
-                int hit = -1;
+        // The row's packed entries, four per step (two 16-byte loads in
+        // flight; slots past `entries` hold no entry of this row).  The first
+        // step's loads go out with the entry count (a table has at least 64
+        // slots) and its match is taken whatever the count says, so the
+        // loads are not held back behind it; a full table ignores it.
+        const uint64_t* tab = d->row_ext + (static_cast<size_t>(y) << d->row_ext_log2);
+        const auto match4 = [&](uint32_t k, const ulonglong2& e01, const ulonglong2& e23) {
+            const uint64_t ev[4] = {e01.x, e01.y, e23.x, e23.y};
+            int hit = -1;
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    if (hit < 0 && k + c < entries && static_cast<uint32_t>(ev[c] >> 32) == root) hit = c;
-                }
-                if (hit >= 0) {
-                    lo = static_cast<uint32_t>(ev[hit] >> 16) & 0xffffu;
-                    hi = (static_cast<uint32_t>(ev[hit]) & 0xffffu) + 1u;
-                    break;
-                }
+            for (int c = 0; c < 4; ++c) {
+                if (hit < 0 && k + c < entries && static_cast<uint32_t>(ev[c] >> 32) == root) hit = c;
+            }
+            return hit < 0 ? 0ull : ev[hit] | (1ull << 63);
+        };
+        const ulonglong2 f01 = *reinterpret_cast<const ulonglong2*>(tab);
+        const ulonglong2 f23 = *reinterpret_cast<const ulonglong2*>(tab + 2);
+        // An empty asm statement that takes the three loaded values keeps
+        // the compiler from sinking the table loads into the branch below,
+        // behind the count's round trip.
+        asm volatile("" : : "v"(entries), "v"(f01.x), "v"(f23.x));
+        const uint64_t first = match4(0u, f01, f23);
+        if (entries != kRowTableFull) {
+            uint64_t found = first;
+            for (uint32_t k = 4; found == 0ull && k < entries; k += 4)
+                found = match4(k, *reinterpret_cast<const ulonglong2*>(tab + k),
+                               *reinterpret_cast<const ulonglong2*>(tab + k + 2));
+            if (found != 0ull) {
+                lo = static_cast<uint32_t>(found >> 16) & 0xffffu;
+                hi = (static_cast<uint32_t>(found) & 0xffffu) + 1u;
             }
         } else if (d->row_roi[4 * (y / kTile) + 3]) {
             // The row's items whose band root's component root (its label,
