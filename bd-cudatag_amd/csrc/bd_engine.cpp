@@ -384,7 +384,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         }
     };
     set_band_runs(&plan_host_, 0, nf);
-    for (int k = 0; k < SH_COUNT; ++k) plan_host_.blocks[k] = prefix[k * (nf + 1) + nf];
+    plan_set_blocks(&plan_host_, prefix.data(), nf);
     plan_dev_ = plan_host_;
     plan_dev_.frames = d_desc_dev_;
     for (int l = 0; l < 2 && lanes_ == 2; ++l) {
@@ -397,7 +397,7 @@ bool Engine::init(const Params& params, const std::vector<int>& widths,
         p.nframes = n;
         p.work_cap = static_cast<int32_t>(lane_cap[l]);
         set_band_runs(&p, lane_first[l], n);
-        for (int k = 0; k < SH_COUNT; ++k) p.blocks[k] = lane_prefix[l][static_cast<size_t>(k) * (n + 1) + n];
+        plan_set_blocks(&p, lane_prefix[l].data(), n);
         lane_dev_[l] = p;
         lane_dev_[l].frames = d_desc_dev_ + lane_first[l];
     }

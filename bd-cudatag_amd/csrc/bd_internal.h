@@ -225,6 +225,10 @@ struct Plan {
     int32_t class_merge_stage[kWidthClasses];
     Tune tune;
     int32_t blocks[SH_COUNT];
+    // Blocks of every frame for a shape whose frames all have the same
+    // count (0: the counts differ), so that kernels find a block's frame by
+    // one division instead of the prefix table.
+    int32_t per_frame[SH_COUNT];
     // Nonzero: k_binarize_runs also writes the runs and labels of the rows
     // inside each band, which no later kernel reads (stage capture for
     // stage-by-stage comparisons).
@@ -235,6 +239,19 @@ struct Plan {
     // workgroup and more workgroups in flight.
     int32_t wide_blocks;
 };
+
+// Fills blocks and per_frame from the shapes' prefix tables (SH_COUNT
+// tables of n + 1 entries).
+inline void plan_set_blocks(Plan* p, const int32_t* prefix, int n) {
+    for (int k = 0; k < SH_COUNT; ++k) {
+        const int32_t* q = prefix + static_cast<size_t>(k) * (n + 1);
+        p->blocks[k] = q[n];
+        int32_t per = n > 0 ? q[1] - q[0] : 0;
+        for (int f = 1; f < n && per > 0; ++f)
+            if (q[f + 1] - q[f] != per) per = 0;
+        p->per_frame[k] = per;
+    }
+}
 
 // Device-resident serving (bd_server_create_resident): requests of one frame
 // size class arrive in a ring in pinned host memory; a lane's launch admits

@@ -73,15 +73,75 @@ __device__ inline uint4 load16(const gbyte* image, size_t o) {
     return make_uint4(v.x, v.y, v.z, v.w);
 }
 
-// Frame of a block for one grid shape (prefix tables are tiny: one entry
-// per frame).  The tables do not change while a kernel runs, so they are
-// read as constant memory: scalar loads through the scalar cache for the
-// search's dependent steps.
+// Frame of a block for one grid shape.  Frames with equal block counts for
+// the shape (a uniform batch) divide; otherwise the prefix table (one entry
+// per frame; every table starts at 0 and ends at the shape's block count) is
+// searched.  The tables do not change while a kernel runs, so they are read
+// as constant memory, scalar loads through the scalar cache: the frame at
+// the proportional guess and its successor's entry in one 8-byte load
+// first, a binary search on the side the block lies if the guess misses.
 typedef const int32_t __attribute__((address_space(4))) kint32;
+typedef const uint64_t __attribute__((address_space(4))) kuint64;
+__device__ inline int frame_of(const Plan& plan, int shape, int block, int* local) {
+    const int n = plan.nframes;
+    const int per = plan.per_frame[shape];
+    if (n == 1 || per > 0) {
+        const int f = n == 1 ? 0 : block / per;
+        *local = block - f * per;
+        return f;
+    }
+    const int32_t* prefix = plan.prefix + shape * (n + 1);
+    const uint32_t total = static_cast<uint32_t>(plan.blocks[shape]);
+    const int g = min(n - 1, static_cast<int>(static_cast<uint32_t>(block) * static_cast<uint32_t>(n) / max(total, 1u)));
+    // One 8-byte load for both entries (4-byte alignment suffices for
+    // scalar loads).
+    const uint64_t ab = *(const kuint64*)(prefix + g);
+    const int a = static_cast<int>(static_cast<uint32_t>(ab)), b = static_cast<int>(ab >> 32);
+    if (a <= block && block < b) {
+        *local = block - a;
+        return g;
+    }
+    const kint32* p = (const kint32*)prefix;
+    int lo = block < a ? 0 : g + 1, hi = block < a ? g - 1 : n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (p[mid] <= block) lo = mid; else hi = mid - 1;
+    }
+    *local = block - p[lo];
+    return lo;
+}
+
 // Frame descriptors as the pipeline kernels read them: constant memory too
 // (only the host and the admission kernels, in earlier launches, write
 // them), so the uniform descriptor fields come through scalar loads.
 typedef const FrameDesc __attribute__((address_space(4))) KFrame;
+
+// The descriptor's first eight words (frame pointer, active flag, width,
+// height, stride, tile columns and rows) in one scalar load, so that the
+// active check and the frame geometry share one round trip.
+struct FrameHead {
+    const uint8_t* gray;
+    int active, width, height, stride, tile_w, tile_h;
+};
+__device__ inline FrameHead frame_head(const KFrame* d) {
+    static_assert(offsetof(FrameDesc, gray) == 0 && offsetof(FrameDesc, active) == 8 &&
+                      offsetof(FrameDesc, width) == 12 && offsetof(FrameDesc, height) == 16 &&
+                      offsetof(FrameDesc, stride) == 20 && offsetof(FrameDesc, tile_w) == 24 &&
+                      offsetof(FrameDesc, tile_h) == 28,
+                  "descriptor head layout");
+    typedef int i8v __attribute__((ext_vector_type(8)));
+    const i8v v = *(const __attribute__((address_space(4))) i8v*)d;
+    FrameHead h;
+    h.gray = reinterpret_cast<const uint8_t*>((static_cast<uint64_t>(static_cast<uint32_t>(v[1])) << 32) |
+                                              static_cast<uint32_t>(v[0]));
+    h.active = v[2];
+    h.width = v[3];
+    h.height = v[4];
+    h.stride = v[5];
+    h.tile_w = v[6];
+    h.tile_h = v[7];
+    return h;
+}
 
 // A uniform read of data that no workgroup of the running kernel writes
 // (earlier launches wrote it): a constant-memory load, so a scalar load
@@ -89,16 +149,6 @@ typedef const FrameDesc __attribute__((address_space(4))) KFrame;
 template <class T>
 __device__ inline T kld(const T* p) {
     return *(const T __attribute__((address_space(4)))*)p;
-}
-__device__ inline int frame_of(const Plan& plan, int shape, int block, int* local) {
-    const kint32* p = (const kint32*)(plan.prefix + shape * (plan.nframes + 1));
-    int lo = 0, hi = plan.nframes - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (p[mid] <= block) lo = mid; else hi = mid - 1;
-    }
-    *local = block - p[lo];
-    return lo;
 }
 
 // Inclusive prefix sum over the wave by DPP (the whole wave must be
@@ -217,9 +267,10 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
     if (blockIdx.x == 0 && threadIdx.x < kWorkItems) plan.work[threadIdx.x] = 0;
     int blk;
     const KFrame* d = (const KFrame*)plan.frames + frame_of(plan, SH_TSTAT, blockIdx.x, &blk);
-    if (!d->active) return;
-    const int W = d->width, H = d->height, stride = d->stride, tw = d->tile_w, th = d->tile_h;
-    const uint8_t* gray = d->gray;
+    const FrameHead fh = frame_head(d);
+    if (!fh.active) return;
+    const int W = fh.width, H = fh.height, stride = fh.stride, tw = fh.tile_w, th = fh.tile_h;
+    const uint8_t* gray = fh.gray;
     const int t = threadIdx.x;
     // The counts later kernels accumulate start at zero (the histograms and
     // the active extents this kernel accumulates were cleared by the
@@ -239,6 +290,23 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
     const int g1 = min(g0 + kTileStatBlocks, th * pairs);
     const int ty0 = g0 / pairs;             // first tile row of the workgroup
     const int nrows = (g1 - 1) / pairs - ty0 + 1;
+    // This thread's block: its eight 16-byte rows are issued first, so
+    // that their latency overlaps the LDS set-up below.
+    const int g = g0 + t;
+    const bool have = g < g1;
+    const int ty = have ? g / pairs : ty0;
+    const int p = g - ty * pairs;
+    const int y0 = ty * kTile;
+    const int rows = min(y0 + kTile, H) - y0;
+    const int x0 = p * 16;
+    const bool vec = ((reinterpret_cast<uintptr_t>(gray) | static_cast<uintptr_t>(stride)) & 15u) == 0;
+    const bool whole = have && vec && x0 + 16 <= W && rows == kTile;
+    uint4 v[kTile];
+    if (whole) {
+        const gbyte* src = frame_bytes(gray) + static_cast<size_t>(y0) * stride + x0;
+#pragma unroll
+        for (int r = 0; r < kTile; ++r) v[r] = load16(src, static_cast<size_t>(r) * stride);
+    }
     // Histograms, one copy per wave: neighbouring tiles often share a bin,
     // and atomics to one LDS address serialise.
     constexpr int kWaves = kTileStatBlocks / kWave;
@@ -248,39 +316,28 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
     // and written with coalesced word stores.
     constexpr int kStage = 2 * kTileStatBlocks;
     __shared__ __align__(4) uint8_t s_min[kStage], s_max[kStage], s_mean[kStage];
-    __shared__ int s_top, s_bottom;
+    __shared__ uint32_t s_top, s_bottom1;
     for (int i = t; i < kWaves * 320; i += blockDim.x) (&hs[0][0])[i] = 0;
     if (t < nrows) {
         lo_s[t] = tw;
         hi_s[t] = -1;
     }
     if (t == 0) {
-        s_top = INT32_MAX;
-        s_bottom = -1;
+        s_top = 0xffffffffu;
+        s_bottom1 = 0;
     }
     const int first_tile = ty0 * tw + 2 * (g0 - ty0 * pairs);
     __syncthreads();
 
     uint32_t* my_hs = hs[t / kWave];
-    const int g = g0 + t;
-    if (g < g1) {
-        const int ty = g / pairs;
-        const int p = g - ty * pairs;
-        const int y0 = ty * kTile;
-        const int y1 = min(y0 + kTile, H);
-        const int rows = y1 - y0;
-        const bool vec = ((reinterpret_cast<uintptr_t>(gray) | static_cast<uintptr_t>(stride)) & 15u) == 0;
-        const int x0 = p * 16;
+    // The workgroup's first and last active tile rows (a wave reduction,
+    // one LDS atomic per wave): 1 + row of this thread's active tiles.
+    uint32_t act_top = 0xffffffffu, act_bottom1 = 0;
+    if (have) {
         uint32_t mnA = 255, mxA = 0, sumA = 0, mnB = 255, mxB = 0, sumB = 0;
         const int nA = min(8, W - x0);
         const int nB = min(8, W - x0 - 8);
-        if (vec && x0 + 16 <= W && rows == kTile) {
-            // A whole 16x8 block: all eight row loads in flight before
-            // any is used.
-            const gbyte* src = frame_bytes(gray) + static_cast<size_t>(y0) * stride + x0;
-            uint4 v[kTile];
-#pragma unroll
-            for (int r = 0; r < kTile; ++r) v[r] = load16(src, static_cast<size_t>(r) * stride);
+        if (whole) {
 #pragma unroll
             for (int r = 0; r < kTile; ++r) {
                 acc_word(v[r].x, mnA, mxA, sumA);
@@ -289,27 +346,29 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
                 acc_word(v[r].w, mnB, mxB, sumB);
             }
         } else if (vec && x0 + 16 <= W) {
+            const int y1 = y0 + rows;
             for (int y = y0; y < y1; ++y) {
-                const uint4 v = load16(frame_bytes(gray), static_cast<size_t>(y) * stride + x0);
-                acc_word(v.x, mnA, mxA, sumA);
-                acc_word(v.y, mnA, mxA, sumA);
-                acc_word(v.z, mnB, mxB, sumB);
-                acc_word(v.w, mnB, mxB, sumB);
+                const uint4 w = load16(frame_bytes(gray), static_cast<size_t>(y) * stride + x0);
+                acc_word(w.x, mnA, mxA, sumA);
+                acc_word(w.y, mnA, mxA, sumA);
+                acc_word(w.z, mnB, mxB, sumB);
+                acc_word(w.w, mnB, mxB, sumB);
             }
         } else {
+            const int y1 = y0 + rows;
             for (int y = y0; y < y1; ++y) {
                 const uint8_t* row = gray + static_cast<size_t>(y) * stride + x0;
                 for (int x = 0; x < nA; ++x) {
-                    const uint32_t v = row[x];
-                    mnA = min(mnA, v);
-                    mxA = max(mxA, v);
-                    sumA += v;
+                    const uint32_t px = row[x];
+                    mnA = min(mnA, px);
+                    mxA = max(mxA, px);
+                    sumA += px;
                 }
                 for (int x = 0; x < nB; ++x) {
-                    const uint32_t v = row[8 + x];
-                    mnB = min(mnB, v);
-                    mxB = max(mxB, v);
-                    sumB += v;
+                    const uint32_t px = row[8 + x];
+                    mnB = min(mnB, px);
+                    mxB = max(mxB, px);
+                    sumB += px;
                 }
             }
         }
@@ -340,9 +399,23 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
         if (hi >= 0) {
             atomicMin(&lo_s[ty - ty0], lo);
             atomicMax(&hi_s[ty - ty0], hi);
+            act_top = static_cast<uint32_t>(ty);
+            act_bottom1 = static_cast<uint32_t>(ty) + 1u;
         }
     }
+    act_top = wave_min(act_top);
+    act_bottom1 = wave_max(act_bottom1);
+    if (t % kWave == 0 && act_bottom1 != 0) {
+        atomicMin(&s_top, act_top);
+        atomicMax(&s_bottom1, act_bottom1);
+    }
     __syncthreads();
+    // One pair of global atomics per workgroup for the frame's first and
+    // last active tile rows.
+    if (t == 0 && s_bottom1 != 0) {
+        atomicMin(&d->scal[S_TOP], s_top);
+        atomicMax(&d->scal[S_BOTTOM1], s_bottom1);
+    }
     for (int i = t; i < 320; i += blockDim.x) {
         uint32_t n = 0;
 #pragma unroll
@@ -353,8 +426,6 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
         const int row = ty0 + t;
         atomicMin(&d->trow_lo[row], lo_s[t]);
         atomicMax(&d->trow_hi[row], hi_s[t]);
-        atomicMin(&s_top, row);
-        atomicMax(&s_bottom, row);
     }
     // The staged tiles: bytes [first, first + count) of each tile array, as
     // aligned words with the unaligned ends byte by byte.
@@ -379,13 +450,6 @@ __global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
         d->tile_min[first + o] = s_min[o];
         d->tile_max[first + o] = s_max[o];
         d->tile_mean[first + o] = s_mean[o];
-    }
-    __syncthreads();
-    // One pair of global atomics per workgroup for the frame's first and
-    // last active tile rows.
-    if (t == 0 && s_bottom >= 0) {
-        atomicMin(&d->scal[S_TOP], static_cast<uint32_t>(s_top));
-        atomicMax(&d->scal[S_BOTTOM1], static_cast<uint32_t>(s_bottom + 1));
     }
 }
 
@@ -1046,7 +1110,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6))) v
     int unit;
     const KFrame* d =
         (const KFrame*)plan.frames + frame_of(plan, (kBands == 1 ? SH_CLASS : SH_CLASS_PAIR) + cls, blockIdx.x, &unit);
-    if (!d->active) return;
+    if (!frame_head(d).active) return;
     __shared__ FrameRef s_fr;
 #pragma unroll 1
     for (int b = 0; b < kBands; ++b) {
@@ -1167,7 +1231,7 @@ template <int kThreads, int kSlots>
 __global__ __launch_bounds__(kThreads) void k_merge(Plan plan, int cls) {
     int ty;
     const KFrame* d = (const KFrame*)plan.frames + frame_of(plan, SH_CLASS + cls, blockIdx.x, &ty);
-    if (!d->active) return;
+    if (!frame_head(d).active) return;
     const int H = d->height;
     const int y0 = ty * kTile;
     const int nrows = min(kTile, H - y0);
@@ -1274,7 +1338,7 @@ __global__ __launch_bounds__(kThreads) void k_stats(Plan plan) {
     constexpr int kRowSlots = 1 << kRowLog2;
     int ty;
     const KFrame* d = (const KFrame*)plan.frames + frame_of(plan, SH_CLASS + (kRowLog2 - kRowExtLog2Min), blockIdx.x, &ty);
-    if (!d->active) return;
+    if (!frame_head(d).active) return;
     if (!kld(d->row_roi + 4 * ty)) {
         if (threadIdx.x == 0) d->band_nroots[ty] = 0;  // an inactive band has no runs
         return;
@@ -1592,7 +1656,7 @@ __device__ inline bool candidate_key(const KFrame* d, uint32_t idx, uint64_t* ke
 __global__ __launch_bounds__(64) void k_collect(Plan plan) {
     int ty;
     const KFrame* d = (const KFrame*)plan.frames + frame_of(plan, SH_TILE_ROW, blockIdx.x, &ty);
-    if (!d->active) return;
+    if (!frame_head(d).active) return;
     const uint32_t base = static_cast<uint32_t>(ty * kTile) * static_cast<uint32_t>(d->row_cap);
     const uint32_t n = kld(d->band_nroots + ty);
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
@@ -1628,7 +1692,7 @@ __device__ inline void bitonic_sort_lds(uint64_t* keys, int n) {
 // buffer first find the key of rank max_candidates by an 8-bit radix select.
 __global__ __launch_bounds__(256) void k_select(Plan plan) {
     const KFrame* d = (const KFrame*)plan.frames + blockIdx.x;
-    if (!d->active) return;
+    if (!frame_head(d).active) return;
     __shared__ uint64_t keys[kSortCap];
     __shared__ uint32_t hist[256];
     __shared__ uint64_t s_prefix, s_mask;
@@ -2007,7 +2071,6 @@ __device__ void fit_candidate(const KFrame* d, int ci, int t) {
     const int by1 = static_cast<int>(d->bbox[4 * static_cast<size_t>(root) + 3]);
     const int rows = by1 - by0 + 1;
     const int kept = min(rows, kMaxBoundaryRows);
-    const int points = 2 * kept;
 
     // Row extremes: the component's leftmost start and rightmost end in
     // each sampled row, looked up in the row's component table (written by
@@ -2870,7 +2933,7 @@ __device__ inline void centre_side(const float* c, float* side, float* cx, float
 // homography; the records and the frame's counts go to pinned host memory.
 __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
     const KFrame* d = (const KFrame*)plan.frames + blockIdx.x;
-    if (!d->active) return;
+    if (!frame_head(d).active) return;
     const int t = threadIdx.x;
     // k_tile_stats' accumulators (histograms, active tile rows and each
     // tile row's active extent) start cleared for the frame's next pass;

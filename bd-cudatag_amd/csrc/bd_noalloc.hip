@@ -173,7 +173,7 @@ int detect_noalloc(const Params& p, int width, int height, const uint8_t* gray, 
     plan.tune = default_tune();
     plan.wide_blocks = binarize_wide_blocks();
     plan.tune.streams = 1;
-    for (int k = 0; k < SH_COUNT; ++k) plan.blocks[k] = blocks[k];
+    for (int k = 0; k < SH_COUNT; ++k) plan.blocks[k] = plan.per_frame[k] = blocks[k];
 
     hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(256), 0, stream, a);
     hipError_t st = hipGetLastError();
