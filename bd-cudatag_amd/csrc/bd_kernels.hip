@@ -857,7 +857,9 @@ __device__ void binarize_band(const Plan& plan, int cls, const KFrame* d, FrameR
     }
     __syncthreads();
 
-    const int wave = t / kWave, lane = t % kWave;
+    // The wave index in a scalar register: rows loop per wave, and the row
+    // offsets indexed by it stay scalar.
+    const int wave = __builtin_amdgcn_readfirstlane(t / kWave), lane = t % kWave;
     const int nwaves = static_cast<int>(blockDim.x) / kWave;
     // Start/end masks of word w of band row r.
     auto edges = [&](int r, int w, uint64_t* starts, uint64_t* ends) {
@@ -964,7 +966,7 @@ __device__ void binarize_band(const Plan& plan, int cls, const KFrame* d, FrameR
     // starts at or below i, minus one.
     uint32_t so[kTile + 1];
 #pragma unroll
-    for (int q = 0; q <= kTile; ++q) so[q] = q <= nrows ? s_off[q] : nb;
+    for (int q = 0; q <= kTile; ++q) so[q] = __builtin_amdgcn_readfirstlane(q <= nrows ? s_off[q] : nb);
     const auto row_of = [&so](uint32_t i) {
         int r = 0;
 #pragma unroll
@@ -1063,7 +1065,7 @@ __device__ void binarize_band(const Plan& plan, int cls, const KFrame* d, FrameR
             }
             const bool is_root = valid && root == i;
             const int rr = row_of(root);
-            const uint32_t ridx = root - so[rr];
+            const uint32_t ridx = root - s_off[rr];  // an LDS read, not a select over so
             if (valid && (whole_row || is_root)) {
                 d->runs[row_slot + k] = run | (is_root ? kRootFlag : 0u);
                 d->label[row_slot + k] = static_cast<uint32_t>(y0 + rr) * static_cast<uint32_t>(row_cap) + ridx;
