@@ -902,21 +902,32 @@ __device__ void binarize_band(const Plan& plan, int cls, const KFrame* d, FrameR
         const uint32_t lbase = s_off[r];
         uint32_t carry_s = 0, carry_e = 0;
         // One 32-pixel half word per lane, starts and ends extracted in
-        // one loop.
+        // one loop.  The neighbouring halves' edge pixels come from the
+        // neighbouring lanes (DPP; the row's ends and halves past it are
+        // white, as is everything around image row 0's 32-px fragments,
+        // reference k_find_runs :454, :485-487), and one wave scan counts
+        // starts and ends together (16 bits each: a pass covers 2048 px).
         const int nh = 2 * w64;
+        const uint32_t* masks32 = reinterpret_cast<const uint32_t*>(masks) + r * nh;
+        const bool row0 = y == 0;
+        uint32_t carry_word = 0xffffffffu;  // the half before the pass
         for (int h0 = 0; h0 < nh; h0 += kWave) {
             const int hw = h0 + lane;
-            uint32_t starts = 0, ends = 0;
-            if (hw < nh) {
-                uint64_t st, en;
-                edges(r, hw >> 1, &st, &en);
-                const int sh = 32 * (hw & 1);
-                starts = static_cast<uint32_t>(st >> sh);
-                ends = static_cast<uint32_t>(en >> sh);
-            }
-            uint32_t tot_s, tot_e;
-            uint32_t ks = carry_s + wave_excl_scan(__popc(starts), lane, &tot_s);
-            uint32_t ke = carry_e + wave_excl_scan(__popc(ends), lane, &tot_e);
+            const uint32_t m32 = hw < nh ? masks32[hw] : 0xffffffffu;
+            const uint32_t below = dpp_from_lower(carry_word, m32);
+            uint32_t above = dpp_from_upper(0xffffffffu, m32);
+            if (lane == kWave - 1 && hw + 1 < nh) above = masks32[hw + 1];
+            const uint32_t prev_bit = row0 ? 1u : below >> 31;
+            const uint32_t next_bit = row0 ? 1u : above & 1u;
+            uint32_t starts = ~m32 & ((m32 << 1) | prev_bit);
+            uint32_t ends = ~m32 & ((m32 >> 1) | (next_bit << 31));
+            carry_word = __builtin_amdgcn_readlane(m32, kWave - 1);
+            uint32_t tot;
+            const uint32_t ex = wave_excl_scan(static_cast<uint32_t>(__popc(starts)) |
+                                                   (static_cast<uint32_t>(__popc(ends)) << 16), lane, &tot);
+            uint32_t ks = carry_s + (ex & 0xffffu);
+            uint32_t ke = carry_e + (ex >> 16);
+            const uint32_t tot_s = tot & 0xffffu, tot_e = tot >> 16;
             if (hw < nh) end_before[r * 2 * w64 + hw] = static_cast<uint16_t>(ke);
             const int xb = hw * 32;
             if (local) {
