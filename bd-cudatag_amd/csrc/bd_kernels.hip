@@ -896,6 +896,11 @@ __device__ void binarize_band(const Plan& plan, int cls, const KFrame* d, FrameR
     // run a root) and initialise the component statistics.
     uint16_t* runs16 = reinterpret_cast<uint16_t*>(d->runs);
     uint16_t* lds16 = reinterpret_cast<uint16_t*>(b_run);
+    // The band row of each of the band's first kRowTab runs, in the
+    // threshold buffer (no longer read): one LDS read instead of counting
+    // row starts wherever a run's row is needed.
+    constexpr uint32_t kRowTab = sizeof(thr);
+    uint8_t* rowtab = thr;
     for (int r = wave; r < nrows; r += nwaves) {
         const int y = y0 + r;
         const uint32_t base = static_cast<uint32_t>(y) * static_cast<uint32_t>(row_cap);
@@ -938,7 +943,10 @@ __device__ void binarize_band(const Plan& plan, int cls, const KFrame* d, FrameR
                     const int bs = __ffs(starts) - 1, be = __ffs(ends) - 1;
                     starts &= starts - 1;
                     ends &= ends - 1;
-                    if (has_s) lds16[2 * (lbase + ks)] = static_cast<uint16_t>(xb + bs);
+                    if (has_s) {
+                        lds16[2 * (lbase + ks)] = static_cast<uint16_t>(xb + bs);
+                        if (lbase + ks < kRowTab) rowtab[lbase + ks] = static_cast<uint8_t>(r);
+                    }
                     if (has_e) lds16[2 * (lbase + ke) + 1] = static_cast<uint16_t>(xb + be);
                     ks += has_s ? 1u : 0u;
                     ke += has_e ? 1u : 0u;
@@ -978,7 +986,8 @@ __device__ void binarize_band(const Plan& plan, int cls, const KFrame* d, FrameR
     uint32_t so[kTile + 1];
 #pragma unroll
     for (int q = 0; q <= kTile; ++q) so[q] = __builtin_amdgcn_readfirstlane(q <= nrows ? s_off[q] : nb);
-    const auto row_of = [&so](uint32_t i) {
+    const auto row_of = [&so, rowtab](uint32_t i) {
+        if (i < kRowTab) return static_cast<int>(rowtab[i]);
         int r = 0;
 #pragma unroll
         for (int q = 1; q < kTile; ++q) r += i >= so[q] ? 1 : 0;
