@@ -1261,7 +1261,9 @@ __global__ __launch_bounds__(kThreads) void k_merge(Plan plan, int cls) {
     // The band's ROI word and its first row pair's run counts, loaded
     // together (the counts do not depend on the ROI; frames have at least
     // two rows).
-    const int roi_on = kld(d->row_roi + 4 * ty), roi_local = kld(d->row_roi + 4 * ty + 3);
+    typedef int i4v __attribute__((ext_vector_type(4)));
+    const i4v roi = kld(reinterpret_cast<const i4v*>(d->row_roi + 4 * ty));
+    const int roi_on = roi[0], roi_local = roi[3];
     const int n_first = static_cast<int>(kld(d->row_count + y0 + first_row));
     const int pn_first = static_cast<int>(kld(d->row_count + y0 + first_row - 1));
     if (!roi_on) return;
@@ -1276,7 +1278,7 @@ __global__ __launch_bounds__(kThreads) void k_merge(Plan plan, int cls) {
     uint32_t* s_cur_lab = merge_stage + stage_cap;
     uint32_t* s_prev = merge_stage + 2 * stage_cap;
     uint32_t* s_prev_lab = merge_stage + 3 * stage_cap;
-    for (int e = t; e < kSlots; e += blockDim.x) pairs[e] = ~0ull;
+    for (int e = t; e < kSlots; e += kThreads) pairs[e] = ~0ull;
     const uint32_t row_cap = static_cast<uint32_t>(d->row_cap);
     uint32_t* label = d->label;
     __syncthreads();
@@ -1293,19 +1295,32 @@ __global__ __launch_bounds__(kThreads) void k_merge(Plan plan, int cls) {
         const uint32_t prev_base = static_cast<uint32_t>(y - 1) * row_cap;
         if (n <= stage_cap && pn <= stage_cap) {
             __syncthreads();  // the previous pair's readers are done
-            for (int i = t; i < n; i += blockDim.x) {
-                s_cur[i] = d->runs[cur_base + i];
-                s_cur_lab[i] = label[cur_base + i];
-            }
-            for (int i = t; i < pn; i += blockDim.x) {
-                s_prev[i] = d->runs[prev_base + i];
-                s_prev_lab[i] = label[prev_base + i];
+            // Both rows in one pass: a thread's four loads are issued
+            // before any of its LDS stores waits for them.
+            for (int i = t; i < max(n, pn); i += kThreads) {
+                uint32_t cr = 0, cl = 0, pr = 0, pl = 0;
+                if (i < n) {
+                    cr = d->runs[cur_base + i];
+                    cl = label[cur_base + i];
+                }
+                if (i < pn) {
+                    pr = d->runs[prev_base + i];
+                    pl = label[prev_base + i];
+                }
+                if (i < n) {
+                    s_cur[i] = cr;
+                    s_cur_lab[i] = cl;
+                }
+                if (i < pn) {
+                    s_prev[i] = pr;
+                    s_prev_lab[i] = pl;
+                }
             }
             __syncthreads();
-            for (int i = t; i < n; i += blockDim.x)
+            for (int i = t; i < n; i += kThreads)
                 merge_run<kSlots>(pairs, label, s_cur, s_cur_lab, s_prev, s_prev_lab, pn, i);
         } else {
-            for (int i = t; i < n; i += blockDim.x)
+            for (int i = t; i < n; i += kThreads)
                 merge_run<kSlots>(pairs, label, d->runs + cur_base, label + cur_base, d->runs + prev_base,
                                   label + prev_base, pn, i);
         }
@@ -1319,7 +1334,7 @@ __global__ __launch_bounds__(kThreads) void k_merge(Plan plan, int cls) {
     __syncthreads();
     unsigned long long mine[kSlots / kThreads];
     int nmine = 0;
-    for (int e = t; e < kSlots; e += blockDim.x) {
+    for (int e = t; e < kSlots; e += kThreads) {
         const unsigned long long key = pairs[e];
         if (key != ~0ull) mine[nmine++] = key;
     }
@@ -1327,7 +1342,7 @@ __global__ __launch_bounds__(kThreads) void k_merge(Plan plan, int cls) {
     for (int q = 0; q < nmine; ++q) pairs[atomicAdd(&s_npairs, 1u)] = mine[q];
     __syncthreads();
     const uint32_t npairs = s_npairs;
-    for (uint32_t k = t; k < npairs; k += blockDim.x) {
+    for (uint32_t k = t; k < npairs; k += kThreads) {
         const unsigned long long key = pairs[k];
         uf_union(label, static_cast<uint32_t>(key >> 32), static_cast<uint32_t>(key & 0xffffffffu));
     }
@@ -1361,11 +1376,13 @@ __global__ __launch_bounds__(kThreads) void k_stats(Plan plan) {
     int ty;
     const KFrame* d = (const KFrame*)plan.frames + frame_of(plan, SH_CLASS + (kRowLog2 - kRowExtLog2Min), blockIdx.x, &ty);
     if (!frame_head(d).active) return;
-    if (!kld(d->row_roi + 4 * ty)) {
+    typedef int i4v __attribute__((ext_vector_type(4)));
+    const i4v roi = kld(reinterpret_cast<const i4v*>(d->row_roi + 4 * ty));
+    if (!roi[0]) {
         if (threadIdx.x == 0) d->band_nroots[ty] = 0;  // an inactive band has no runs
         return;
     }
-    const bool local = kld(d->row_roi + 4 * ty + 3) != 0;
+    const bool local = roi[3] != 0;
     const int H = d->height;
     const int y0 = ty * kTile;
     const int nrows = min(kTile, H - y0);
