@@ -2895,13 +2895,19 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(8))
     const int ncodes = min(d0->code_count, kMaxCodes);
     for (int i = threadIdx.x; i < ncodes; i += blockDim.x) s_codes[i] = d0->codes[i];
     __syncthreads();
-    const int wave = static_cast<int>(threadIdx.x) / kWave, lane = static_cast<int>(threadIdx.x) % kWave;
+    // The wave index in a scalar register, so that the item, its frame and
+    // the descriptor fields are scalar too.
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+    const int lane = static_cast<int>(threadIdx.x) % kWave;
     const int slot = wave / kWavesPerCand, wslot = wave % kWavesPerCand;
-    for (uint32_t first = blockIdx.x * kPerBlock; first < total; first += gridDim.x * kPerBlock) {
+    // Item, frame and descriptor stay in scalar registers: the step is read
+    // once into one, and a slot past the list reads the last item (ignored)
+    // rather than selecting a value in a divergent branch.
+    const uint32_t step = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(gridDim.x)) * kPerBlock;
+    for (uint32_t first = blockIdx.x * kPerBlock; first < total; first += step) {
         const uint32_t item = first + static_cast<uint32_t>(slot);
         const bool have = item < total;
-        // Uniform across the wave: scalar registers for the descriptor.
-        const uint32_t code = have ? __builtin_amdgcn_readfirstlane(work_item(plan, item, nhead)) : 0u;
+        const uint32_t code = work_item(plan, have ? item : total - 1u, nhead);
         const KFrame* d = (const KFrame*)plan.frames + (code >> 10);
         const int ci = static_cast<int>(code & 1023u);
         // A candidate without a quad decodes to nothing.
