@@ -2607,7 +2607,7 @@ __device__ inline uint32_t next_item(uint32_t* cursor) {
     __syncthreads();
     if (threadIdx.x == 0) s_next = gridDim.x + atomicAdd(cursor, 1u);
     __syncthreads();
-    return s_next;
+    return __builtin_amdgcn_readfirstlane(s_next);  // uniform: a scalar loop counter
 }
 
 // Entry `item` of the batch candidate list: the head region (each frame's
