@@ -2994,8 +2994,10 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
     }
     const int maxd = d->max_detections;
     const int ncand = static_cast<int>(kld(d->scal + S_NCAND));
-    __shared__ int list[256];
-    __shared__ int k_src[256];      // candidate index of each kept entry
+    // The decoded candidates' records, compacted in candidate order into LDS
+    // as they are loaded (one global round trip for ids and records).
+    __shared__ bd_detection s_rec[256];
+    __shared__ int k_pos[256];      // record position of each kept entry
     __shared__ float k_cx[256], k_cy[256], k_margin[256];
     __shared__ int k_id[256];
     __shared__ int s_kept, s_dup, s_any_shared;
@@ -3009,7 +3011,10 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
     // Compaction in candidate order, chunks of 256.
     for (int c0 = 0; c0 < ncand; c0 += blockDim.x) {
         const int ci = c0 + t;
-        const bool valid = ci < ncand && d->dec[ci].id >= 0;
+        bd_detection rec;
+        rec.id = -1;
+        if (ci < ncand) rec = d->dec[ci];
+        const bool valid = rec.id >= 0;
         // block-wide exclusive scan of `valid`
         __shared__ uint32_t wsum[4];
         const int lane = t % kWave, wave = t / kWave;
@@ -3021,22 +3026,18 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
         for (int w = 0; w < wave; ++w) off += wsum[w];
         if (valid) {
             const uint32_t pos = off + before;
-            if (pos < static_cast<uint32_t>(maxd)) list[pos] = ci;
+            if (pos < static_cast<uint32_t>(maxd)) s_rec[pos] = rec;
         }
         __syncthreads();
         if (t == 0) s_base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
         __syncthreads();
     }
     const int n = min(static_cast<int>(s_base), maxd);
-    // The entries' ids, loaded once (the loops below compare every pair).
-    __shared__ int s_lid[256];
-    for (int i = t; i < n; i += blockDim.x) s_lid[i] = d->dec[list[i]].id;
-    __syncthreads();
     // Any id shared by two entries?  If not, de-duplication keeps every entry.
     for (int i = t; i < n; i += blockDim.x) {
-        const int id = s_lid[i];
+        const int id = s_rec[i].id;
         for (int j = 0; j < n; ++j) {
-            if (j != i && s_lid[j] == id) {
+            if (j != i && s_rec[j].id == id) {
                 s_any_shared = 1;
                 break;
             }
@@ -3045,13 +3046,13 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
     __syncthreads();
     if (!s_any_shared) {
         for (int i = t; i < n; i += blockDim.x) {
-            k_src[i] = list[i];
-            k_id[i] = s_lid[i];
+            k_pos[i] = i;
+            k_id[i] = s_rec[i].id;
         }
         if (t == 0) s_kept = n;
     } else {
         for (int i = 0; i < n; ++i) {
-            const bd_detection& e = d->dec[list[i]];
+            const bd_detection& e = s_rec[i];
             float side, cx, cy;
             centre_side(e.corners, &side, &cx, &cy);
             if (t == 0) s_dup = 0x7fffffff;
@@ -3070,14 +3071,14 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
                     if (e.decision_margin > k_margin[dup]) {
                         float sd, ox, oy;
                         centre_side(e.corners, &sd, &ox, &oy);
-                        k_src[dup] = list[i];
+                        k_pos[dup] = i;
                         k_id[dup] = e.id;
                         k_margin[dup] = e.decision_margin;
                         k_cx[dup] = ox;
                         k_cy[dup] = oy;
                     }
                 } else if (kept < maxd) {
-                    k_src[kept] = list[i];
+                    k_pos[kept] = i;
                     k_id[kept] = e.id;
                     k_margin[kept] = e.decision_margin;
                     k_cx[kept] = cx;
@@ -3099,7 +3100,7 @@ __global__ __launch_bounds__(256) void k_finalize(Plan plan) {
             const int oj = k_id[j];
             rank += (oj < id) || (oj == id && j < i);
         }
-        bd_detection o = d->dec[k_src[i]];
+        bd_detection o = s_rec[k_pos[i]];
         o.reserved = 0;
         fill_homography(o.corners, o.homography);
         d->out[rank] = o;
