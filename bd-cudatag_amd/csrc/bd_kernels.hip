@@ -1740,6 +1740,9 @@ __global__ __launch_bounds__(256) void k_select(Plan plan) {
     const uint32_t n = kld(d->scal + S_CAND_ACC);
     const uint32_t nsel = min(n, static_cast<uint32_t>(d->max_candidates));
 
+    __shared__ uint32_t s_head, s_tail, s_nhead;
+    // All keys fit one per thread: the head count needs no sorted order.
+    const bool early = n <= blockDim.x;
     uint32_t m = n;
     if (n > static_cast<uint32_t>(kSortCap)) {
         if (t == 0) {
@@ -1794,6 +1797,22 @@ __global__ __launch_bounds__(256) void k_select(Plan plan) {
         // and one barrier place them.
         __syncthreads();
         const uint64_t mine = static_cast<uint32_t>(t) < m ? keys[t] : ~0ull;
+        if (early) {
+            // Every key is here, one per thread: the head count (the
+            // selection's keys of at least head_area pixels, i.e. all such
+            // keys up to nsel) and the work-list reservations go out before
+            // the ranking, which hides the atomics' round trip.
+            const bool big = static_cast<uint32_t>(t) < m &&
+                             ~static_cast<uint32_t>(mine >> 32) >= static_cast<uint32_t>(plan.tune.head_area);
+            const uint32_t nbig = min(static_cast<uint32_t>(__syncthreads_count(big)), nsel);
+            const uint32_t nhead = min(min(nsel, static_cast<uint32_t>(kHeadSlots)),
+                                       max(nbig, static_cast<uint32_t>(plan.tune.head_ranks)));
+            if (t == 0) {
+                s_nhead = nhead;
+                s_head = atomicAdd(&plan.work[kWorkHeadCount], nhead);
+                s_tail = atomicAdd(&plan.work[kWorkCount], nsel - nhead);
+            }
+        }
         uint32_t rank = 0;
         for (uint32_t j = 0; j < m; ++j) rank += keys[j] < mine ? 1u : 0u;
         __syncthreads();
@@ -1810,20 +1829,23 @@ __global__ __launch_bounds__(256) void k_select(Plan plan) {
     // that the longest fits of the batch start first: its first head_ranks
     // and every other one of at least head_area pixels (a prefix of the
     // sorted keys, whose high word is the complemented area).
-    __shared__ uint32_t s_head, s_tail;
-    uint32_t nbig = 0;
-    for (uint32_t i0 = 0; i0 < nsel; i0 += blockDim.x) {
-        const uint32_t i = i0 + t;
-        const bool big = i < nsel && ~static_cast<uint32_t>(keys[i] >> 32) >= static_cast<uint32_t>(plan.tune.head_area);
-        nbig += static_cast<uint32_t>(__syncthreads_count(big));
-    }
-    const uint32_t nhead = min(min(nsel, static_cast<uint32_t>(kHeadSlots)),
-                               max(nbig, static_cast<uint32_t>(plan.tune.head_ranks)));
-    if (t == 0) {
-        s_head = atomicAdd(&plan.work[kWorkHeadCount], nhead);
-        s_tail = atomicAdd(&plan.work[kWorkCount], nsel - nhead);
+    if (!early) {
+        uint32_t nbig = 0;
+        for (uint32_t i0 = 0; i0 < nsel; i0 += blockDim.x) {
+            const uint32_t i = i0 + t;
+            const bool big = i < nsel && ~static_cast<uint32_t>(keys[i] >> 32) >= static_cast<uint32_t>(plan.tune.head_area);
+            nbig += static_cast<uint32_t>(__syncthreads_count(big));
+        }
+        const uint32_t nhead = min(min(nsel, static_cast<uint32_t>(kHeadSlots)),
+                                   max(nbig, static_cast<uint32_t>(plan.tune.head_ranks)));
+        if (t == 0) {
+            s_nhead = nhead;
+            s_head = atomicAdd(&plan.work[kWorkHeadCount], nhead);
+            s_tail = atomicAdd(&plan.work[kWorkCount], nsel - nhead);
+        }
     }
     __syncthreads();
+    const uint32_t nhead = s_nhead;
     uint32_t* head = plan.work + kWorkItems;
     uint32_t* tail = head + kHeadSlots * plan.nframes;
     for (uint32_t i = t; i < nsel; i += blockDim.x) {
