@@ -590,6 +590,17 @@ __device__ inline int3 tile_row_roi(const FrameRef& fr, int ty, int lo, int hi, 
 
 // Run words are x0 | x1 << 16; bit 31 marks a band root (see
 // k_binarize_runs).  x0, x1 < 8192.
+// n / d for 0 <= n < 2^20 and 1 <= d < 2^12 from a float reciprocal of d
+// (rd = 1.0f / d): the truncated product is within one of the quotient (its
+// relative error is below 2^-22), and one correction each way fixes it.
+__device__ inline int div_small(int n, int d, float rd) {
+    int q = static_cast<int>(static_cast<float>(n) * rd);
+    const int rem = n - q * d;
+    q += rem >= d ? 1 : 0;
+    q -= rem < 0 ? 1 : 0;
+    return q;
+}
+
 __device__ inline int run_x0(uint32_t r) { return static_cast<int>(r & 0xffffu); }
 __device__ inline int run_x1(uint32_t r) { return static_cast<int>((r >> 16) & 0x7fffu); }
 constexpr uint32_t kRootFlag = 0x80000000u;
@@ -703,9 +714,11 @@ __device__ void binarize_band(const Plan& plan, int cls, const KFrame* d, FrameR
 
     // Pooled 3x3 thresholds for this tile row (k_tile_threshold): the three
     // tile rows are staged in LDS (padded with neutral values), then pooled.
+    const float r_tw2 = 1.0f / static_cast<float>(tw + 2);
     for (int i = t; i < 3 * (tw + 2); i += blockDim.x) {
-        const int dy = i / (tw + 2) - 1;
-        const int xx = i % (tw + 2) - 1;
+        const int qd = div_small(i, tw + 2, r_tw2);
+        const int dy = qd - 1;
+        const int xx = i - qd * (tw + 2) - 1;
         const int yy = ty + dy;
         uint8_t mn = 255, mx = 0;
         if (yy >= 0 && yy < th && xx >= 0 && xx < tw) {
@@ -769,9 +782,10 @@ __device__ void binarize_band(const Plan& plan, int cls, const KFrame* d, FrameR
     // width launch parameter: 64 or 128 px).
     const int wchunks = (w64 + kChunkWords - 1) / kChunkWords;
     const int items = nrows * wchunks;
+    const float r_wchunks = 1.0f / static_cast<float>(wchunks);
     uint32_t frag_acc = 0;
     for (int it = t; it < items; it += blockDim.x) {
-        const int r = it / wchunks;
+        const int r = div_small(it, wchunks, r_wchunks);
         const int y = y0 + r;
 #pragma unroll
         for (int cw = 0; cw < kChunkWords; ++cw) {
