@@ -86,7 +86,10 @@ inline int row_ext_log2_for_width(int width) {
 
 // k_tile_stats: 16x8-pixel blocks per workgroup (one per thread), numbered
 // across each frame's tile rows.
-constexpr int kTileStatBlocks = 256;
+#ifndef BD_TILE_STAT_BLOCKS
+#define BD_TILE_STAT_BLOCKS 256
+#endif
+constexpr int kTileStatBlocks = BD_TILE_STAT_BLOCKS;
 inline int tile_stat_blocks(int tile_w, int tile_h) {
     const int pairs = (tile_w + 1) / 2;
     return (tile_h * pairs + kTileStatBlocks - 1) / kTileStatBlocks;

@@ -261,7 +261,7 @@ __device__ inline void acc_word(uint32_t w, uint32_t& mn, uint32_t& mx, uint32_t
 // of the previous pass, or creation, resets it).  reference
 // gpu_detector.cu:104-166 (k_tile_stats, k_tile_histograms) plus the per-row
 // active extents that k_build_row_roi (:282-357) needs.
-__global__ __launch_bounds__(256) void k_tile_stats(Plan plan) {
+__global__ __launch_bounds__(kTileStatBlocks) void k_tile_stats(Plan plan) {
     // The batch's candidate list starts empty (block 0 exists in every
     // launch, whichever frames are active).
     if (blockIdx.x == 0 && threadIdx.x < kWorkItems) plan.work[threadIdx.x] = 0;
@@ -3325,7 +3325,7 @@ Tune default_tune() {
 hipError_t launch_stage(const Plan& plan, int stage, hipStream_t stream) {
     const int nf = plan.nframes;
     switch (stage) {
-        case 0: hipLaunchKernelGGL(k_tile_stats, dim3(plan.blocks[SH_TSTAT]), dim3(256), 0, stream, plan); break;
+        case 0: hipLaunchKernelGGL(k_tile_stats, dim3(plan.blocks[SH_TSTAT]), dim3(kTileStatBlocks), 0, stream, plan); break;
         case 1:
             // One launch per width class present, with the class's band capacity.
             for (int c = 0; c < kWidthClasses; ++c) {
