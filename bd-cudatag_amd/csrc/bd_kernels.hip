@@ -1123,11 +1123,14 @@ __device__ void binarize_band(const Plan& plan, int cls, const KFrame* d, FrameR
             const uint32_t end_x1 = static_cast<uint32_t>(__shfl(static_cast<int>(x1), end));
             const uint32_t end_incl = static_cast<uint32_t>(__shfl(static_cast<int>(incl), end));
             if (head) {
-                const uint64_t area = end_incl - incl + len;
+                // The item's two words (x0 | end x1 << 13 | (pixels - 1) << 26
+                // | root flag << 39 | band row << 40 | index << 43) built in
+                // 32-bit halves.
+                const uint32_t am1 = end_incl - incl + len - 1u;  // < 2^13
+                const uint32_t lo = x0 | (end_x1 << 13) | (am1 << 26);
+                const uint32_t hi = (am1 >> 6) | ((is_root ? 1u : 0u) << 7) | (static_cast<uint32_t>(rr) << 8) | (ridx << 11);
                 row_items[placed + static_cast<uint32_t>(__popcll(heads & ((1ull << lane) - 1ull)))] =
-                    static_cast<uint64_t>(x0) | (static_cast<uint64_t>(end_x1) << 13) | ((area - 1u) << 26) |
-                    (static_cast<uint64_t>(is_root ? 1u : 0u) << 39) | (static_cast<uint64_t>(rr) << 40) |
-                    (static_cast<uint64_t>(ridx) << 43);
+                    (static_cast<uint64_t>(hi) << 32) | lo;
             }
             placed += static_cast<uint32_t>(__popcll(heads));
         }
