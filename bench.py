@@ -48,8 +48,8 @@ BYTES_PER_PIXEL = 1.0  # algorithmic bytes: the u8 frame read once
 # HBM bytes per input pixel that each kernel moves on the headline workload
 # (64 x 1280x800), from rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by the
 # same counters on a device copy of known size (profiles/traffic_batch64_1280x800.md).
-TRAFFIC_B_PER_PX = {"tile_stats": 1.083, "binarize_runs": 1.85, "merge": 0.614, "stats": 1.186,
-                    "collect": 0.176, "select": 0.031, "fit": 1.326, "decode": 0.411, "finalize": 0.021}
+TRAFFIC_B_PER_PX = {"tile_stats": 1.083, "binarize_runs": 1.85, "merge": 0.612, "stats": 1.184,
+                    "collect": 0.176, "select": 0.031, "fit": 1.325, "decode": 0.41, "finalize": 0.021}
 
 
 def percentile(v, q):
